@@ -1,0 +1,170 @@
+/*
+ * pss.h -- C ABI of the MI355X-native coupled SBE-PSTD step (libpss.so).
+ *
+ * All pointers are device pointers (HIP, gfx950) unless a comment says "host".
+ * Complex arrays are interleaved (re, im) float64 pairs, i.e. the layout of
+ * numpy/torch complex128.  Every call returns 0 on success or a nonzero
+ * status; pss_last_error() gives the message.  No call allocates memory: the
+ * caller owns every buffer (state, tables, workspace).  A descriptor array is
+ * used by one host thread at a time; calls are asynchronous on `stream`.
+ *
+ * Citations are paths relative to the reference's src/pulsesuite.
+ */
+#ifndef PSS_H
+#define PSS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Many-body flags, bit positions follow mb.params order (PSTD3D/SBEs.py:5016-5034). */
+enum {
+  PSS_OPTICS = 1 << 0,
+  PSS_EXCITONS = 1 << 1,
+  PSS_EHS = 1 << 2,
+  PSS_SCREENED = 1 << 3,
+  PSS_PHONON = 1 << 4,
+  PSS_LF = 1 << 6,
+  PSS_DIAGDPH = 1 << 8,
+  PSS_OFFDIAGDPH = 1 << 9,
+  PSS_IGNOREWIRE = 1 << 12,
+  PSS_LORENTZDELTA = 1 << 14,
+};
+
+/* Precision of the contractions. */
+enum { PSS_FP64 = 0, PSS_MIXED = 1 };
+
+/*
+ * One quantum wire (one sub-band index w of an SBESolver) resident on the GPU.
+ * Filled by the host shim; copied once to device memory; the step kernels read
+ * it from there (one descriptor per wire in a batch).
+ */
+typedef struct pss_wire_desc {
+  int32_t Nk, Nr, nrr, w;        /* k points, wire grid points, propagation points, 1-based wire */
+  int32_t flags, precision, pad0, pad1;
+  double dt, L, gap, ehint, area, dcv, dkr, dr_r, drr, yw, zsign;
+  double gam_e, gam_h, gam_eh;
+  double Vscale, N0, hOph, hGph, Gph;
+  double density_max, gate;      /* gate = 1e-3 * Emax0 (wireoff threshold) */
+  double pad2[3];
+  /* tables (read only) */
+  const double *kr, *Ee, *Eh;    /* [Nk] */
+  const double *win;             /* [Nr] QW window on r */
+  const double *r;               /* [Nr] */
+  const double *V0;              /* [3][Nk] unscreened V_eh, V_ee, V_hh by |k1-k2| */
+  const double *chie, *chih, *qe, *qh; /* [Nk] Toeplitz rows of Chi1De/Chi1Dh/qe/qh */
+  const int32_t *ktab;           /* [6][Nk*Nk] k_p_q, k_m_q, k1p, k1p_m_q, k1, k1_m_q (row k, col q) */
+  const double *xe, *xh;         /* [Nk] */
+  const int32_t *iqv;            /* [Nk] Vxx2 index table */
+  const double *twid;            /* [2*Nr] exp(-2 pi i m/Nr), interleaved */
+  /* spline of the propagation grid rr (Prop2QW source) and its evaluation at r */
+  const double *sa_x, *sa_t, *sa_b, *sa_h; /* [nrr] knots, elimination factors, pivots, spacings */
+  const double *sa_end;          /* [8] end-condition constants */
+  const int32_t *sa_idx;         /* [Nr] interval index of r_j (-1 = outside) */
+  const double *sa_dx;           /* [Nr] r_j - rr[idx] */
+  /* spline of the wire grid r (QW2Prop source) and its evaluation at rr */
+  const double *sb_t, *sb_b, *sb_h, *sb_end;
+  const int32_t *sb_idx;         /* [nrr] */
+  const double *sb_dx;           /* [nrr] */
+  /* state (read/write) */
+  double *p[3], *C[3], *D[3];    /* complex [Nk*Nk], rotating prev/cur/next */
+  int32_t *ctl;                  /* [4] step counter of this wire, -, -, - */
+  int32_t *on;                   /* wireoff latch, shared by the wires of one solver */
+  /* per-step input / output (complex [nrr]) */
+  const double *Exx, *Eyy, *Ezz;
+  double *Pxx, *Pyy, *Pzz, *Rho;
+  /* workspace: pss_sbe_workspace_bytes(Nk, Nr, nrr) bytes, 256-byte aligned */
+  double *ws;
+} pss_wire_desc;
+
+/* Bytes of workspace one wire needs. */
+int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr);
+
+/*
+ * One SBE time step of `nwires` wires whose descriptors sit in device memory
+ * at `d_descs`.  Replaces, per wire, SBESolver.QWCalculator minus file output
+ * (PSTD3D/SBEs.py:5838-5997): Prop2QW, _sbe_calculator, QW2Prop.
+ * max_* bound the per-wire sizes in the batch (grid sizing).
+ */
+int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+                 int32_t max_nrr, void* stream);
+
+/*
+ * Coulomb integral V(q) = prefactor * Vint(q) for a list of q (PSTD3D/coulomb.py:70-104,
+ * 502-577), with the reference's series/asymptotic K0 (PSTD3D/coulomb.py:39-66).
+ * y: [Ny] wire grid; q: [nq]; out: [nq].
+ */
+int pss_vint(const double* y, int32_t Ny, const double* q, int32_t nq, double alpha1, double alpha2,
+             double Delta0, double prefactor, double* out, void* stream);
+
+/* ------------------------------------------------------------------ PSTD */
+
+/* 3-D grid (C order: x slowest, z fastest), complex128 fields. */
+typedef struct pss_grid {
+  int32_t Nx, Ny, Nz, npml_x, npml_y, npml_z, pad0, pad1;
+  double dt, v2, mu0;
+  const double *kx, *ky, *kz;            /* [N] FFT-order wavenumbers */
+  const double *bx, *cx, *ikx;           /* [Nx] CPML b, c, 1/kappa - 1 */
+  const double *by, *cy, *iky;
+  const double *bz, *cz, *ikz;
+  const double *twx, *twy, *twz;         /* [N] FFT twiddles exp(-2 pi i m/N), interleaved, N/2 used */
+} pss_grid;
+
+/*
+ * E += i v2 dt (k x B) - mu0 v2 dt J  (PSTD3D/PSTD3D.py:29-76).  J is either a dense k-space
+ * field triple (Jx,Jy,Jz non-null) or separable: J_c(k) = jline_c(kx) gy(ky) gz(kz) given by
+ * jline [3][Nx], gy [Ny], gz [Nz] (complex), as deposited by the wire coupling.
+ */
+int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
+                   const double* Jx, const double* Jy, const double* Jz, const double* jline,
+                   const double* gy, const double* gz, double* Ex, double* Ey, double* Ez, void* stream);
+/* B -= i dt (k x E)  (PSTD3D/PSTD3D.py:79-113). */
+int pss_update_b3d(const pss_grid* g, const double* Ex, const double* Ey, const double* Ez,
+                   double* Bx, double* By, double* Bz, void* stream);
+/* In-place 3-D FFT, numpy convention: forward unnormalised, inverse scaled 1/N
+ * (core/fftw.py:217-226).  dir = -1 forward, +1 inverse. */
+int pss_fft3d(const pss_grid* g, double* F, int32_t dir, void* stream);
+
+/*
+ * One CPML half step on fields F (k-space in, k-space out) driven by G:
+ * which = 0: F = E, G = B (ApplyCPML_E, PSTD3D/cpml.py:640-702, scale v2*dt, sign +1);
+ * which = 1: F = B, G = E (ApplyCPML_B, PSTD3D/cpml.py:575-638, scale dt, sign -1).
+ * psi: 6 real arrays [Nx*Ny*Nz] (order xy, xz, yx, yz, zx, zy).  scratch: 6 complex [N].
+ * src_comp >= 0 fuses UpdateTFSC (PSTD3D/tfsf.py:63-142) on that component: the
+ * additive S(x) E_inc(x,y,z,t) dx is applied in real space before the forward FFT;
+ * src_par (host, 9 doubles) = {t, amp, omega0, tau_G, chirp, v, Tp, 1/w0^2, dx};
+ * srcx: [Nx] profile; rx/ry/rz: grid coordinates.
+ * sample_idx (may be null): 8 flat cell indices (-1 = unused) and sample_w 8 weights
+ * of a trilinear sample (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085) of the corrected
+ * real-space F, written to sample_out (complex [3]) -- the wire's point field.
+ */
+int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz,
+                  const double* Gx, const double* Gy, const double* Gz, double* psi, double* scratch,
+                  int32_t src_comp, const double* src_par, const double* srcx, const double* rx,
+                  const double* ry, const double* rz, const int32_t* sample_idx, const double* sample_w,
+                  double* sample_out, void* stream);
+
+/* One FFT pass along `axis` of the grid's shape (the other axes batch), in -> out
+ * (may alias).  Used for 1-D transforms of the separable current factors. */
+int pss_fft_line(const pss_grid* g, const double* in, double* out, int32_t axis, int32_t dir, void* stream);
+
+/* Fill three uniform wire lines [3][n] with the point sample (complex [3]): the
+ * wire sees E at one point (QWArray.QuantumWire, PSTD3D/rhoPJ.py:1160-1164). */
+int pss_fill_line(const double* sample, double* lines, int32_t n, void* stream);
+
+/* Current line of the wires (PSTD3D/rhoPJ.py:1183-1194 with CalcJfx :561-592):
+ * J_c = sum_w (P_cw - Pold_cw)/dt/Nw, J_x += CalcJfx(sum Rho, sum RhoOld, dt, dx)/Nw;
+ * then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx], jline: [3][Nx]. */
+int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
+              double dt, double dx, double* jline, void* stream);
+
+/* Message of the last failing call (host string, valid until the next call). */
+const char* pss_last_error(void);
+/* Library version string (host). */
+const char* pss_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

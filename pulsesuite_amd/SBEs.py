@@ -1,0 +1,358 @@
+"""Quantum-wire SBE solver on MI355X: drop-in for PulseSuite's ``SBEs`` module.
+
+Same call surface as PSTD3D/SBEs.py: ``SBESolver(q, rr, r0, Emaxxx, lam, Nw, QW,
+backend)`` (:4635), the module shims ``InitializeSBE`` / ``QWCalculator`` /
+``chiqw`` / ``getqc`` / ``QWArea`` / ``ShutOffOptics`` (:6020-6078), and
+``QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP,
+DoQWDl)`` with 1-based ``w`` and in-place outputs.  The wire state lives on the
+GPU; every step runs in libpss (there is no host fallback).  File output of the
+reference (dataQW/*.dat, PdotE) is not produced.
+
+Extra, GPU-native entry points: ``step_all`` advances all Nw wires in one
+batched launch from device-resident field lines; ``get_state`` / ``set_state``
+read and write (p1, p2, C1, C2, D1, D2) for checkpointing and tests.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+from .wire_model import MBFlags, QWParams, build_model, read_mb_params, read_qw_params
+
+__all__ = ["SBESolver", "InitializeSBE", "QWCalculator", "chiqw", "getqc", "QWArea", "ShutOffOptics",
+           "DeviceWires", "QWParams", "MBFlags"]
+
+_CDT = torch.complex128
+
+
+def _dev(device):
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+class DeviceWires:
+    """Device-resident state, tables, workspace and descriptors of Nw wires sharing one model.
+
+    ``models`` may be one WireModel (the SBESolver case, every wire identical up
+    to its index w) or a list with one model per wire (batched wires of
+    different Nk).  ``nrr`` is the propagation-grid length of every wire.
+    """
+
+    def __init__(self, models, Nw, Emax0, dt=0.0, precision="fp64", device=None, w_index=None, groups=None):
+        self.device = _dev(device)
+        if not isinstance(models, (list, tuple)):
+            models = [models] * Nw
+        if len(models) != Nw:
+            raise ValueError("need one model per wire")
+        self.models = list(models)
+        self.Nw = Nw
+        self.precision = precision
+        self.prec_code = _lib.FP64 if precision == "fp64" else _lib.MIXED
+        self.dt = float(dt)
+        dev = self.device
+        self._keep = []
+        self._tables = {}
+        self.state = []
+        self.ctl = torch.zeros((Nw, 4), dtype=torch.int32, device=dev)
+        # wireoff latch: one per solver group (group id per wire; default all wires share one)
+        self.group = list(groups) if groups is not None else [0] * Nw
+        self.on = torch.zeros(max(self.group) + 1, dtype=torch.int32, device=dev)
+        nrr = [len(m.arrays["sa_x"]) for m in self.models]
+        self.nrr = nrr
+        self.E_in = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
+        self.P_out = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
+        self.Rho_out = [torch.zeros(n, dtype=_CDT, device=dev) for n in nrr]
+        self.ws = []
+        self.descs = (_lib.WireDesc * Nw)()
+        w_index = w_index or list(range(1, Nw + 1))
+        self.w_index = w_index
+        self.gate = 1e-3 * Emax0
+        for i, m in enumerate(self.models):
+            self._setup_wire(i, m, w_index[i])
+        self.d_descs = torch.empty(C.sizeof(self.descs), dtype=torch.uint8, device=dev)
+        self.upload_descs()
+        self.max_Nk = max(m.Nk for m in self.models)
+        self.max_Nr = max(m.Nr for m in self.models)
+        self.max_nrr = max(nrr)
+
+    # ------------------------------------------------------------ set-up
+    def _table(self, m, name):
+        key = (id(m), name)
+        if key not in self._tables:
+            a = m.arrays[name]
+            if np.iscomplexobj(a):
+                t = torch.as_tensor(np.ascontiguousarray(a), dtype=_CDT).view(torch.float64)
+            elif a.dtype.kind in "iu":
+                t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32))
+            else:
+                t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64))
+            self._tables[key] = t.to(self.device)
+        return self._tables[key]
+
+    def _v0(self, m):
+        key = (id(m), "V0")
+        if key not in self._tables:
+            s = m.scalars
+            y = self._table(m, "r")
+            q = torch.as_tensor(m.arrays["Vq"], dtype=torch.float64, device=self.device)
+            V0 = torch.empty((3, m.Nk), dtype=torch.float64, device=self.device)
+            st = _lib.stream_ptr()
+            for n, (a1, a2) in enumerate(((s["ae"], s["ah"]), (s["ae"], s["ae"]), (s["ah"], s["ah"]))):
+                _lib.check(_lib.lib.pss_vint(_lib.ptr(y), m.Nr, _lib.ptr(q), m.Nk, a1, a2, s["Delta0"], s["vpref"],
+                                             _lib.ptr(V0[n]), st), "pss_vint")
+            self._tables[key] = V0
+        return self._tables[key]
+
+    def _setup_wire(self, i, m, w):
+        dev = self.device
+        Nk, Nr = m.Nk, m.Nr
+        nrr = self.nrr[i]
+        s = m.scalars
+        st = {}
+        for name in ("p", "C", "D"):
+            st[name] = torch.zeros((3, Nk, Nk), dtype=_CDT, device=dev)
+        fe = torch.as_tensor(m.arrays["fe0"], dtype=_CDT, device=dev)
+        fh = torch.as_tensor(m.arrays["fh0"], dtype=_CDT, device=dev)
+        for b in range(3):
+            st["C"][b].diagonal().copy_(fe)
+            st["D"][b].diagonal().copy_(fh)
+        self.state.append(st)
+        wsb = _lib.lib.pss_sbe_workspace_bytes(Nk, Nr, nrr)
+        ws = torch.zeros(wsb // 8, dtype=torch.float64, device=dev)
+        self.ws.append(ws)
+        d = self.descs[i]
+        d.Nk, d.Nr, d.nrr, d.w = Nk, Nr, nrr, w
+        d.flags, d.precision = m.flags, self.prec_code
+        d.dt = self.dt
+        for k in ("L", "gap", "ehint", "area", "dcv", "dkr", "dr_r", "drr", "gam_e", "gam_h", "gam_eh", "Vscale",
+                  "N0", "hOph", "hGph", "Gph", "density_max"):
+            setattr(d, k, float(s[k]))
+        d.yw = float((-1) ** int(np.floor((w - 1) / 2.0)))
+        d.zsign = float((-1) ** w)
+        d.gate = self.gate
+        for k in ("kr", "Ee", "Eh", "win", "r", "chie", "chih", "qe", "qh", "ktab", "xe", "xh", "iqv", "twid",
+                  "sa_x", "sa_t", "sa_b", "sa_h", "sa_end", "sa_idx", "sa_dx", "sb_t", "sb_b", "sb_h", "sb_end",
+                  "sb_idx", "sb_dx"):
+            setattr(d, k, self._table(m, k).data_ptr())
+        d.V0 = self._v0(m).data_ptr()
+        for b in range(3):
+            d.p[b] = st["p"][b].data_ptr()
+            d.C[b] = st["C"][b].data_ptr()
+            d.D[b] = st["D"][b].data_ptr()
+        d.ctl = self.ctl[i].data_ptr()
+        d.on = self.on[self.group[i]:self.group[i] + 1].data_ptr()
+        d.Exx, d.Eyy, d.Ezz = (self.E_in[i][c].data_ptr() for c in range(3))
+        d.Pxx, d.Pyy, d.Pzz = (self.P_out[i][c].data_ptr() for c in range(3))
+        d.Rho = self.Rho_out[i].data_ptr()
+        d.ws = ws.data_ptr()
+
+    def upload_descs(self):
+        raw = bytes(memoryview(self.descs).cast("B"))
+        host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
+        self.d_descs.copy_(host)
+
+    def set_dt(self, dt):
+        dt = float(dt)
+        if dt != self.dt:
+            self.dt = dt
+            for i in range(self.Nw):
+                self.descs[i].dt = dt
+            self.upload_descs()
+
+    # ------------------------------------------------------------ stepping
+    def step(self, wires=None, stream=None):
+        """One step of the wires in the contiguous range ``wires`` (default: all)."""
+        lo, hi = (0, self.Nw) if wires is None else wires
+        sz = C.sizeof(_lib.WireDesc)
+        ptr = C.c_void_p(self.d_descs.data_ptr() + lo * sz)
+        sub = self.models[lo:hi]
+        _lib.check(_lib.lib.pss_sbe_step(ptr, hi - lo, max(m.Nk for m in sub), max(m.Nr for m in sub),
+                                         max(self.nrr[lo:hi]), _lib.stream_ptr(stream)), "pss_sbe_step")
+
+    # ------------------------------------------------------------ state access
+    def _cur(self, i):
+        return 1 + (int(self.ctl[i, 0].item()) & 1)
+
+    def get_state(self, i):
+        st = self.state[i]
+        c = self._cur(i)
+        return {k: st[n][b].cpu().numpy().copy() for k, n, b in
+                (("p1", "p", 0), ("p2", "p", c), ("C1", "C", 0), ("C2", "C", c), ("D1", "D", 0), ("D2", "D", c))}
+
+    def set_state(self, i, p1, p2, C1, C2, D1, D2):
+        st = self.state[i]
+        c = self._cur(i)
+        for n, b, a in (("p", 0, p1), ("p", c, p2), ("C", 0, C1), ("C", c, C2), ("D", 0, D1), ("D", c, D2)):
+            st[n][b].copy_(torch.as_tensor(np.ascontiguousarray(a), dtype=_CDT))
+
+    def force_on(self):
+        self.on.fill_(1)
+
+    def is_on(self, i):
+        return bool(self.on[self.group[i]].item())
+
+    def workspace_view(self, i, name):
+        """Read-only view of a named workspace segment (diagnostics and tests)."""
+        m = self.models[i]
+        Nk, Nr, nrr = m.Nk, m.Nr, self.nrr[i]
+        lay = ws_layout(Nk, Nr, nrr)
+        sizes = {"vc": 3 * Nk, "e1d": Nk, "gam": 2 * Nk, "De": (2 * Nk + 1) * Nk, "Dh": (2 * Nk + 1) * Nk,
+                 "H": 3 * Nk * Nk * 2, "W": 4 * Nk, "n3": 2 * Nk, "eq": 3 * (2 * Nk - 1) * 2, "pr": 3 * Nr,
+                 "rho": 2 * Nr * 2, "er": 3 * Nr}
+        o = lay[name]
+        return self.ws[i][o:o + sizes[name]]
+
+
+def ws_layout(Nk, Nr, nrr):
+    """Offsets (in doubles) of the workspace segments; mirrors ws_layout in csrc/sbe.hip."""
+    def al(n):
+        return (n + 31) // 32 * 32
+    nd = 2 * Nk - 1
+    segs = [("er", 3 * Nr), ("spa", 3 * nrr), ("eq", 3 * nd * 2), ("vc", 3 * Nk), ("e1d", Nk), ("ne2", 2 * Nk),
+            ("nh2", 2 * Nk), ("tr", 4 * nd * 2), ("gam", 2 * Nk), ("De", (2 * Nk + 1) * Nk),
+            ("Dh", (2 * Nk + 1) * Nk), ("H", 3 * Nk * Nk * 2), ("W", 4 * Nk), ("n3", 4 * Nk), ("red", 32),
+            ("ds", 4 * nd * 2), ("pr", 3 * Nr), ("rho", 2 * Nr * 2), ("spb", 5 * Nr), ("outr", 5 * nrr)]
+    out = {}
+    o = 0
+    for n, sz in segs:
+        out[n] = o
+        o += al(sz)
+    out["total"] = o
+    return out
+
+
+class SBESolver:
+    """Class-based SBE solver, GPU resident (PSTD3D/SBEs.py:4631-6013).
+
+    Keyword-only extras: ``qw_params`` / ``mb_flags`` replace the files
+    params/qw.params and params/mb.params read from the CWD; ``precision``
+    selects "fp64" (default) or "mixed" contractions; ``device`` the GPU.
+    """
+
+    def __init__(self, q, rr, r0, Emaxxx, lam, Nw, QW, backend="auto", *, qw_params=None, mb_flags=None,
+                 precision="fp64", device=None):
+        self._backend = backend
+        self.Nw = Nw
+        self.QW = bool(QW)
+        if not QW:
+            return
+        qw = qw_params if qw_params is not None else read_qw_params("params/qw.params")
+        mb = mb_flags if mb_flags is not None else read_mb_params("params/mb.params")
+        self.model = build_model(qw, mb, rr, r0)
+        self.mb = self.model.mb
+        s = self.model.scalars
+        self.Nk, self.Nr = self.model.Nk, self.model.Nr
+        self.L, self.area, self.qc, self.dcv, self.gap = s["L"], s["area"], s["qc"], s["dcv"], s["gap"]
+        self.kr = self.model.arrays["kr"]
+        self.r = self.model.arrays["r"]
+        self.Emax0 = Emaxxx
+        self.lam = lam
+        self.rr = np.asarray(rr, dtype=np.float64)
+        self.dev = DeviceWires(self.model, Nw, Emaxxx, 0.0, precision, device)
+        self.t = 0.0
+        self.xxx = 1
+        self.jjj = 1
+        self.jmax = qw.jmax
+        self._on_host = [False] * Nw
+
+    # ---------------------------------------------------------- public step
+    def QWCalculator(self, Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl):
+        """PSTD3D/SBEs.py:5838-5997.  Arrays may be numpy or CUDA tensors; outputs in place."""
+        if w == 1:  # PSTD3D/SBEs.py:5849-5856
+            if self.jjj == self.jmax:
+                self.jjj = 0
+            self.xxx += 1
+            self.jjj += 1
+            self.t += dt
+        i = w - 1
+        dev = self.dev
+        dev.set_dt(dt)
+        for c, F in enumerate((Exx, Eyy, Ezz)):
+            dev.E_in[i][c].copy_(torch.as_tensor(F).to(dev.device, _CDT), non_blocking=True)
+        dev.step((i, i + 1))
+        for c, F in enumerate((Pxx, Pyy, Pzz)):
+            _copy_out(F, dev.P_out[i][c])
+        _copy_out(Rho, dev.Rho_out[i])
+        if not self._on_host[i]:
+            self._on_host[i] = dev.is_on(i)
+        if self._on_host[i]:
+            DoQWP[0] = self.mb.Optics
+            DoQWDl[0] = self.mb.LF
+
+    def step_all(self, dt, stream=None):
+        """Advance all Nw wires from the device lines ``dev.E_in`` (one launch)."""
+        self.t += dt
+        self.xxx += 1
+        self.dev.set_dt(dt)
+        self.dev.step(None, stream)
+
+    # ---------------------------------------------------------- accessors
+    def chiqw(self):
+        from .wire_model import e0, eps0, hbar
+        ww = 2 * np.pi * 2.99792458e8 / self.lam
+        Ee = self.model.arrays["Ee"] + self.gap
+        Eh = self.model.arrays["Eh"]
+        geh = self.model.scalars["gam_eh"]
+        den1 = Ee + Eh - 1j * hbar * geh - hbar * ww
+        den2 = Ee + Eh + 1j * hbar * geh + hbar * ww
+        return 4 * self.dcv**2 / eps0 / self.area * self.model.scalars["dkr"] / (2 * np.pi) * np.sum(
+            (Ee + Eh) / (den1 * den2))
+
+    def getqc(self):
+        return self.qc
+
+    def QWArea(self):
+        return self.area
+
+    def ShutOffOptics(self):
+        self.mb.Optics = False
+        for i in range(self.Nw):
+            self.dev.descs[i].flags &= ~_lib.OPTICS
+        self.dev.upload_descs()
+
+    def get_state(self, w):
+        return self.dev.get_state(w - 1)
+
+    def set_state(self, w, **kw):
+        self.dev.set_state(w - 1, **kw)
+
+
+def _copy_out(dst, src):
+    if isinstance(dst, torch.Tensor):
+        dst.copy_(src.to(dst.device, dst.dtype))
+    else:
+        dst[:] = src.cpu().numpy()
+
+
+_default_solver = None
+
+
+def InitializeSBE(q, rr, r0, Emaxxx, lam, Nw, QW, **kw):
+    """Module shim (PSTD3D/SBEs.py:6023-6026)."""
+    global _default_solver
+    _default_solver = SBESolver(q, rr, r0, Emaxxx, lam, Nw, QW, **kw)
+
+
+def QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl):
+    """Module shim (PSTD3D/SBEs.py:6029-6048)."""
+    _default_solver.QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl)
+
+
+def chiqw():
+    return _default_solver.chiqw()
+
+
+def getqc():
+    return _default_solver.getqc()
+
+
+def QWArea():
+    return _default_solver.QWArea()
+
+
+def ShutOffOptics():
+    _default_solver.ShutOffOptics()

@@ -1,0 +1,109 @@
+"""ctypes binding of libpss.so (the C ABI declared in include/pss.h).
+
+The library is the only compute path: importing this module on a machine
+without the built library raises immediately, and every call checks its
+status and raises RuntimeError with the library's message.
+"""
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpss.so")
+
+# mb flag bits (pss.h)
+OPTICS = 1 << 0
+EXCITONS = 1 << 1
+EHS = 1 << 2
+SCREENED = 1 << 3
+PHONON = 1 << 4
+LF = 1 << 6
+DIAGDPH = 1 << 8
+OFFDIAGDPH = 1 << 9
+IGNOREWIRE = 1 << 12
+LORENTZDELTA = 1 << 14
+
+FP64 = 0
+MIXED = 1
+
+_vp = C.c_void_p
+_d = C.c_double
+_i = C.c_int32
+
+
+class WireDesc(C.Structure):
+    _fields_ = (
+        [("Nk", _i), ("Nr", _i), ("nrr", _i), ("w", _i), ("flags", _i), ("precision", _i), ("pad0", _i), ("pad1", _i)]
+        + [(n, _d) for n in ("dt", "L", "gap", "ehint", "area", "dcv", "dkr", "dr_r", "drr", "yw", "zsign",
+                             "gam_e", "gam_h", "gam_eh", "Vscale", "N0", "hOph", "hGph", "Gph",
+                             "density_max", "gate")]
+        + [("pad2", _d * 3)]
+        + [(n, _vp) for n in ("kr", "Ee", "Eh", "win", "r", "V0", "chie", "chih", "qe", "qh", "ktab", "xe", "xh",
+                              "iqv", "twid", "sa_x", "sa_t", "sa_b", "sa_h", "sa_end", "sa_idx", "sa_dx",
+                              "sb_t", "sb_b", "sb_h", "sb_end", "sb_idx", "sb_dx")]
+        + [("p", _vp * 3), ("C", _vp * 3), ("D", _vp * 3), ("ctl", _vp), ("on", _vp)]
+        + [(n, _vp) for n in ("Exx", "Eyy", "Ezz", "Pxx", "Pyy", "Pzz", "Rho", "ws")]
+    )
+
+
+class Grid(C.Structure):
+    _fields_ = (
+        [(n, _i) for n in ("Nx", "Ny", "Nz", "npml_x", "npml_y", "npml_z", "pad0", "pad1")]
+        + [("dt", _d), ("v2", _d), ("mu0", _d)]
+        + [(n, _vp) for n in ("kx", "ky", "kz", "bx", "cx", "ikx", "by", "cy", "iky", "bz", "cz", "ikz",
+                              "twx", "twy", "twz")]
+    )
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m pulsesuite_amd._build` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "pss_sbe_workspace_bytes": (C.c_int64, [_i, _i, _i]),
+        "pss_sbe_step": (C.c_int, [_vp, _i, _i, _i, _i, _vp]),
+        "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
+        "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
+        "pss_update_b3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 6 + [_vp]),
+        "pss_fft3d": (C.c_int, [C.POINTER(Grid), _vp, _i, _vp]),
+        "pss_fft_line": (C.c_int, [C.POINTER(Grid), _vp, _vp, _i, _i, _vp]),
+        "pss_cpml_half": (C.c_int, [C.POINTER(Grid), _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
+                                    C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
+        "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
+        "pss_last_error": (C.c_char_p, []),
+        "pss_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
+           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_last_error",
+           "pss_version")
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib.pss_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)

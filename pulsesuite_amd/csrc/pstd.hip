@@ -1,0 +1,429 @@
+// PSTD Maxwell step on MI355X (gfx950): spectral curl updates, LDS-staged
+// radix-2 FFT passes, CPML recursion with the soft source fused in, and the
+// wire<->grid coupling kernels (point sample, separable current).
+//
+// Fields are complex128, C order (x slowest, z fastest), k-space between
+// half steps exactly as PSTD3D/PSTD3D.py keeps them.
+#include "common.h"
+#include "../../include/pss.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------- spectral curl updates
+// PSTD3D/PSTD3D.py:29-76
+__global__ void __launch_bounds__(256) k_update_e(pss_grid g, const cplx* __restrict__ Bx, const cplx* __restrict__ By,
+                                                  const cplx* __restrict__ Bz, const cplx* __restrict__ Jx,
+                                                  const cplx* __restrict__ Jy, const cplx* __restrict__ Jz,
+                                                  const cplx* __restrict__ jl, const cplx* __restrict__ gy,
+                                                  const cplx* __restrict__ gz, cplx* Ex, cplx* Ey, cplx* Ez) {
+  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int iz = (int)(o % g.Nz);
+  const int iy = (int)((o / g.Nz) % g.Ny);
+  const int ix = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
+  const double coeff = g.v2 * g.dt;
+  const cplx bx = Bx[o], by = By[o], bz = Bz[o];
+  cplx jx = mk(0, 0), jy = mk(0, 0), jz = mk(0, 0);
+  if (Jx) {
+    jx = Jx[o]; jy = Jy[o]; jz = Jz[o];
+  } else if (jl) {
+    const cplx gg = gy[iy] * gz[iz];
+    jx = jl[ix] * gg;
+    jy = jl[g.Nx + ix] * gg;
+    jz = jl[2 * g.Nx + ix] * gg;
+  }
+  const cplx cx = qy * bz - qz * by, cy = qz * bx - qx * bz, cz = qx * by - qy * bx;
+  // i * c * coeff - mu0 * J * coeff
+  Ex[o] = Ex[o] + (mk(-cx.y, cx.x) * coeff - (g.mu0 * jx) * coeff);
+  Ey[o] = Ey[o] + (mk(-cy.y, cy.x) * coeff - (g.mu0 * jy) * coeff);
+  Ez[o] = Ez[o] + (mk(-cz.y, cz.x) * coeff - (g.mu0 * jz) * coeff);
+}
+
+// PSTD3D/PSTD3D.py:79-113
+__global__ void __launch_bounds__(256) k_update_b(pss_grid g, const cplx* __restrict__ Ex, const cplx* __restrict__ Ey,
+                                                  const cplx* __restrict__ Ez, cplx* Bx, cplx* By, cplx* Bz) {
+  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int iz = (int)(o % g.Nz);
+  const int iy = (int)((o / g.Nz) % g.Ny);
+  const int ix = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
+  const cplx ex = Ex[o], ey = Ey[o], ez = Ez[o];
+  const cplx cx = qy * ez - qz * ey, cy = qz * ex - qx * ez, cz = qx * ey - qy * ex;
+  Bx[o] = Bx[o] - mk(-cx.y, cx.x) * g.dt;
+  By[o] = By[o] - mk(-cy.y, cy.x) * g.dt;
+  Bz[o] = Bz[o] - mk(-cz.y, cz.x) * g.dt;
+}
+
+// ------------------------------------------------------------ FFT pass
+// One radix-2 DIT transform of length N along `axis` for TB lines per block,
+// staged in LDS (32 KB of data), loads/stores coalesced along z.  Optional
+// pre-multiply by i*k_m along axis `mul_axis` (spectral derivative), scale on
+// store (1/N per pass for the inverse).  In place is allowed (each block owns
+// its lines).
+struct PassArgs {
+  const cplx* in;
+  cplx* out;
+  const double* tw;     // exp(-2 pi i m/N), m < N/2, interleaved
+  const double* kmul;   // wavenumbers along mul_axis (null = none)
+  int Nx, Ny, Nz, axis, mul_axis, dir;  // dir -1 forward, +1 inverse
+  double scale;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
+  constexpr int TB = (2048 / N) > 0 ? (2048 / N) : 1;
+  constexpr int LOG = __builtin_ctz(N);
+  __shared__ cplx buf[TB * N];
+  __shared__ cplx tws[N / 2 > 0 ? N / 2 : 1];
+  const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
+  // line geometry
+  int64_t base0, sa, st;
+  int nvalid;
+  int cx = 0, cy = 0, cz0 = 0;  // coordinates of line t=0 (the pass axis coordinate is 0)
+  if (a.axis == 2) {
+    const int64_t nlines = (int64_t)Nx * Ny;
+    const int64_t l0 = (int64_t)blockIdx.x * TB;
+    if (l0 >= nlines) return;
+    nvalid = (int)min((int64_t)TB, nlines - l0);
+    base0 = l0 * N;
+    sa = 1;
+    st = N;
+    cx = (int)(l0 / Ny);
+    cy = (int)(l0 % Ny);
+  } else {
+    const int nch = (Nz + TB - 1) / TB;
+    const int other = (int)(blockIdx.x / nch);
+    cz0 = (int)(blockIdx.x % nch) * TB;
+    if (a.axis == 1) {
+      if (other >= Nx) return;
+      cx = other;
+      base0 = (int64_t)other * Ny * Nz + cz0;
+      sa = Nz;
+    } else {
+      if (other >= Ny) return;
+      cy = other;
+      base0 = (int64_t)other * Nz + cz0;
+      sa = (int64_t)Ny * Nz;
+    }
+    st = 1;
+    nvalid = min(TB, Nz - cz0);
+  }
+  for (int m = threadIdx.x; m < N / 2; m += 256) {
+    const double wr = a.tw[2 * m], wi = a.tw[2 * m + 1];
+    tws[m] = mk(wr, a.dir > 0 ? -wi : wi);
+  }
+  // load (bit-reversed within each line)
+  for (int e = threadIdx.x; e < TB * N; e += 256) {
+    int t, i;
+    if (a.axis == 2) { t = e / N; i = e % N; } else { i = e / TB; t = e % TB; }
+    cplx v = mk(0, 0);
+    if (t < nvalid) {
+      v = a.in[base0 + (int64_t)t * st + (int64_t)i * sa];
+      if (a.kmul) {
+        int coord;
+        if (a.mul_axis == a.axis) coord = i;
+        else if (a.axis == 2) {
+          // lines (x, y) consecutive in y
+          const int64_t l = (int64_t)cx * Ny + cy + t;
+          coord = a.mul_axis == 0 ? (int)(l / Ny) : (int)(l % Ny);
+        } else if (a.mul_axis == 2) coord = cz0 + t;
+        else coord = a.mul_axis == 0 ? cx : cy;
+        const double k = a.kmul[coord];
+        v = mk(-k * v.y, k * v.x);  // i k v
+      }
+    }
+    const int r = __builtin_bitreverse32((unsigned)i) >> (32 - LOG);
+    buf[t * N + r] = v;
+  }
+  __syncthreads();
+  // radix-2 stages
+#pragma unroll 1
+  for (int s = 0; s < LOG; ++s) {
+    const int half = 1 << s;
+    for (int b = threadIdx.x; b < TB * (N / 2); b += 256) {
+      const int t = b / (N / 2), bb = b % (N / 2);
+      const int grp = bb >> s, j = bb & (half - 1);
+      const int i0 = (grp << (s + 1)) + j, i1 = i0 + half;
+      const cplx w = tws[j << (LOG - 1 - s)];
+      const cplx x0 = buf[t * N + i0], x1 = buf[t * N + i1] * w;
+      buf[t * N + i0] = x0 + x1;
+      buf[t * N + i1] = x0 - x1;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < TB * N; e += 256) {
+    int t, i;
+    if (a.axis == 2) { t = e / N; i = e % N; } else { i = e / TB; t = e % TB; }
+    if (t < nvalid) a.out[base0 + (int64_t)t * st + (int64_t)i * sa] = buf[t * N + i] * a.scale;
+  }
+}
+
+int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
+                hipStream_t s) {
+  const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
+  if (N == 1) {
+    if (in != out) return -1;
+    return 0;
+  }
+  PassArgs a;
+  a.in = in;
+  a.out = out;
+  a.tw = axis == 0 ? g->twx : axis == 1 ? g->twy : g->twz;
+  a.kmul = kmul;
+  a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+  a.axis = axis;
+  a.mul_axis = mul_axis;
+  a.dir = dir;
+  a.scale = dir > 0 ? 1.0 / N : 1.0;
+  const int TB = (2048 / N) > 0 ? (2048 / N) : 1;
+  int64_t blocks;
+  if (axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
+  else blocks = (int64_t)(axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
+  dim3 grid((unsigned)blocks), blk(256);
+  switch (N) {
+#define PSS_CASE(n) case n: hipLaunchKernelGGL(k_fft_pass<n>, grid, blk, 0, s, a); break;
+    PSS_CASE(2) PSS_CASE(4) PSS_CASE(8) PSS_CASE(16) PSS_CASE(32) PSS_CASE(64) PSS_CASE(128) PSS_CASE(256)
+    PSS_CASE(512) PSS_CASE(1024) PSS_CASE(2048)
+#undef PSS_CASE
+    default: return -2;
+  }
+  return 0;
+}
+
+// Full 3-D transform from `in` to `out` (may alias), optional i*k premultiply.
+int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* kmul, int mul_axis, hipStream_t s) {
+  const cplx* src = in;
+  bool first = true;
+  for (int axis = 0; axis < 3; ++axis) {
+    const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
+    if (N == 1) continue;
+    int rc = launch_pass(g, src, out, axis, dir, first ? kmul : nullptr, mul_axis, s);
+    if (rc) return rc;
+    first = false;
+    src = out;
+  }
+  if (first && in != out) return -1;  // all axes of length 1: nothing to do in place only
+  return 0;
+}
+
+// ------------------------------------------------------------ CPML
+// Real-space correction of PSTD3D/cpml.py:150-334 (psi only in PML cells),
+// soft-source add of PSTD3D/tfsf.py:106-136 on component src_comp.
+struct SrcPar {
+  double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
+};
+
+__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* Fx, cplx* Fy, cplx* Fz,
+                                              const cplx* __restrict__ d0, const cplx* __restrict__ d1,
+                                              const cplx* __restrict__ d2, const cplx* __restrict__ d3,
+                                              const cplx* __restrict__ d4, const cplx* __restrict__ d5, double* psi,
+                                              int src_comp, SrcPar sp, const double* __restrict__ srcx,
+                                              const double* __restrict__ rx, const double* __restrict__ ry,
+                                              const double* __restrict__ rz) {
+  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int k = (int)(o % g.Nz);
+  const int j = (int)((o / g.Nz) % g.Ny);
+  const int i = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const bool inx = i < g.npml_x || i >= g.Nx - g.npml_x;
+  const bool iny = j < g.npml_y || j >= g.Ny - g.npml_y;
+  const bool inz = k < g.npml_z || k >= g.Nz - g.npml_z;
+  if (inx || iny || inz) {
+    // d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
+    const double gzy = d0[o].x, gyz = d1[o].x, gxz = d2[o].x, gzx = d3[o].x, gyx = d4[o].x, gxy = d5[o].x;
+    double* pxy = psi;
+    double* pxz = psi + n;
+    double* pyx = psi + 2 * n;
+    double* pyz = psi + 3 * n;
+    double* pzx = psi + 4 * n;
+    double* pzy = psi + 5 * n;
+    const double vxy = g.by[j] * pxy[o] + g.cy[j] * gzy;
+    const double vxz = g.bz[k] * pxz[o] + g.cz[k] * gyz;
+    const double vyx = g.bx[i] * pyx[o] + g.cx[i] * gzx;
+    const double vyz = g.bz[k] * pyz[o] + g.cz[k] * gxz;
+    const double vzx = g.bx[i] * pzx[o] + g.cx[i] * gyx;
+    const double vzy = g.by[j] * pzy[o] + g.cy[j] * gxy;
+    pxy[o] = vxy; pxz[o] = vxz; pyx[o] = vyx; pyz[o] = vyz; pzx[o] = vzx; pzy[o] = vzy;
+    const double cx_ = g.iky[j] * gzy + vxy - g.ikz[k] * gyz - vxz;
+    const double cy_ = g.ikz[k] * gxz + vyz - g.ikx[i] * gzx - vyx;
+    const double cz_ = g.ikx[i] * gyx + vzx - g.iky[j] * gxy - vzy;
+    if (which == 0) {
+      const double sc = g.v2 * g.dt;
+      Fx[o].x += sc * cx_;
+      Fy[o].x += sc * cy_;
+      Fz[o].x += sc * cz_;
+    } else {
+      Fx[o].x -= g.dt * cx_;
+      Fy[o].x -= g.dt * cy_;
+      Fz[o].x -= g.dt * cz_;
+    }
+  }
+  if (src_comp >= 0) {
+    const double tau = sp.t - rx[i] / sp.v - sp.Tp;
+    const double gyz = exp(-(ry[j] * ry[j] + rz[k] * rz[k]) * sp.w0inv2);
+    const double einc = sp.amp * gyz * exp(-(tau * tau) / (sp.tau_G * sp.tau_G)) * cos(sp.omega0 * tau + sp.chirp * tau * tau);
+    cplx* F = src_comp == 0 ? Fx : src_comp == 1 ? Fy : Fz;
+    F[o].x += srcx[i] * einc * sp.dx;
+  }
+}
+
+// Trilinear sample of three real-space fields at 8 corners (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085).
+__global__ void k_sample(const cplx* Fx, const cplx* Fy, const cplx* Fz, const int32_t* idx, const double* w,
+                         double* out) {
+  if (threadIdx.x != 0) return;
+  cplx s[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  for (int c = 0; c < 8; ++c) {
+    const int64_t o = idx[c];
+    if (o < 0) continue;
+    s[0] = s[0] + w[c] * Fx[o];
+    s[1] = s[1] + w[c] * Fy[o];
+    s[2] = s[2] + w[c] * Fz[o];
+  }
+  for (int f = 0; f < 3; ++f) { out[2 * f] = s[f].x; out[2 * f + 1] = s[f].y; }
+}
+
+// Uniform wire line: Exx[f][x] = sample[f] for all x (QuantumWire, PSTD3D/rhoPJ.py:1160-1164).
+__global__ void k_fill_line(const double* sample, double* lines, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int f = 0; f < 3; ++f) {
+    lines[2 * ((int64_t)f * n + i)] = sample[2 * f];
+    lines[2 * ((int64_t)f * n + i) + 1] = sample[2 * f + 1];
+  }
+}
+
+// J line from the wires' polarisation and charge (PSTD3D/rhoPJ.py:1183-1194, CalcJfx :561-592):
+// J_c(x) = sum_w (P_cw - Pold_cw)/dt / Nw, J_x += CalcJfx(sum_w Rho_w, sum_w RhoOld_w)/Nw;
+// then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx] complex.  One block.
+__global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, cplx* Pold, cplx* RhoOld, int Nw,
+                                               int Nx, double dt, double dx, cplx* jl) {
+  __shared__ cplx drho[2048];
+  for (int x = threadIdx.x; x < Nx; x += 256) {
+    for (int c = 0; c < 3; ++c) {
+      cplx s = mk(0, 0);
+      for (int w = 0; w < Nw; ++w) {
+        const int64_t o = ((int64_t)w * 3 + c) * Nx + x;
+        s = s + (P[o] - Pold[o]);
+        Pold[o] = P[o];
+      }
+      jl[c * Nx + x] = mk(s.x / dt / Nw, s.y / dt / Nw);
+    }
+    cplx rn = mk(0, 0), ro = mk(0, 0);
+    for (int w = 0; w < Nw; ++w) {
+      const int64_t o = (int64_t)w * Nx + x;
+      rn = rn + Rho[o];
+      ro = ro + RhoOld[o];
+      RhoOld[o] = Rho[o];
+    }
+    const cplx dr = rn - ro;
+    drho[x] = mk(dr.x / dt, dr.y / dt);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cplx acc = mk(0, 0);
+    for (int x = 1; x < Nx; ++x) {
+      acc = acc - drho[x] * dx;
+      jl[x] = jl[x] + mk(acc.x / Nw, acc.y / Nw);
+    }
+  }
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
+                              const double* Jx, const double* Jy, const double* Jz, const double* jline,
+                              const double* gy, const double* gz, double* Ex, double* Ey, double* Ez, void* stream) {
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  hipLaunchKernelGGL(k_update_e, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g,
+                     (const cplx*)Bx, (const cplx*)By, (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy,
+                     (const cplx*)Jz, (const cplx*)jline, (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey,
+                     (cplx*)Ez);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_update_b3d(const pss_grid* g, const double* Ex, const double* Ey, const double* Ez, double* Bx,
+                              double* By, double* Bz, void* stream) {
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  hipLaunchKernelGGL(k_update_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g,
+                     (const cplx*)Ex, (const cplx*)Ey, (const cplx*)Ez, (cplx*)Bx, (cplx*)By, (cplx*)Bz);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_fft3d(const pss_grid* g, double* F, int32_t dir, void* stream) {
+  int rc = fft3(g, (const cplx*)F, (cplx*)F, dir, nullptr, -1, (hipStream_t)stream);
+  if (rc) return pss_set_msg(2, "pss_fft3d: axis length must be a power of two <= 2048");
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_fft_line(const pss_grid* g, const double* in, double* out, int32_t axis, int32_t dir,
+                            void* stream) {
+  int rc = launch_pass(g, (const cplx*)in, (cplx*)out, axis, dir, nullptr, -1, (hipStream_t)stream);
+  if (rc) return pss_set_msg(2, "pss_fft_line: bad axis length");
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
+                             const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
+                             const double* src_par, const double* srcx, const double* rx, const double* ry,
+                             const double* rz, const int32_t* sample_idx, const double* sample_w,
+                             double* sample_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  cplx* sc = (cplx*)scratch;
+  const cplx *gx = (const cplx*)Gx, *gy = (const cplx*)Gy, *gz = (const cplx*)Gz;
+  // spectral derivatives  d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
+  const cplx* srcs[6] = {gz, gy, gx, gz, gy, gx};
+  const int ax[6] = {1, 2, 2, 0, 0, 1};
+  const double* ks[3] = {g->kx, g->ky, g->kz};
+  for (int q = 0; q < 6; ++q) {
+    int rc = fft3(g, srcs[q], sc + q * n, +1, ks[ax[q]], ax[q], s);
+    if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
+  }
+  cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
+  for (int c = 0; c < 3; ++c) {
+    int rc = fft3(g, F[c], F[c], +1, nullptr, -1, s);
+    if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
+  }
+  SrcPar sp{};
+  if (src_comp >= 0) {
+    sp.t = src_par[0]; sp.amp = src_par[1]; sp.omega0 = src_par[2]; sp.tau_G = src_par[3];
+    sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
+  }
+  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, F[0], F[1], F[2], sc,
+                     sc + n, sc + 2 * n, sc + 3 * n, sc + 4 * n, sc + 5 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  if (sample_idx)
+    hipLaunchKernelGGL(k_sample, dim3(1), dim3(64), 0, s, F[0], F[1], F[2], sample_idx, sample_w, sample_out);
+  for (int c = 0; c < 3; ++c) {
+    int rc = fft3(g, F[c], F[c], -1, nullptr, -1, s);
+    if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
+  }
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_fill_line(const double* sample, double* lines, int32_t n, void* stream) {
+  hipLaunchKernelGGL(k_fill_line, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, sample, lines, n);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
+                         double dt, double dx, double* jline, void* stream) {
+  if (Nx > 2048) return pss_set_msg(2, "pss_jline: Nx > 2048");
+  hipLaunchKernelGGL(k_jline, dim3(1), dim3(256), 0, (hipStream_t)stream, (const cplx*)P, (const cplx*)Rho,
+                     (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}

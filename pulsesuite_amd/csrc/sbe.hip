@@ -1,0 +1,1096 @@
+// SBE wire step on MI355X (gfx950): FP64 path.
+//
+// One call advances every wire of a batch by one leapfrog step, following
+// SBESolver.QWCalculator -> Prop2QW -> _sbe_calculator -> QW2Prop
+// (PSTD3D/SBEs.py:5838-5997, 5624-5832; PSTD3D/qwoptics.py:673-811).
+// Wires are independent; every kernel takes the batch index from blockIdx.z
+// (or blockIdx.y) and reads that wire's descriptor from device memory, so wires
+// of different Nk share one launch (blocks past a wire's extent exit).
+//
+// State rotation: prev (X1) lives in buffer 0 and is averaged in place; cur (X2)
+// and next (X3) swap between buffers 1 and 2 by the parity of the wire's step
+// counter ctl[0]; *on is the solver-wide latch of the reference's wireoff gate.
+#include "common.h"
+#include "../../include/pss.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kHbar = 1.054571817e-34;    // scipy.constants.hbar
+constexpr double kE0 = 1.602176634e-19;      // scipy.constants.e
+
+// ---------------------------------------------------------------- workspace
+struct WS {
+  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, total;
+};
+
+__host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
+
+__host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
+  WS w;
+  int64_t o = 0;
+  int64_t nd = 2 * Nk - 1;
+  w.er = o;   o += al(3 * (int64_t)Nr);
+  w.spa = o;  o += al(3 * (int64_t)nrr);
+  w.eq = o;   o += al(3 * nd * 2);
+  w.vc = o;   o += al(3 * (int64_t)Nk);
+  w.e1d = o;  o += al(Nk);
+  w.ne2 = o;  o += al(2 * (int64_t)Nk);
+  w.nh2 = o;  o += al(2 * (int64_t)Nk);
+  w.tr = o;   o += al(4 * nd * 2);
+  w.gam = o;  o += al(2 * (int64_t)Nk);
+  w.De = o;   o += al((2 * (int64_t)Nk + 1) * Nk);
+  w.Dh = o;   o += al((2 * (int64_t)Nk + 1) * Nk);
+  w.H = o;    o += al(3 * (int64_t)Nk * Nk * 2);
+  w.W = o;    o += al(4 * (int64_t)Nk);
+  w.n3 = o;   o += al(4 * (int64_t)Nk);
+  w.red = o;  o += al(32);
+  w.ds = o;   o += al(4 * nd * 2);
+  w.pr = o;   o += al(3 * (int64_t)Nr);
+  w.rho = o;  o += al(2 * (int64_t)Nr * 2);
+  w.spb = o;  o += al(5 * (int64_t)Nr);
+  w.outr = o; o += al(5 * (int64_t)nrr);
+  w.total = o;
+  return w;
+}
+
+struct Bufs {
+  const cplx *p1, *p2, *C1, *C2, *D1, *D2;  // prev / cur
+  cplx *pp, *Cp, *Dp;                       // prev (written in place)
+  cplx *pn, *Cn, *Dn;                       // next
+};
+
+__device__ inline Bufs bufs(const pss_wire_desc& d) {
+  const int s = d.ctl[0] & 1;
+  const int cur = 1 + s, nxt = 2 - s;
+  Bufs b;
+  b.p1 = (const cplx*)d.p[0]; b.C1 = (const cplx*)d.C[0]; b.D1 = (const cplx*)d.D[0];
+  b.pp = (cplx*)d.p[0];       b.Cp = (cplx*)d.C[0];       b.Dp = (cplx*)d.D[0];
+  b.p2 = (const cplx*)d.p[cur]; b.C2 = (const cplx*)d.C[cur]; b.D2 = (const cplx*)d.D[cur];
+  b.pn = (cplx*)d.p[nxt]; b.Cn = (cplx*)d.C[nxt]; b.Dn = (cplx*)d.D[nxt];
+  return b;
+}
+
+__device__ inline bool wire_on(const pss_wire_desc& d) { return *d.on != 0; }
+
+// ------------------------------------------------------------ FMM spline
+// Second-derivative-like coefficients c[] of libpulsesuite/spliner.py:1287-1363
+// for real data y on knots with spacings h, given the y-independent elimination
+// factors t[] and pivots b[] (precomputed on the host with the same operations).
+// n >= 4.  Sequential recurrence: one thread per field.
+__device__ void spline_solve(const double* y, int ystride, int n, const double* h, const double* t,
+                             const double* b, const double* e, double* c) {
+  // interior second differences  c[i] = D_i - D_{i-1}
+  double Dprev = (y[ystride] - y[0]) / h[0];
+  double c1 = 0.0, c2 = 0.0, cm2 = 0.0, cm3 = 0.0;
+  for (int i = 1; i < n - 1; ++i) {
+    const double Di = (y[(int64_t)(i + 1) * ystride] - y[(int64_t)i * ystride]) / h[i];
+    const double ci = Di - Dprev;
+    c[i] = ci;
+    if (i == 1) c1 = ci;
+    if (i == 2) c2 = ci;
+    if (i == n - 2) cm2 = ci;
+    if (i == n - 3) cm3 = ci;
+    Dprev = Di;
+  }
+  // end conditions: e = {x3-x1, x2-x0, h0^2, x3-x0, x[n-1]-x[n-3], x[n-2]-x[n-4], h[n-2]^2, x[n-1]-x[n-4]}
+  double c0 = c2 / e[0] - c1 / e[1];
+  double cn = cm2 / e[4] - cm3 / e[5];
+  c0 = c0 * e[2] / e[3];
+  cn = -cn * e[6] / e[7];
+  c[0] = c0;
+  c[n - 1] = cn;
+  // forward elimination
+  double cp = c0;
+  for (int i = 1; i < n; ++i) {
+    cp = c[i] - t[i] * cp;
+    c[i] = cp;
+  }
+  // back substitution
+  double cn1 = c[n - 1] / b[n - 1];
+  c[n - 1] = cn1;
+  for (int i = n - 2; i >= 0; --i) {
+    cn1 = (c[i] - h[i] * cn1) / b[i];
+    c[i] = cn1;
+  }
+}
+
+// Evaluate the spline of (y, c) at interval i with offset dx (seval_dpc, spliner.py:1367-1396).
+__device__ inline double spline_eval(const double* y, int ystride, const double* c, const double* h, int n,
+                                     int i, double dx) {
+  double bi, ci, di;
+  const double yi = y[(int64_t)i * ystride];
+  if (i < n - 1) {
+    const double Di = (y[(int64_t)(i + 1) * ystride] - yi) / h[i];
+    bi = Di - h[i] * (c[i + 1] + 2.0 * c[i]);
+    di = (c[i + 1] - c[i]) / h[i];
+    ci = 3.0 * c[i];
+  } else {
+    const double Dl = (yi - y[(int64_t)(n - 2) * ystride]) / h[n - 2];
+    bi = Dl + h[n - 2] * (c[n - 2] + 2.0 * c[n - 1]);
+    di = (c[n - 1] - c[n - 2]) / h[n - 2];
+    ci = 3.0 * c[n - 1];
+  }
+  return yi + dx * (bi + dx * (ci + dx * di));
+}
+
+// -------------------------------------------------- K1: Prop2QW (rr -> r)
+// Gate + three spline solves on rr of Re(E) (Prop2QW takes the real part after
+// windowing; the spline is real-linear so Re commutes with it).
+__global__ void __launch_bounds__(256) k_prop_solve(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  __shared__ double sh[16];
+  double* ws = d.ws;
+  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
+  // wireoff gate (PSTD3D/SBEs.py:5861-5870): max_r sqrt(|Ex|^2+|Ey|^2+|Ez|^2) vs 1e-3 Emax0
+  double m = 0.0;
+  for (int i = threadIdx.x; i < d.nrr; i += 256) {
+    const double a = d.Exx[2 * i], b = d.Exx[2 * i + 1];
+    const double c = d.Eyy[2 * i], e = d.Eyy[2 * i + 1];
+    const double f = d.Ezz[2 * i], g = d.Ezz[2 * i + 1];
+    m = fmax(m, sqrt((a * a + b * b) + (c * c + e * e) + (f * f + g * g)));
+  }
+  m = block_max<256>(m, sh);
+  if (threadIdx.x == 0) {
+    if (!(*d.on == 0 && m < d.gate)) *d.on = 1;
+  }
+  __syncthreads();
+  if (!wire_on(d)) return;
+  if (threadIdx.x < 3) {
+    const double* y = (threadIdx.x == 0 ? d.Exx : threadIdx.x == 1 ? d.Eyy : d.Ezz);
+    spline_solve(y, 2, d.nrr, d.sa_h, d.sa_t, d.sa_b, d.sa_end, ws + L.spa + threadIdx.x * d.nrr);
+  }
+}
+
+// Evaluate at r_j, window, keep real part.
+__global__ void __launch_bounds__(256) k_prop_eval(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= d.Nr) return;
+  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const int i = d.sa_idx[j];
+  for (int f = 0; f < 3; ++f) {
+    double v = 0.0;
+    if (i >= 0) {
+      const double* y = (f == 0 ? d.Exx : f == 1 ? d.Eyy : d.Ezz);
+      v = spline_eval(y, 2, ws + L.spa + f * d.nrr, d.sa_h, d.nrr, i, d.sa_dx[j]);
+    }
+    ws[L.er + f * d.Nr + j] = v * d.win[j];
+  }
+}
+
+// Centred DFT (FFTG, PSTD3D/usefulsubs.py:2278-2304) of the three real fields at
+// the 2Nk-1 wavenumber indices m = NQ0 + dd that kkp reaches (NQ0 = Nk+1).
+__global__ void __launch_bounds__(256) k_prop_dft(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d)) return;
+  const int nd = 2 * d.Nk - 1;
+  const int od = blockIdx.x;
+  if (od >= nd) return;
+  __shared__ double sh[16];
+  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const int Nr = d.Nr;
+  const int m = (d.Nk + 1) + (od - (d.Nk - 1));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = threadIdx.x; j < Nr; j += 256) {
+    const int64_t tix = ((int64_t)j * m) % Nr;
+    const double wr = d.twid[2 * tix], wi = d.twid[2 * tix + 1];
+    const double s = (j & 1) ? -1.0 : 1.0;
+    for (int f = 0; f < 3; ++f) {
+      const double v = s * ws[L.er + f * Nr + j];
+      acc[2 * f] += v * wr;
+      acc[2 * f + 1] += v * wi;
+    }
+  }
+  for (int q = 0; q < 6; ++q) acc[q] = block_sum<256>(acc[q], sh);
+  if (threadIdx.x == 0) {
+    const double sm = (m & 1) ? -1.0 : 1.0;
+    for (int f = 0; f < 3; ++f) {
+      ws[L.eq + (f * nd + od) * 2] = -(sm * acc[2 * f]) / Nr;
+      ws[L.eq + (f * nd + od) * 2 + 1] = -(sm * acc[2 * f + 1]) / Nr;
+    }
+  }
+}
+
+// ---------------------------------- K2: populations, screening (CalcScreenedArrays)
+__global__ void __launch_bounds__(256) k_prep(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  if (!wire_on(d)) return;
+  __shared__ double sh[16];
+  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int Nk = d.Nk;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < Nk; k += 256) {
+    const cplx c = B.C2[(int64_t)k * Nk + k], e = B.D2[(int64_t)k * Nk + k];
+    ws[L.ne2 + 2 * k] = c.x; ws[L.ne2 + 2 * k + 1] = c.y;
+    ws[L.nh2 + 2 * k] = e.x; ws[L.nh2 + 2 * k + 1] = e.y;
+    s += c.x + e.x;
+  }
+  s = block_sum<256>(s, sh);
+  // PSTD3D/coulomb.py:1218-1230
+  double n1 = s / 2.0 / d.L;
+  n1 = fmin(n1, d.density_max);
+  const bool scr = d.flags & PSS_SCREENED;
+  for (int q = threadIdx.x; q < Nk; q += 256) {
+    double e1 = 1.0;
+    if (scr) {
+      e1 = 1.0 - d.chie[q] * 2 * log(fabs((d.qe[q] - kPi * n1) / (d.qe[q] + n1)))
+               - d.chih[q] * 2 * log(fabs((d.qh[q] - kPi * n1) / (d.qh[q] + n1)));
+    }
+    ws[L.e1d + q] = e1;
+    for (int n = 0; n < 3; ++n) ws[L.vc + n * Nk + q] = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
+  }
+  if (threadIdx.x == 0) ws[L.red + 0] = n1;
+}
+
+// ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H
+__global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d)) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1;
+  if ((int)blockIdx.x >= nd + Nk) return;
+  __shared__ double sh[16];
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  if ((int)blockIdx.x < nd) {
+    const int qd = (int)blockIdx.x - (Nk - 1);
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int k0 = max(0, -qd), k1 = min(Nk - 1, Nk - 1 - qd);
+    for (int k = k0 + threadIdx.x; k <= k1; k += 256) {
+      const cplx c = B.C2[(int64_t)k * Nk + k + qd];   // C[k, k+qd]
+      const cplx e = B.D2[(int64_t)(k + qd) * Nk + k]; // D[k+qd, k]
+      const cplx f = B.D2[(int64_t)k * Nk + k + qd];   // D[k, k+qd]
+      const cplx g = B.C2[(int64_t)(k + qd) * Nk + k]; // C[k+qd, k]
+      a[0] += c.x; a[1] += c.y; a[2] += e.x; a[3] += e.y; a[4] += f.x; a[5] += f.y; a[6] += g.x; a[7] += g.y;
+    }
+    for (int q = 0; q < 8; ++q) a[q] = block_sum<256>(a[q], sh);
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 4; ++q) {
+        ws[L.tr + (q * nd + blockIdx.x) * 2] = a[2 * q];
+        ws[L.tr + (q * nd + blockIdx.x) * 2 + 1] = a[2 * q + 1];
+      }
+    return;
+  }
+  // CalcGammaE/H  PSTD3D/dephasing.py:29-103 with ne = Re(C2_kk), nh = Re(D2_kk)
+  const int k = (int)blockIdx.x - nd;
+  const double* vc = ws + L.vc;
+  const double* ne = ws + L.ne2;  // interleaved complex; real part at 2*i
+  const double* nh = ws + L.nh2;
+  const int NN = Nk * Nk;
+  const int32_t* kpq = d.ktab + 0 * NN;
+  const int32_t* kmq = d.ktab + 1 * NN;
+  const int32_t* k1p = d.ktab + 2 * NN;
+  const int32_t* k1pmq = d.ktab + 3 * NN;
+  const int32_t* k1t = d.ktab + 4 * NN;
+  const int32_t* k1mq = d.ktab + 5 * NN;
+  const double f = kPi / kHbar;
+  double ge = 0.0, gh = 0.0;
+  if (d.flags & PSS_DIAGDPH) {
+    for (int q = threadIdx.x; q < Nk; q += 256) {
+      const int iq = d.iqv[q];
+      const double veh = vc[0 * Nk + iq], vee = vc[1 * Nk + iq], vhh = vc[2 * Nk + iq];
+      const double Veh2 = veh * veh, Vee2 = vee * vee, Vhh2 = vhh * vhh;
+      const double axe = fabs(d.xe[q]), axh = fabs(d.xh[q]);
+      const int64_t kq = (int64_t)k * Nk + q;
+      int a = kpq[kq];
+      if (a >= 0 && a < Nk) {
+        const double na = ne[2 * a], ha = nh[2 * a];
+        ge += f * Vee2 * na * (1.0 - na) * axe;
+        gh += f * Vhh2 * ha * (1.0 - ha) * axh;
+      }
+      const int c = kmq[kq];
+      a = k1pmq[kq];
+      int b = k1p[kq];
+      if (a >= 0 && a < Nk && b >= 0 && b < Nk && c >= 0 && c < Nk) {
+        const double nha = nh[2 * a], nhb = nh[2 * b], nec = ne[2 * c];
+        ge += f * Veh2 * (nha * (1.0 - nhb) * nec + (1.0 - nha) * nhb * (1.0 - nec)) * axh;
+      }
+      a = k1mq[kq];
+      b = k1t[kq];
+      if (a >= 0 && a < Nk && b >= 0 && b < Nk && c >= 0 && c < Nk) {
+        const double nea = ne[2 * a], neb = ne[2 * b], nhc = nh[2 * c];
+        gh += f * Veh2 * (nea * (1.0 - neb) * nhc + (1.0 - nea) * neb * (1.0 - nhc)) * axe;
+      }
+    }
+    ge = block_sum<256>(ge, sh);
+    gh = block_sum<256>(gh, sh);
+  } else {
+    ge = d.gam_e;
+    gh = d.gam_h;
+  }
+  if (threadIdx.x == 0) {
+    ws[L.gam + k] = ge;
+    ws[L.gam + Nk + k] = gh;
+  }
+}
+
+// ------------------------------- K4: off-diagonal dephasing tables De, Dh
+// PSTD3D/dephasing.py:227-317 (CalcOffDiagDeph_E2/H2) with n = |n2|.
+// Thread per (q, k) output; blockIdx.y: 0 = electrons, 1 = holes.
+__global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || !(d.flags & PSS_OFFDIAGDPH)) return;
+  const int Nk = d.Nk;
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)(2 * Nk + 1) * Nk) return;
+  const int qi = (int)(idx / Nk), k = (int)(idx % Nk);
+  const int q = qi - Nk;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const double* vc = ws + L.vc;
+  const bool holes = blockIdx.y == 1;
+  const double* nA = ws + (holes ? L.nh2 : L.ne2);
+  const double* nB = ws + (holes ? L.ne2 : L.nh2);
+  const double* EA = holes ? d.Eh : d.Ee;
+  const double* EB = holes ? d.Ee : d.Eh;
+  // g = [gam_eh, gam_e, gam_h]; E2 gets (g0, g2), H2 gets (g1, g2)  (dephasing.py:735-736)
+  const double bAA = kHbar * (holes ? d.gam_e : d.gam_eh);
+  const double bAB = kHbar * d.gam_h;
+  const int aq = q < 0 ? -q : q;
+  double vAA = 0.0, vAB = 0.0;
+  if (aq >= 1 && aq <= Nk - 1) {
+    const double a = vc[(holes ? 2 : 1) * Nk + aq], b = vc[0 * Nk + aq];
+    vAA = a * a;
+    vAB = b * b;
+  }
+  double acc = 0.0;
+  const int kq = k + q;
+  if (kq >= 0 && kq < Nk) {
+    const double nk = cabs(mk(nA[2 * k], nA[2 * k + 1]));
+    const double EAk = EA[k], EAkq = EA[kq];
+    if (vAA != 0.0) {
+      const int p0 = max(q + 1, 0), p1 = min(q + Nk - 1, Nk - 1);
+      for (int p = p0; p <= p1; ++p) {
+        const int pm = p - q;
+        const double npm = cabs(mk(nA[2 * pm], nA[2 * pm + 1])), np_ = cabs(mk(nA[2 * p], nA[2 * p + 1]));
+        const double Ed = EAkq + EA[pm] - EA[p] - EAk;
+        const double lr = (bAA / kPi) / (Ed * Ed + bAA * bAA);
+        acc += vAA * lr * (npm * (1.0 - np_) * (1.0 - nk) + (1.0 - npm) * np_ * nk);
+      }
+    }
+    if (vAB != 0.0) {
+      const int p0 = max(-q, 0), p1 = min(Nk - 1 - q, Nk - 1);
+      for (int p = p0; p <= p1; ++p) {
+        const int pp = p + q;
+        const double npp = cabs(mk(nB[2 * pp], nB[2 * pp + 1])), np_ = cabs(mk(nB[2 * p], nB[2 * p + 1]));
+        const double Ed = EAkq + EB[pp] - EB[p] - EAk;
+        const double lr = (bAB / kPi) / (Ed * Ed + bAB * bAB);
+        acc += vAB * lr * (npp * (1.0 - np_) * (1.0 - nk) + (1.0 - npp) * np_ * nk);
+      }
+    }
+  }
+  ws[(holes ? L.Dh : L.De) + idx] = acc * kPi / kHbar;
+}
+
+// ------------------------------------------------- K5: Hamiltonians (CalcH)
+// PSTD3D/SBEs.py:820-935 and _calc_Meh :5165-5175.  Exchange terms are Toeplitz
+// products along diagonals; direct terms use the diagonal traces of K3.
+__global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d)) return;
+  const int Nk = d.Nk;
+  const int k1 = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int k2 = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (k1 >= Nk || k2 >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const double* vc = ws + L.vc;
+  const int nd = 2 * Nk - 1;
+  const int64_t o = (int64_t)k1 * Nk + k2;
+  cplx heh = mk(0, 0), hee = mk(0, 0), hhh = mk(0, 0);
+  if (d.flags & PSS_OPTICS) {
+    // Meh[ke=k1, kh=k2] = -ehint (Xcv Ex[q] + Ycv Ey[q] yw + Zcv Ez[q] (-1)^w), q = kkp
+    const int od = (k1 - k2) + (Nk - 1);
+    const cplx ex = *(const cplx*)(ws + L.eq + (0 * nd + od) * 2);
+    const cplx ey = *(const cplx*)(ws + L.eq + (1 * nd + od) * 2);
+    const cplx ez = *(const cplx*)(ws + L.eq + (2 * nd + od) * 2);
+    const double xcv = d.dcv * (((k2 + 1) & 1) ? -1.0 : 1.0);
+    heh = (-d.ehint * xcv) * ex - (d.ehint * d.dcv) * ey * d.yw - (d.ehint * -d.dcv) * ez * d.zsign;
+  }
+  if (k1 == k2) {
+    hee = mk(d.Ee[k1] + d.gap, 0.0);
+    hhh = mk(d.Eh[k1], 0.0);
+  }
+  if (d.flags & PSS_EXCITONS) {
+    const bool lf = d.flags & PSS_LF;
+    const int qmin = max(-k1, -k2), qmax = min(Nk - 1 - k1, Nk - 1 - k2);
+    cplx a = mk(0, 0), b = mk(0, 0), c = mk(0, 0);
+    for (int q = qmin; q <= qmax; ++q) {
+      if (q == 0) continue;
+      const int aq = q < 0 ? -q : q;
+      const int64_t t = (int64_t)(k2 + q) * Nk + (k1 + q);  // X^T[k1+q, k2+q]
+      a = a + vc[aq] * B.p2[t];
+      if (lf || k1 == k2) {
+        b = b + vc[Nk + aq] * B.C2[t];
+        c = c + vc[2 * Nk + aq] * B.D2[t];
+      }
+    }
+    heh = heh + a;
+    hee = hee - b;
+    hhh = hhh - c;
+    if (lf && k1 != k2) {
+      const int qd = k1 - k2;
+      const int aq = qd < 0 ? -qd : qd;
+      const int od = qd + (Nk - 1);
+      const cplx tC = *(const cplx*)(ws + L.tr + (0 * nd + od) * 2);
+      const cplx tD = *(const cplx*)(ws + L.tr + (1 * nd + od) * 2);
+      const cplx tD2 = *(const cplx*)(ws + L.tr + (2 * nd + od) * 2);
+      const cplx tC2 = *(const cplx*)(ws + L.tr + (3 * nd + od) * 2);
+      hee = hee + (vc[Nk + aq] * tC - vc[aq] * tD);
+      hhh = hhh + (vc[2 * Nk + aq] * tD2 - vc[aq] * tC2);
+    }
+  }
+  cplx* H = (cplx*)(ws + L.H);
+  const int64_t NN = (int64_t)Nk * Nk;
+  H[o] = heh;
+  H[NN + o] = hee;
+  H[2 * NN + o] = hhh;
+}
+
+// ------------------------------------------- K6: RHS GEMMs + leapfrog epilogue
+// Output X in {p, C, D} = sum of complex products op(A) op(B) (dpdt/dCdt/dDdt,
+// PSTD3D/SBEs.py:1953-1976, 2320-2338, 2618-2636), then
+//   X3 = X1 + 2 dt R, R = S/(i hbar) (+ elementwise terms for p),
+//   next <- X3, prev <- (X1 + X3)/2   (PSTD3D/SBEs.py:5685-5718).
+// The off-diagonal dephasing OffP = i hbar (Mh p + p Me^T) (dephasing.py:321-356)
+// enters as two more products with operands pre-scaled by i*hbar.
+// Tile 32x32 per wave (64 threads), v_mfma_f64_16x16x4f64, K block 16.
+enum Op { OP_N = 0, OP_T = 1, OP_C = 2, OP_H = 3, OP_MH = 4, OP_MET = 5 };
+enum Src { S_HHH, S_HEE, S_HEH, S_C, S_D, S_P, S_NONE };
+
+struct Term {
+  int8_t a, aop, b, bop;
+  int8_t neg;
+};
+
+// terms per output: p: 6, C: 4, D: 4
+__constant__ Term kTerms[3][6] = {
+    {{S_HHH, OP_N, S_P, OP_N, 0}, {S_P, OP_N, S_HEE, OP_T, 0}, {S_HEH, OP_T, S_C, OP_N, 1},
+     {S_D, OP_T, S_HEH, OP_T, 1}, {S_NONE, OP_MH, S_P, OP_N, 0}, {S_P, OP_N, S_NONE, OP_MET, 0}},
+    {{S_C, OP_N, S_HEE, OP_T, 0}, {S_HEE, OP_T, S_C, OP_N, 1}, {S_P, OP_H, S_HEH, OP_T, 0},
+     {S_HEH, OP_C, S_P, OP_N, 1}, {S_NONE, 0, S_NONE, 0, 0}, {S_NONE, 0, S_NONE, 0, 0}},
+    {{S_D, OP_N, S_HHH, OP_T, 0}, {S_HHH, OP_T, S_D, OP_N, 1}, {S_P, OP_C, S_HEH, OP_N, 0},
+     {S_HEH, OP_H, S_P, OP_T, 1}, {S_NONE, 0, S_NONE, 0, 0}, {S_NONE, 0, S_NONE, 0, 0}},
+};
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline const cplx* src_ptr(int s, const Bufs& B, const cplx* H, int64_t NN) {
+  switch (s) {
+    case S_HEH: return H;
+    case S_HEE: return H + NN;
+    case S_HHH: return H + 2 * NN;
+    case S_C: return B.C2;
+    case S_D: return B.D2;
+    default: return B.p2;
+  }
+}
+
+// element (i, k) of op(X) where X is Nk x Nk; for OP_MH / OP_MET the real
+// matrices built from the dephasing tables, times i*hbar.
+__device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, const double* Dh,
+                               const double* De) {
+  if (i >= Nk || k >= Nk) return mk(0, 0);
+  switch (op) {
+    case OP_N: return X[(int64_t)i * Nk + k];
+    case OP_T: return X[(int64_t)k * Nk + i];
+    case OP_C: return conj(X[(int64_t)i * Nk + k]);
+    case OP_H: return conj(X[(int64_t)k * Nk + i]);
+    case OP_MH: {  // Mh[kh=i, k'=k] = Dh[k'-kh+Nk, kh]
+      if (i == k) return mk(0, 0);
+      return mk(0.0, kHbar * Dh[(int64_t)(k - i + Nk) * Nk + i]);
+    }
+    default: {  // OP_MET: Me^T[k'=i, ke=k] = Me[ke, k'] = De[k'-ke+Nk, ke]
+      if (i == k) return mk(0, 0);
+      return mk(0.0, kHbar * De[(int64_t)(i - k + Nk) * Nk + k]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!wire_on(d)) return;
+  const int Nk = d.Nk;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  if (m0 >= Nk || n0 >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int64_t NN = (int64_t)Nk * Nk;
+  const cplx* H = (const cplx*)(ws + L.H);
+  const double* Dh = ws + L.Dh;
+  const double* De = ws + L.De;
+  const int lane = threadIdx.x;
+
+  __shared__ double Ar[16][32], Ai[16][32], Br[16][32], Bi[16][32];
+  v4d accr[2][2], acci[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      accr[a][b] = v4d{0, 0, 0, 0};
+      acci[a][b] = v4d{0, 0, 0, 0};
+    }
+  const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
+  for (int t = 0; t < nterms; ++t) {
+    const Term T = kTerms[out][t];
+    const cplx* Ap = src_ptr(T.a, B, H, NN);
+    const cplx* Bp = src_ptr(T.b, B, H, NN);
+    const double sg = T.neg ? -1.0 : 1.0;
+    const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
+    const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
+    for (int k0 = 0; k0 < Nk; k0 += 16) {
+      // A tile: 32 (m) x 16 (k), coalesced along the contiguous global index
+      for (int e = lane; e < 512; e += 64) {
+        int mm, kk;
+        if (!aT) { mm = e >> 4; kk = e & 15; } else { kk = e >> 5; mm = e & 31; }
+        const cplx v = load_op(Ap, T.aop, m0 + mm, k0 + kk, Nk, Dh, De);
+        Ar[kk][mm] = sg * v.x;
+        Ai[kk][mm] = sg * v.y;
+      }
+      for (int e = lane; e < 512; e += 64) {
+        int kk, nn;
+        if (!bT) { kk = e >> 5; nn = e & 31; } else { nn = e >> 4; kk = e & 15; }
+        // op(B)[k, n]: for transposed ops load_op's (i, k) = (k, n) of op(B)
+        const cplx v = load_op(Bp, T.bop, k0 + kk, n0 + nn, Nk, Dh, De);
+        Br[kk][nn] = v.x;
+        Bi[kk][nn] = v.y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        const int li = lane & 15;
+        double ar[2], ai[2], br[2], bi[2];
+        for (int a = 0; a < 2; ++a) {
+          ar[a] = Ar[kr][a * 16 + li];
+          ai[a] = Ai[kr][a * 16 + li];
+          br[a] = Br[kr][a * 16 + li];
+          bi[a] = Bi[kr][a * 16 + li];
+        }
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) {
+            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
+          }
+      }
+      __syncthreads();
+    }
+  }
+  // epilogue
+  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
+  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
+  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
+  const double* gam = ws + L.gam;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + a * 16 + 4 * (lane >> 4) + v;
+        const int j = n0 + b * 16 + (lane & 15);
+        if (i >= Nk || j >= Nk) continue;
+        cplx S = mk(accr[a][b][v], acci[a][b][v]);
+        const int64_t o = (int64_t)i * Nk + j;
+        if (out == 0) {
+          // + Heh^T[kh=i, ke=j] - i hbar (GamE[ke] + GamH[kh]) p[kh, ke]
+          const cplx hT = H[(int64_t)j * Nk + i];
+          const cplx pv = B.p2[o];
+          const double g = kHbar * (gam[j] + gam[Nk + i]);
+          S = S + hT - mk(-g * pv.y, g * pv.x);
+        }
+        const cplx R = div_ih(S, kHbar);
+        const cplx x1 = X1[o];
+        const cplx x3 = x1 + (R * d.dt) * 2.0;
+        Xn[o] = x3;
+        if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+      }
+}
+
+// --------------------------- K7: carrier-carrier + phonon collision rates
+// MBCE/MBCH (PSTD3D/coulomb.py:957-1071) with the Ceh/Cee/Chh tables generated
+// on the fly and read with the reference's one-slot index shift (slot (a,b,c)
+// holds the entry built for (a-1,b-1,c-1); slot 0 is zero); MBPE/MBPH
+// (PSTD3D/phonons.py:140-268).  n = |Re n3| for MBC*, Re n3 for MBP*.
+struct CollCtx {
+  const double *Ee, *Eh, *ne, *nh;  // ne/nh here: 1-based padded |Re n3|, index 0 = 0
+  int Nk;
+  double heh, hee, hhh;             // hbar*gam_eh, hbar*gam_e, hbar*gam_h
+  double geh, gee, ghh;
+  bool lorentz;
+};
+
+// value read as C_kind[a, b, c] (0-based read indices); kind 0 = eh, 1 = ee, 2 = hh
+__device__ inline double coll(const CollCtx& x, int kind, int a, int b, int c) {
+  const int A = a - 1, Bq = b - 1, Cq = c - 1;
+  if (A < 0 || Bq < 0 || Cq < 0) return 0.0;
+  int k30;
+  const double *E1, *E2;
+  double hw, g;
+  if (kind == 0) { k30 = Cq + Bq - A; E1 = x.Ee; E2 = x.Eh; hw = x.heh; g = x.geh; }
+  else if (kind == 1) { k30 = A + Bq - Cq; E1 = x.Ee; E2 = x.Ee; hw = x.hee; g = x.gee; }
+  else { k30 = A + Bq - Cq; E1 = x.Eh; E2 = x.Eh; hw = x.hhh; g = x.ghh; }
+  if (k30 < 0 || k30 >= x.Nk) return 0.0;
+  if (A == Cq || Bq == k30) return 0.0;
+  const double Ed = E1[A] + E2[Bq] - E2[k30] - E1[Cq];
+  if (x.lorentz) return 2.0 * g / (Ed * Ed + hw * hw);
+  const double z = Ed / hw;
+  return (2.0 * kPi / kHbar) * (1.0 / (sqrt(kPi) * hw) * exp(-(z * z)));
+}
+
+__global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || !(d.flags & (PSS_EHS | PSS_PHONON))) return;
+  const int Nk = d.Nk;
+  const int kk = blockIdx.x;  // 0-based output index
+  if (kk >= Nk) return;
+  const bool holes = blockIdx.y == 1;
+  __shared__ double sh[16];
+  __shared__ double nsh[2][4097];
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  // populations from the new diagonal (C3, D3 before relaxation)
+  for (int k = threadIdx.x; k <= Nk; k += 256) {
+    if (k == 0) { nsh[0][0] = 0.0; nsh[1][0] = 0.0; continue; }
+    nsh[0][k] = fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    nsh[1][k] = fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+  }
+  __syncthreads();
+  const double* ne = nsh[0];
+  const double* nh = nsh[1];
+  const double* vc = ws + L.vc;
+  CollCtx x{d.Ee, d.Eh, ne, nh, Nk, kHbar * d.gam_eh, kHbar * d.gam_e, kHbar * d.gam_h,
+            d.gam_eh, d.gam_e, d.gam_h, (d.flags & PSS_LORENTZDELTA) != 0};
+  double win = 0.0, wout = 0.0;
+  if (d.flags & PSS_EHS) {
+    const int64_t npair = (int64_t)Nk * Nk;
+    for (int64_t pr = threadIdx.x; pr < npair; pr += 256) {
+      const int q1 = (int)(pr / Nk) + 1, q2 = (int)(pr % Nk) + 1;
+      if (!holes) {
+        const int k = kk + 1;
+        {  // e-h in/out
+          const int kp = q1, k1 = q2;
+          int v = kp + k1 - k;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[abs(k - k1)];
+            win += V * V * (1.0 - nh[kp]) * nh[v] * ne[k1] * coll(x, 0, k - 1, kp - 1, k1 - 1);
+          }
+          v = kp + k - k1;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[abs(k1 - k)];
+            wout += V * V * (1.0 - ne[k1]) * (1.0 - nh[kp]) * nh[v] * coll(x, 0, k1 - 1, kp - 1, k - 1);
+          }
+        }
+        {  // e-e in/out
+          const int k2 = q1, k4 = q2;
+          int v = k + k2 - k4;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[Nk + abs(k - k4)];
+            win += V * V * (1.0 - ne[k2]) * ne[v] * ne[k4] * coll(x, 1, k - 1, k2 - 1, k4 - 1);
+          }
+          v = k4 + k2 - k;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[Nk + abs(k4 - k)];
+            wout += V * V * (1.0 - ne[k4]) * (1.0 - ne[k2]) * ne[v] * coll(x, 1, k4 - 1, k2 - 1, k - 1);
+          }
+        }
+      } else {
+        const int kp = kk + 1;
+        {
+          const int k = q1, k1 = q2;
+          int v = kp + k1 - k;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[abs(k - k1)];
+            win += V * V * (1.0 - ne[k]) * nh[v] * ne[k1] * coll(x, 0, k - 1, kp - 1, k1 - 1);
+          }
+          v = kp + k - k1;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[abs(k1 - k)];
+            wout += V * V * (1.0 - ne[k]) * (1.0 - nh[v]) * ne[k1] * coll(x, 0, k - 1, v - 1, k1 - 1);
+          }
+        }
+        {
+          const int k2p = q1, k4p = q2;
+          int v = kp + k2p - k4p;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[2 * Nk + abs(kp - k4p)];
+            win += V * V * (1.0 - nh[k2p]) * nh[v] * nh[k4p] * coll(x, 2, kp - 1, k2p - 1, k4p - 1);
+          }
+          v = k4p + k2p - kp;
+          if (v >= 1 && v <= Nk) {
+            const double V = vc[2 * Nk + abs(k4p - kp)];
+            wout += V * V * (1.0 - nh[k4p]) * (1.0 - nh[k2p]) * nh[v] * coll(x, 2, k4p - 1, k2p - 1, kp - 1);
+          }
+        }
+      }
+    }
+  }
+  if (d.flags & PSS_PHONON) {
+    // MBPE: Win[k] += sum_i Vep[k,i] n[i] EP[k,i];  Wout[k] += sum_i Vep[k,i] (1-n[i]) EP[i,k]
+    // MBPH: Win[k] += sum_i Vhp[i,k] n[i] HP[k,i];  Wout[k] += sum_i Vhp[i,k] (1-n[i]) HP[i,k]
+    // EP[a,b] = 2 Gph idel (N0/((E[a]-E[b]-hO)^2+hG^2) + (N0+1)/((E[a]-E[b]+hO)^2+hG^2))
+    const double* E = holes ? d.Eh : d.Ee;
+    const double* e1d = ws + L.e1d;
+    const double* vrow = vc + (holes ? 2 : 1) * Nk;
+    const Bufs& Bq = B;
+    const double N0 = d.N0, hO = d.hOph, hG2 = d.hGph * d.hGph, G2 = 2.0 * d.Gph;
+    for (int i = threadIdx.x; i < Nk; i += 256) {
+      if (i == kk) continue;
+      const cplx ni_c = holes ? Bq.Dn[(int64_t)i * Nk + i] : Bq.Cn[(int64_t)i * Nk + i];
+      const double ni = ni_c.x;
+      const int dq = abs(kk - i);
+      const double V = vrow[dq] / e1d[dq] * d.Vscale;
+      const double dEab = E[kk] - E[i];  // EP[k, i]
+      const double dEba = E[i] - E[kk];  // EP[i, k]
+      const double Pki = (N0 / ((dEab - hO) * (dEab - hO) + hG2) + (N0 + 1.0) / ((dEab + hO) * (dEab + hO) + hG2)) * G2;
+      const double Pik = (N0 / ((dEba - hO) * (dEba - hO) + hG2) + (N0 + 1.0) / ((dEba + hO) * (dEba + hO) + hG2)) * G2;
+      win += V * ni * Pki;
+      wout += V * (1.0 - ni) * Pik;
+    }
+  }
+  win = block_sum<256>(win, sh);
+  wout = block_sum<256>(wout, sh);
+  if (threadIdx.x == 0) {
+    ws[L.W + (holes ? 2 : 0) * Nk + kk] = win;
+    ws[L.W + (holes ? 3 : 1) * Nk + kk] = wout;
+  }
+}
+
+// --------------------- K8: exponential relaxation + renormalisation of n3
+// PSTD3D/SBEs.py:5321-5333 and 5704-5718 (diagonals of C3, D3 and of the averaged prev).
+__global__ void __launch_bounds__(256) k_pops(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  if (!wire_on(d)) return;
+  __shared__ double sh[16];
+  const int Nk = d.Nk;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const bool relax = d.flags & (PSS_EHS | PSS_PHONON);
+  const double dt = d.dt;
+  double se = 0.0, sh_ = 0.0;
+  for (int k = threadIdx.x; k < Nk; k += 256) {
+    const int64_t o = (int64_t)k * Nk + k;
+    cplx ne = B.Cn[o], nh = B.Dn[o];
+    if (relax) {
+      const double wi = ws[L.W + k], wo = ws[L.W + Nk + k];
+      const double hi = ws[L.W + 2 * Nk + k], ho = ws[L.W + 3 * Nk + k];
+      const double We = wi + wo, Wh = hi + ho;
+      const double ee = exp(-We * dt), eh = exp(-Wh * dt);
+      const double Ee1 = -1 + exp(We * dt), Eh1 = -1 + exp(Wh * dt);
+      ne = ee * ((mk(Ee1, 0) + ne) * wi + ne * wo);
+      ne = mk(ne.x / (We + 1e-200), ne.y / (We + 1e-200));
+      nh = (eh * ((mk(Eh1, 0) + nh) * hi + nh * ho));
+      nh = mk(nh.x / (Wh + 1e-200), nh.y / (Wh + 1e-200));
+    }
+    const double ae = cabs(ne), ah = cabs(nh);
+    ws[L.n3 + k] = ae;
+    ws[L.n3 + Nk + k] = ah;
+    se += ae;
+    sh_ += ah;
+  }
+  se = block_sum<256>(se, sh);
+  sh_ = block_sum<256>(sh_, sh);
+  const double total = (se + sh_) / 2.0;
+  for (int k = threadIdx.x; k < Nk; k += 256) {
+    const int64_t o = (int64_t)k * Nk + k;
+    const double ne = ws[L.n3 + k] * total / (se + 1e-200);
+    const double nh = ws[L.n3 + Nk + k] * total / (sh_ + 1e-200);
+    ws[L.n3 + k] = ne;
+    ws[L.n3 + Nk + k] = nh;
+    B.Cn[o] = mk(ne, 0.0);
+    B.Dn[o] = mk(nh, 0.0);
+    const cplx c1 = B.C1[o], d1 = B.D1[o];
+    B.Cp[o] = (c1 + mk(ne, 0.0)) * 0.5;
+    B.Dp[o] = (d1 + mk(nh, 0.0)) * 0.5;
+  }
+}
+
+// --------------------------- K9: diagonal sums of p3, C3, conj(D3)
+// QWPolarization3 / QWRho5 depend on k_e - k_h (resp. k2 - k1) only through
+// exp(i (k_e - k_h) r) with k_e - k_h = dd * dkr, so the O(Nr Nk^2) sums of
+// PSTD3D/qwoptics.py:56-82, 367-380 reduce to sums along diagonals dd.
+__global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d)) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1;
+  if ((int)blockIdx.x >= nd) return;
+  __shared__ double sh[16];
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int dd = (int)blockIdx.x - (Nk - 1);
+  const int a0 = max(0, -dd), a1 = min(Nk - 1, Nk - 1 - dd);
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = a0 + threadIdx.x; a <= a1; a += 256) {
+    const int64_t o = (int64_t)a * Nk + a + dd;
+    const cplx pv = B.pn[o];
+    const double sg = ((a + 1) & 1) ? -1.0 : 1.0;  // (-1)^(kh+1)
+    s[0] += sg * pv.x; s[1] += sg * pv.y;
+    s[2] += pv.x; s[3] += pv.y;
+    const cplx cv = B.Cn[o], dv = B.Dn[o];
+    s[4] += cv.x; s[5] += cv.y;
+    s[6] += dv.x; s[7] -= dv.y;
+  }
+  for (int q = 0; q < 8; ++q) s[q] = block_sum<256>(s[q], sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) {
+      ws[L.ds + (q * nd + blockIdx.x) * 2] = s[2 * q];
+      ws[L.ds + (q * nd + blockIdx.x) * 2 + 1] = s[2 * q + 1];
+    }
+}
+
+// P(r), rho(r) on the wire grid (before the centred FFT pair, which cancels).
+__global__ void __launch_bounds__(256) k_pol_rho(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= d.Nr) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const cplx* ds = (const cplx*)(ws + L.ds);
+  const double a = d.dkr * d.r[j];
+  cplx sx = mk(0, 0), sp = mk(0, 0), sc = mk(0, 0), sd = mk(0, 0);
+  const bool lf = d.flags & PSS_LF, opt = d.flags & PSS_OPTICS;
+  for (int od = 0; od < nd; ++od) {
+    double sn, cs;
+    sincos((double)(od - (Nk - 1)) * a, &sn, &cs);
+    const cplx e = mk(cs, sn);
+    if (opt) {
+      sx = sx + ds[od] * e;
+      sp = sp + ds[nd + od] * e;
+    }
+    if (lf) {
+      sc = sc + ds[2 * nd + od] * e;
+      sd = sd + ds[3 * nd + od] * e;
+    }
+  }
+  const double w = d.win[j];
+  const double f = 2 * d.ehint / d.area / d.L;
+  ws[L.pr + j] = opt ? (sx.x * d.dcv) * w * f : 0.0;
+  ws[L.pr + d.Nr + j] = opt ? (sp.x * d.dcv) * w * d.yw * f : 0.0;
+  ws[L.pr + 2 * d.Nr + j] = opt ? (sp.x * -d.dcv) * w * d.zsign * f : 0.0;
+  cplx* rho = (cplx*)(ws + L.rho);
+  rho[j] = lf ? sc * (w / (2.0 * d.L)) : mk(0, 0);
+  rho[d.Nr + j] = lf ? sd * (w / (2.0 * d.L)) : mk(0, 0);
+}
+
+// QWRho5 edge removal + normalisation (qwoptics.py:1065-1077), the 2 e0 ehint/area
+// scale (SBEs.py:5764-5765), QW2Prop plasmonic normalisation (qwoptics.py:784-789)
+// and the five spline solves on r for QW2Prop.
+__global__ void __launch_bounds__(256) k_rho_spline(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  if (!wire_on(d)) return;
+  __shared__ double sh[16];
+  const int Nk = d.Nk, Nr = d.Nr;
+  const WS L = ws_layout(Nk, Nr, d.nrr);
+  double* ws = d.ws;
+  cplx* rho = (cplx*)(ws + L.rho);
+  const bool lf = d.flags & PSS_LF;
+  double* y5 = ws + L.pr;  // Px, Py, Pz already here; re, rh appended below as real arrays
+  if (lf) {
+    const double dr = d.r[1] - d.r[0];
+    for (int s = 0; s < 2; ++s) {
+      cplx* R = rho + s * Nr;
+      double edge = 0.0;
+      for (int j = threadIdx.x; j < 10; j += 256) edge += R[j].x + R[Nr - 10 + j].x;
+      edge = block_sum<256>(edge, sh);
+      const double sub = edge / 20.0;
+      double sa = 0.0;
+      for (int j = threadIdx.x; j < Nr; j += 256) {
+        R[j].x -= sub;
+        sa += cabs(R[j]);
+      }
+      sa = block_sum<256>(sa, sh);
+      double ntot = 0.0;
+      for (int k = threadIdx.x; k < Nk; k += 256) ntot += fabs(ws[L.n3 + s * Nk + k]);
+      ntot = block_sum<256>(ntot, sh) + 1e-50;
+      const double sc = ntot / (sa * dr + 1e-100);
+      for (int j = threadIdx.x; j < Nr; j += 256) {
+        cplx v = R[j] * sc;
+        v = mk(2.0 * kE0 * v.x / d.area * d.ehint, 2.0 * kE0 * v.y / d.area * d.ehint);
+        R[j] = v;
+      }
+    }
+    __syncthreads();
+    // QW2Prop plasmonic step: re = |re|, normalise both to the common total
+    double s0 = 0.0, s1 = 0.0;
+    for (int j = threadIdx.x; j < Nr; j += 256) {
+      const double a = cabs(rho[j]), b = cabs(rho[Nr + j]);
+      ws[L.spb + 3 * Nr + j] = a;  // temporary
+      ws[L.spb + 4 * Nr + j] = b;
+      s0 += a;
+      s1 += b;
+    }
+    s0 = block_sum<256>(s0, sh);
+    s1 = block_sum<256>(s1, sh);
+    const double total = (s0 * dr + s1 * dr + 1e-100) / 2.0;
+    const double f0 = total / (s0 * dr + 1e-100), f1 = total / (s1 * dr + 1e-100);
+    for (int j = threadIdx.x; j < Nr; j += 256) {
+      rho[j] = mk(ws[L.spb + 3 * Nr + j] * f0, 0.0);
+      rho[Nr + j] = mk(ws[L.spb + 4 * Nr + j] * f1, 0.0);
+    }
+    if (threadIdx.x == 0) ws[L.red + 1] = total;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int f = threadIdx.x;
+    if (f >= 3 && !lf) return;
+    const double* y = f < 3 ? (y5 + f * Nr) : (const double*)(rho + (f - 3) * Nr);
+    const int ys = f < 3 ? 1 : 2;
+    spline_solve(y, ys, Nr, d.sb_h, d.sb_t, d.sb_b, d.sb_end, ws + L.spb + f * Nr);
+  }
+}
+
+// Evaluate on rr, |.| and final normalisation of rho, Rho = RhoH - RhoE, zeros when
+// gated off or ignorewire; advance the step counter.
+__global__ void __launch_bounds__(256) k_qw2prop(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  const int n = d.nrr;
+  const bool on = wire_on(d);
+  const bool zero = !on || (d.flags & PSS_IGNOREWIRE);
+  __shared__ double sh[16];
+  const int Nr = d.Nr;
+  const WS L = ws_layout(d.Nk, Nr, n);
+  double* ws = d.ws;
+  const bool lf = d.flags & PSS_LF;
+  double se = 0.0, sh_ = 0.0;
+  if (on) {
+    const double* y5 = ws + L.pr;
+    const cplx* rho = (const cplx*)(ws + L.rho);
+    for (int i = threadIdx.x; i < n; i += 256) {
+      const int ix = d.sb_idx[i];
+      for (int f = 0; f < 5; ++f) {
+        double v = 0.0;
+        if (ix >= 0 && (f < 3 || lf)) {
+          const double* y = f < 3 ? (y5 + f * Nr) : (const double*)(rho + (f - 3) * Nr);
+          v = spline_eval(y, f < 3 ? 1 : 2, ws + L.spb + f * Nr, d.sb_h, Nr, ix, d.sb_dx[i]);
+        }
+        ws[L.outr + f * n + i] = v;
+      }
+      if (lf) {
+        const double a = fabs(ws[L.outr + 3 * n + i]), b = fabs(ws[L.outr + 4 * n + i]);
+        ws[L.outr + 3 * n + i] = a;
+        ws[L.outr + 4 * n + i] = b;
+        se += a;
+        sh_ += b;
+      }
+    }
+  }
+  se = block_sum<256>(se, sh);
+  sh_ = block_sum<256>(sh_, sh);
+  const double total = ws[L.red + 1];
+  const double fe = lf ? total / (se * d.drr + 1e-100) : 0.0;
+  const double fh = lf ? total / (sh_ * d.drr + 1e-100) : 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (zero) {
+      d.Pxx[2 * i] = 0; d.Pxx[2 * i + 1] = 0;
+      d.Pyy[2 * i] = 0; d.Pyy[2 * i + 1] = 0;
+      d.Pzz[2 * i] = 0; d.Pzz[2 * i + 1] = 0;
+      d.Rho[2 * i] = 0; d.Rho[2 * i + 1] = 0;
+      continue;
+    }
+    d.Pxx[2 * i] = ws[L.outr + i]; d.Pxx[2 * i + 1] = 0;
+    d.Pyy[2 * i] = ws[L.outr + n + i]; d.Pyy[2 * i + 1] = 0;
+    d.Pzz[2 * i] = ws[L.outr + 2 * n + i]; d.Pzz[2 * i + 1] = 0;
+    const double re = lf ? ws[L.outr + 3 * n + i] * fe : 0.0;
+    const double rh = lf ? ws[L.outr + 4 * n + i] * fh : 0.0;
+    d.Rho[2 * i] = rh - re;
+    d.Rho[2 * i + 1] = 0;
+  }
+  if (threadIdx.x == 0 && on) d.ctl[0] += 1;
+}
+
+// ------------------------------------------------------------ Vint (init)
+__device__ inline double k03_series(double x) {
+  // PSTD3D/coulomb.py:39-66
+  if (x > 100.0) return 0.0;
+  if (x < 1e-10) return 23.0;
+  if (x < 2.0) {
+    const double x2 = x * x;
+    double result = -log(0.5 * x) - 0.5772156649015329;
+    double term = 1.0;
+    for (int k = 1; k < 20; ++k) {
+      term = term * 0.25 * x2 / (double)(k * k);
+      result = result + term * (1.0 / k - 0.5772156649015329);
+      if (fabs(term) < 1e-15) break;
+    }
+    return result;
+  }
+  return sqrt(kPi / (2.0 * x)) * exp(-x);
+}
+
+__global__ void __launch_bounds__(256) k_vint(const double* __restrict__ y, int Ny, const double* __restrict__ q,
+                                              double ae, double ah, double Delta0, double pref, double* out) {
+  __shared__ double sh[16];
+  const int iq = blockIdx.x;
+  const int N1 = Ny / 4, N2 = 3 * Ny / 4;
+  const int m = N2 - N1 + 1;
+  const double kmin = (ae + ah) / 4.0;
+  const double dk = fmax(fabs(q[iq]), kmin);
+  const double mc = ae * ah / kPi * (y[1] - y[0]) * (y[1] - y[0]);
+  double acc = 0.0;
+  for (int64_t t = threadIdx.x; t < (int64_t)m * m; t += 256) {
+    const int i = N1 + (int)(t / m), j = N1 + (int)(t % m);
+    const double dy = y[i] - y[j];
+    const double r = sqrt(dy * dy + Delta0 * Delta0);
+    const double ex = exp(-(ae * y[i]) * (ae * y[i]) - (ah * y[j]) * (ah * y[j]));
+    acc += ex * mc * k03_series(dk * r);
+  }
+  acc = block_sum<256>(acc, sh);
+  if (threadIdx.x == 0) out[iq] = pref * acc;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr) {
+  return ws_layout(Nk, Nr, nrr).total * (int64_t)sizeof(double);
+}
+
+extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+                            int32_t max_nrr, void* stream) {
+  if (nwires <= 0) return 0;
+  if (max_Nk < 4 || max_Nk > 4096 || max_Nr > 8194 || max_nrr < 4)
+    return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
+  hipStream_t s = (hipStream_t)stream;
+  const int nd = 2 * max_Nk - 1;
+  const int nrb = (max_Nr + 255) / 256;
+  hipLaunchKernelGGL(k_prop_solve, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_prop_eval, dim3(nrb, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_prep, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
+  const int64_t nq = (int64_t)(2 * max_Nk + 1) * max_Nk;
+  hipLaunchKernelGGL(k_offdiag, dim3((unsigned)((nq + 255) / 256), 2, nwires), dim3(256), 0, s, d_descs);
+  const int t16 = (max_Nk + 15) / 16;
+  hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
+  const int t32 = (max_Nk + 31) / 32;
+  hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(64), 0, s, d_descs);
+  hipLaunchKernelGGL(k_relax, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_pol_rho, dim3(nrb, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_rho_spline, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_qw2prop, dim3(nwires), dim3(256), 0, s, d_descs);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_vint(const double* y, int32_t Ny, const double* q, int32_t nq, double alpha1, double alpha2,
+                        double Delta0, double prefactor, double* out, void* stream) {
+  if (nq <= 0) return 0;
+  if (Ny < 4) return pss_set_msg(2, "pss_vint: Ny < 4");
+  hipLaunchKernelGGL(k_vint, dim3(nq), dim3(256), 0, (hipStream_t)stream, y, Ny, q, alpha1, alpha2, Delta0,
+                     prefactor, out);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}

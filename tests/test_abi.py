@@ -1,0 +1,60 @@
+"""C ABI checks that need no GPU: the library loads, exports every function that
+include/pss.h declares, and the ctypes mirrors of the descriptor structs have the
+C layout (offsets checked with a tiny gcc program built from the header)."""
+
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "pss.h")
+
+
+def declared():
+    txt = open(HDR).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(pss_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pulsesuite_amd import _lib
+
+    names = declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(_lib.lib, n), n
+    assert set(_lib.EXPORTS) == set(names)
+    assert b"gfx950" in _lib.lib.pss_version()
+
+
+def test_workspace_layout_matches_python_mirror():
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd.SBEs import ws_layout
+
+    for nk, nr, nrr in ((16, 34, 48), (128, 258, 1024), (1024, 2050, 128)):
+        assert _lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr) == 8 * ws_layout(nk, nr, nrr)["total"]
+
+
+@pytest.mark.parametrize("struct", ["pss_wire_desc", "pss_grid"])
+def test_struct_layout_matches_ctypes(struct):
+    from pulsesuite_amd import _lib
+
+    cls = _lib.WireDesc if struct == "pss_wire_desc" else _lib.Grid
+    fields = [f[0] for f in cls._fields_ if not f[0].startswith("pad")]
+    prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', 'int main(void){',
+            f'printf("%zu\\n", sizeof({struct}));']
+    for f in fields:
+        prog.append(f'printf("%zu\\n", offsetof({struct}, {f}));')
+    prog.append("return 0;}")
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "t.c")
+        exe = os.path.join(td, "t")
+        open(src, "w").write("\n".join(prog))
+        subprocess.run(["gcc", src, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(cls)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(cls, f).offset == off, f

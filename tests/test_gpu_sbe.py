@@ -1,0 +1,167 @@
+"""GPU parity of the SBE wire step against the FP64 oracle.
+
+Bar (north star): relative L2 <= 1e-10 on polarisation, populations and the
+propagation-grid outputs, FP64 path.  Inputs are seeded: Hermitian C, D with
+eigenvalues in [0, 0.5], |p| ~ 0.1, a Gaussian-cosine field on rr.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import sbe as O
+
+TOL = 1e-10
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def herm(n, rng, top=0.5):
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Q, _ = np.linalg.qr(X)
+    return (Q * rng.uniform(0, top, n)) @ Q.conj().T
+
+
+def make_pair(nk, mb_kw=None, nrr=48, Nw=1, seed=7):
+    import torch  # noqa: F401
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import MBFlags, QWParams
+
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, nrr)
+    mb_kw = mb_kw or {}
+    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, Nw, O.QWParams(L=L), O.MBFlags(**mb_kw))
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, Nw, True, qw_params=QWParams(L=L), mb_flags=MBFlags(**mb_kw))
+    assert gpu.Nk == ref.Nk == nk
+    rng = np.random.default_rng(seed)
+    for w in range(Nw):
+        st = dict(p1=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+                  p2=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+                  C1=herm(nk, rng), C2=herm(nk, rng), D1=herm(nk, rng), D2=herm(nk, rng))
+        s = ref.state[w]
+        s.p1, s.p2, s.C1, s.C2, s.D1, s.D2 = (st[k].copy() for k in ("p1", "p2", "C1", "C2", "D1", "D2"))
+        gpu.set_state(w + 1, **st)
+    return ref, gpu, rr
+
+
+def field(rr, L, amp=3e7):
+    return (amp * np.exp(-((rr / (1.5 * L)) ** 2)) * np.cos(2.0 * rr / L)).astype(complex)
+
+
+def run_both(ref, gpu, rr, nsteps, dt=2e-17, w=1):
+    import torch
+
+    E = field(rr, ref.L)
+    outs = [np.zeros(len(rr), complex) for _ in range(4)]
+    for _ in range(nsteps):
+        r = ref.QWCalculator(E, 0.5 * E, 0.3 * E, rr, dt, w)
+        gpu.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, dt, w, *outs, [False], [False])
+    torch.cuda.synchronize()
+    return r, outs
+
+
+@pytest.mark.parametrize("nk", [16, 34])
+def test_step_outputs_and_state(nk):
+    ref, gpu, rr = make_pair(nk)
+    r, g = run_both(ref, gpu, rr, 3)
+    for name, a, b in zip(("Pxx", "Pyy", "Pzz", "Rho"), g, r):
+        assert rel(a, b) < TOL, (name, rel(a, b))
+    st = gpu.get_state(1)
+    so = ref.state[0]
+    for k in ("p1", "p2", "C1", "C2", "D1", "D2"):
+        assert rel(st[k], getattr(so, k)) < TOL, (k, rel(st[k], getattr(so, k)))
+
+
+def test_intermediates_one_step():
+    """Screening, Hamiltonians, dephasing and collision rates of one step."""
+    nk = 24
+    ref, gpu, rr = make_pair(nk)
+    so = ref.state[0]
+    C2, D2, p2 = so.C2.copy(), so.D2.copy(), so.p2.copy()
+    E = field(rr, ref.L)
+    Ex, Ey, Ez = ref.Prop2QW(rr, E, 0.5 * E, 0.3 * E)
+    Meh = ref.calc_Meh(Ex, Ey, Ez, 1)
+    ne, nh = np.diag(C2).copy(), np.diag(D2).copy()
+    VC, E1D = ref.CalcScreenedArrays(ne, nh)
+    H = ref.CalcH(Meh, C2, D2, p2, VC)
+    GE, GH = ref.CalcGammaE(ne, nh, VC), ref.CalcGammaH(ne, nh, VC)
+    _, De, Dh = ref.OffDiagDephasing2(ne, nh, p2, VC)
+    run_both(ref, gpu, rr, 1)
+    dev = gpu.dev
+    vc = dev.workspace_view(0, "vc").cpu().numpy().reshape(3, nk)
+    for n in range(3):
+        assert rel(vc[n], VC[0, :, n]) < 1e-13
+    Hd = dev.workspace_view(0, "H").cpu().numpy().view(complex).reshape(3, nk, nk)
+    for a, b in zip(Hd, H):
+        assert rel(a, b) < 1e-12
+    gam = dev.workspace_view(0, "gam").cpu().numpy()
+    assert rel(gam[:nk], GE) < 1e-12 and rel(gam[nk:], GH) < 1e-12
+    assert rel(dev.workspace_view(0, "De").cpu().numpy().reshape(2 * nk + 1, nk), De) < 1e-12
+    assert rel(dev.workspace_view(0, "Dh").cpu().numpy().reshape(2 * nk + 1, nk), Dh) < 1e-12
+
+
+@pytest.mark.parametrize("mb_kw", [
+    dict(Excitons=True, LF=False),
+    dict(Screened=False, DiagDph=False, OffDiagDph=False),
+    dict(EHs=False, Phonon=False),
+    dict(LorentzDelta=True),
+    dict(OBE=True),
+])
+def test_flag_variants(mb_kw):
+    ref, gpu, rr = make_pair(14, mb_kw)
+    r, g = run_both(ref, gpu, rr, 2)
+    for a, b in zip(g, r):
+        assert rel(a, b) < TOL or np.linalg.norm(b) == 0.0
+    st = gpu.get_state(1)
+    so = ref.state[0]
+    for k in ("p2", "C2", "D2"):
+        assert rel(st[k], getattr(so, k)) < TOL
+
+
+def test_two_subband_wires_and_gate():
+    """Nw = 2 (w = 1, 2 signs), starting from the reference's initial state with
+    the wireoff gate live: zero outputs until |E| >= 1e-3 Emax0."""
+    import torch
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import MBFlags, QWParams
+
+    L = O.L_for_nk(18)
+    rr = np.linspace(-3 * L, 3 * L, 40)
+    ref = O.OracleWire(rr, 0.0, 1e8, 800e-9, 2, O.QWParams(L=L), O.MBFlags())
+    gpu = SBESolver(None, rr, 0.0, 1e8, 800e-9, 2, True, qw_params=QWParams(L=L), mb_flags=MBFlags())
+    outs = [[np.zeros(len(rr), complex) for _ in range(4)] for _ in range(2)]
+    for step in range(6):
+        E = field(rr, L, amp=1e4 if step < 2 else 4e7)
+        for w in (1, 2):
+            r = ref.QWCalculator(E, 0.5 * E, 0.3 * E, rr, 2e-17, w)
+            gpu.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, w, *outs[w - 1], [False], [False])
+            torch.cuda.synchronize()
+            for a, b in zip(outs[w - 1], r):
+                if np.linalg.norm(b) == 0:
+                    assert np.linalg.norm(a) == 0
+                else:
+                    assert rel(a, b) < TOL
+    for w in (1, 2):
+        st = gpu.get_state(w)
+        assert rel(st["p2"], ref.state[w - 1].p2) < TOL
+
+
+def test_vint_matches_oracle():
+    import torch
+
+    from pulsesuite_amd import _lib
+
+    L = O.L_for_nk(20)
+    w = O.OracleWire(np.linspace(-L, L, 10), 0.0, 1.0, 800e-9, 1, O.QWParams(L=L), O.MBFlags())
+    y = torch.as_tensor(w.r, device="cuda")
+    q = torch.as_tensor(np.arange(w.Nk) * w.dkr, device="cuda")
+    out = torch.empty(w.Nk, dtype=torch.float64, device="cuda")
+    pref = 1.0
+    _lib.check(_lib.lib.pss_vint(_lib.ptr(y), w.Nr, _lib.ptr(q), w.Nk, w.alphae, w.alphah, w.Delta0, pref,
+                                 _lib.ptr(out), _lib.stream_ptr()), "vint")
+    ref = np.array([O.Vint(qq, w.r, w.alphae, w.alphah, w.Delta0) for qq in np.arange(w.Nk) * w.dkr])
+    assert rel(out.cpu().numpy(), ref) < 1e-13
