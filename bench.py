@@ -1,0 +1,175 @@
+"""Benchmark of the coupled SBE-PSTD time step on one MI355X.
+
+Workload (default): 3-D PSTD on a 128^3 complex grid with a 12-cell CPML on all
+faces, additive soft plane source, spectral curl via 3-D FFTs, and one
+embedded quantum wire (Nk = 256, GaAs parameters, screened = 0 "bare") driven
+by the point field and feeding its current back -- 1000 leapfrog steps.
+
+Metric: k-point.steps/s = (sum over wires of Nk) x executed steps / wall time,
+inputs resident in HBM, timed region bracketed by synchronize on both sides.
+The FP64 form and the mixed form run the same algorithm; the mixed line is
+reported only with its error against the FP64 run.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "k-point·steps/s mixed vs FP64 GPU at ≤1e-6 rel err; p50/p99 solve latency"
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--nk", type=int, default=256)
+    ap.add_argument("--npml", type=int, default=12)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    return ap.parse_args()
+
+
+def setup(grid, nk, npml, precision):
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.rhoPJ import QWArray
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    dx = 20e-9
+    epsr = 9.1
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    sp = ss(3, grid, grid, grid, dx, dx, dx, epsr)
+    pulse = ps(800e-9, 1e8, 100e-15, 50e-15, 0.0)
+    mb = MBFlags(Screened=False)  # bare Coulomb for this config
+    qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, precision=precision)
+    prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type="cpml",
+                            calc_j=qa, npml=npml, check_every=0)
+    return prop, qa
+
+
+def time_steps(prop, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        prop.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prop.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def fft_pass_timing(prop, reps=50):
+    """Average duration of one FP64 3-D FFT (3 LDS radix-2 passes) on this grid,
+    HIP events on the launch stream; algorithmic bytes = 3 passes x (read + write)."""
+    import torch
+
+    from pulsesuite_amd import _lib
+
+    F = prop.Ex.clone()
+    s = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        _lib.lib.pss_fft3d(prop.ctx.gp, _lib.ptr(F), -1, _lib.stream_ptr())
+    ev0.record(s)
+    for _ in range(reps):
+        _lib.check(_lib.lib.pss_fft3d(prop.ctx.gp, _lib.ptr(F), -1, _lib.stream_ptr()), "fft")
+    ev1.record(s)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    n = prop.ctx.n
+    passes = sum(1 for N in prop.ctx.shape if N > 1)
+    bytes_ = passes * 2 * 16 * n
+    return ms, bytes_
+
+
+def cpu_baseline(grid, nk, npml, steps):
+    """The FP64 NumPy oracle on the same workload for `steps` steps (bounded sample)."""
+    from oracle import pstd as OP
+    from oracle.coupling import OracleCoupling
+    from oracle.sbe import L_for_nk, MBFlags, QWParams
+
+    dx = 20e-9
+    epsr = 9.1
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    sp = OP.Space(3, grid, grid, grid, dx, dx, dx, epsr)
+    oc = OracleCoupling(sp, 800e-9, 0.0, Nw=1, qw=QWParams(L=L_for_nk(nk)), mb=MBFlags(Screened=False))
+    prop = OP.Propagator(sp, OP.Time(0.0, 1.0, dt, 0), OP.Pulse(800e-9, 1e8, 100e-15, 50e-15, 0.0), "soft", "cpml",
+                         calc_j=oc.calc_j, npml=npml)
+    prop.step()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prop.step()
+    el = time.perf_counter() - t0
+    return nk * steps / el, el
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    results = {}
+    precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    for prec in precs:
+        prop, qa = setup(a.grid, a.nk, a.npml, prec)
+        el = time_steps(prop, a.steps, a.warmup)
+        results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop)
+    main_prec = precs[-1]
+    r = results[main_prec]
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "k-point*steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128/f64" if main_prec == "fp64" else "mixed",
+        "data": "synthetic (seeded soft-source pulse, reference initial wire state)",
+        "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
+                               f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,
+                   "wires": 1, "precision": main_prec},
+    }
+    if "fp64" in results and "mixed" in results:
+        line["fp64_value"] = results["fp64"]["value"]
+        line["mixed_over_fp64"] = results["mixed"]["value"] / results["fp64"]["value"]
+    if not a.no_kernel_timing:
+        ms, b = fft_pass_timing(r["prop"])
+        ach = b / (ms * 1e-3)
+        line["roofline"] = {"bound": "hbm", "kernel": "k_fft_pass<128> x3 (one 3-D FFT)", "achieved": ach / 1e9,
+                            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                            "avg_ms": ms}
+    if a.cpu_steps > 0:
+        v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps)
+        line["cpu_baseline"] = {"value": v, "unit": "k-point*steps/s", "cores": len(os.sched_getaffinity(0)),
+                                "kind": "port", "sample": f"{a.cpu_steps} steps of the same workload, "
+                                f"{el:.1f} s, NumPy/SciPy FP64 oracle"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -848,6 +848,7 @@ class OracleWire:
         Px, Py, Pz, re, rh = self.sbe_calculator(Ex, Ey, Ez, dt, w)
         Pxx, Pyy, Pzz, RhoE, RhoH = self.QW2Prop(Px, Py, Pz, re, rh, rr)
         Rho = RhoH - RhoE
+        self.last_rho = (RhoE, RhoH)  # scale of the difference, for tolerance checks
         if self.mb.ignorewire:
             return zero, zero.copy(), zero.copy(), zero.copy()
         return Pxx, Pyy, Pzz, Rho

@@ -1,0 +1,183 @@
+"""3-D pseudo-spectral Maxwell propagator on the GPU (drop-in for PSTD3D/PSTD3D.py).
+
+Fields are contiguous complex128 CUDA tensors of shape (Nx, Ny, Nz), kept in
+k-space between half steps as in the reference.  ``UpdateE3D`` / ``UpdateB3D``
+(PSTD3D.py:29-113) update in place; ``PSTD3DPropagator`` (PSTD3D.py:233-858)
+runs the leapfrog loop: E curl update, CPML-E (with the soft source and the
+wire point-sample fused into its real-space pass), t += dt/2, ``calc_j``,
+B curl update, CPML-B (+ source), t += dt/2.
+
+Differences from the reference, on purpose:
+  * no per-step diagnostics IFFTs, Emax.dat, snapshots or summary files; the
+    NaN/overflow guard checks the k-space fields every ``check_every`` steps and
+    raises RuntimeError as the reference does (PSTD3D.py:510-512);
+  * boundary_type "mask" (absorber.py) is not built; "cpml" and "none" are.
+"""
+
+import numpy as np
+import torch
+from scipy.constants import c as _c0
+from scipy.constants import mu_0 as _mu0
+
+from . import _lib
+from ._grid import GridContext
+from .typespace import GetDx, GetEpsr, GetXArray, GetYArray, GetZArray
+
+_twopi = 2.0 * np.pi
+_ctx = {}
+
+
+def _context(space, dt, device):
+    key = (space.Nx, space.Ny, space.Nz, space.dx, space.dy, space.dz, GetEpsr(space), dt, str(device))
+    if key not in _ctx:
+        _ctx[key] = GridContext(space, dt, with_cpml=False, device=device)
+    return _ctx[key]
+
+
+def UpdateE3D(space, time, Bx, By, Bz, Jx, Jy, Jz, Ex, Ey, Ez):
+    """E += i v2 dt (k x B) - mu0 v2 dt J, in place (PSTD3D/PSTD3D.py:29-76)."""
+    ctx = _context(space, time.dt, Ex.device)
+    P = _lib.ptr
+    _lib.check(_lib.lib.pss_update_e3d(ctx.gp, P(Bx), P(By), P(Bz), P(Jx), P(Jy), P(Jz), None, None, None,
+                                       P(Ex), P(Ey), P(Ez), _lib.stream_ptr()), "pss_update_e3d")
+
+
+def UpdateB3D(space, time, Ex, Ey, Ez, Bx, By, Bz):
+    """B -= i dt (k x E), in place (PSTD3D/PSTD3D.py:79-113)."""
+    ctx = _context(space, time.dt, Ex.device)
+    P = _lib.ptr
+    _lib.check(_lib.lib.pss_update_b3d(ctx.gp, P(Ex), P(Ey), P(Ez), P(Bx), P(By), P(Bz), _lib.stream_ptr()),
+               "pss_update_b3d")
+
+
+def SeedInitialCondition(space, time, pulse, Ey, Bz, npml_x=0):
+    """Seed Ey, Bz with the pulse and FFT to k-space (PSTD3D/PSTD3D.py:134-226).
+
+    The seed is evaluated on the host once at set-up; the transform runs on the GPU."""
+    from .fftw import fft_3D
+
+    Nx = space.Nx
+    x, yy, zz = GetXArray(space), GetYArray(space), GetZArray(space)
+    omega0 = pulse.CalcOmega0()
+    tau_G = pulse.CalcTau()
+    v = _c0 / np.sqrt(GetEpsr(space))
+    sigma_x = v * tau_G
+    k_med = omega0 / v
+    xi = (x - x[Nx // 2])[:, None, None]
+    gauss = np.exp(-(yy[None, :, None] ** 2 + zz[None, None, :] ** 2) / pulse.w0**2)
+    E = pulse.Amp * gauss * np.exp(-(xi**2) / sigma_x**2) * np.exp(1j * (k_med * xi + (pulse.chirp / v**2) * xi**2))
+    if npml_x > 0:
+        taper = np.ones(Nx)
+        for i in range(npml_x):
+            taper[i] = 0.5 * (1.0 - np.cos(np.pi * i / npml_x))
+        for i in range(Nx - npml_x, Nx):
+            taper[i] = 0.5 * (1.0 - np.cos(np.pi * (Nx - 1 - i) / npml_x))
+        E = E * taper[:, None, None]
+    Ey.copy_(torch.as_tensor(E, dtype=torch.complex128))
+    Bz.copy_(torch.as_tensor(E / v, dtype=torch.complex128))
+    fft_3D(Ey)
+    fft_3D(Bz)
+
+
+def InitializeTFSF(space, pulse):
+    """Normalised Gaussian source profile, sigma = 5 dx at 25 % of x (PSTD3D/tfsf.py:26-60)."""
+    x = GetXArray(space)
+    dx = GetDx(space)
+    xp = x[int(round(space.Nx * 0.25))]
+    s = 5.0 * dx
+    return np.exp(-0.5 * ((x - xp) / s) ** 2) / (s * np.sqrt(_twopi))
+
+
+class PSTD3DPropagator:
+    """PSTD3D/PSTD3D.py:233-629 on the GPU.  Keyword-only extras: ``npml`` (CPML
+    thickness override), ``check_every`` (NaN guard period), ``device``."""
+
+    def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="cpml", calc_j=None,
+                 output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None):
+        if boundary_type == "mask":
+            raise NotImplementedError("boundary_type='mask' (absorber) is not built on the device; use 'cpml'")
+        if boundary_type not in ("cpml", "none"):
+            raise ValueError(f"unknown boundary_type {boundary_type!r}")
+        if source_type not in ("ic", "soft"):
+            raise ValueError(f"unknown source_type {source_type!r}")
+        self.space, self.time, self.pulse = space, time, pulse
+        self.source_type, self.boundary_type = source_type, boundary_type
+        self.calc_j = calc_j
+        self.check_every = check_every
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        from .cpml import CPML
+
+        self.cpml = CPML(space.Nx, space.Ny, space.Nz, space.dx, space.dy, space.dz, time.dt, GetEpsr(space),
+                         npml=npml if boundary_type == "cpml" else 0, device=self.device)
+        if boundary_type == "none":
+            self.cpml.ctx.g.npml_x = self.cpml.ctx.g.npml_y = self.cpml.ctx.g.npml_z = 0
+        self.ctx = self.cpml.ctx
+        self.v = _c0 / np.sqrt(GetEpsr(space))
+        self.v2 = _c0**2 / GetEpsr(space)
+        f = self.ctx.field
+        self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz = (f() for _ in range(6))
+        self.Jx, self.Jy, self.Jz = (f() for _ in range(3))
+        dev = self.device
+        self._rx = torch.as_tensor(GetXArray(space), device=dev)
+        self._ry = torch.as_tensor(GetYArray(space), device=dev)
+        self._rz = torch.as_tensor(GetZArray(space), device=dev)
+        self._tfsf = None
+        if source_type == "soft":
+            self._tfsf = torch.as_tensor(InitializeTFSF(space, pulse), device=dev)
+        if source_type == "ic":
+            SeedInitialCondition(space, time, pulse, self.Ey, self.Bz, self.cpml.npml_x)
+        if calc_j is not None and hasattr(calc_j, "bind"):
+            calc_j.bind(self)
+        self._nstep = 0
+
+    def _src(self, comp, amp, t):
+        if self._tfsf is None:
+            return None
+        p = self.pulse
+        w0inv2 = 0.0 if not np.isfinite(p.w0) else 1.0 / p.w0**2
+        par = (_lib.C.c_double * 9)(t, amp, p.CalcOmega0(), p.CalcTau(), p.chirp, self.v, p.Tp, w0inv2,
+                                    GetDx(self.space))
+        return (comp, par, self._tfsf, self._rx, self._ry, self._rz)
+
+    def step(self, stream=None):
+        """One leapfrog step (PSTD3D/PSTD3D.py:523-627)."""
+        sp, tm = self.space, self.time
+        g = self.ctx.g
+        g.dt = tm.dt
+        g.v2 = self.v2
+        P = _lib.ptr
+        sep = getattr(self.calc_j, "separable", False)
+        jl, gy, gz = self.calc_j.jsep() if sep else (None, None, None)
+        Jd = (None, None, None) if sep else (self.Jx, self.Jy, self.Jz)
+        st = _lib.stream_ptr(stream)
+        _lib.check(_lib.lib.pss_update_e3d(self.ctx.gp, P(self.Bx), P(self.By), P(self.Bz), *(P(t) for t in Jd),
+                                           P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), st),
+                   "pss_update_e3d")
+        sample = self.calc_j.sample_args() if sep else None
+        self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
+                              src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
+        tm.t += tm.dt / 2.0
+        if self.calc_j is not None:
+            self.calc_j(sp, tm, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+        _lib.check(_lib.lib.pss_update_b3d(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(self.Bx), P(self.By),
+                                           P(self.Bz), st), "pss_update_b3d")
+        self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, dt=tm.dt,
+                              src=self._src(2, self.pulse.Amp / self.v, tm.t), stream=stream)
+        tm.t += tm.dt / 2.0
+        tm.n += 1
+        self._nstep += 1
+        if self.check_every and self._nstep % self.check_every == 0:
+            self.check_finite()
+
+    def check_finite(self):
+        for F in (self.Ex, self.Ey, self.Ez):
+            if not bool(torch.isfinite(torch.view_as_real(F)).all()):
+                raise RuntimeError(f"NaN or overflow at step {self.time.n}")
+
+    def run(self, nsteps=None):
+        n = self.time.CalcNt() + 1 - self.time.n if nsteps is None else nsteps
+        for _ in range(n):
+            self.step()
+
+
+PSTD_3D_Propagator = PSTD3DPropagator

@@ -63,9 +63,19 @@ class DeviceWires:
         self.on = torch.zeros(max(self.group) + 1, dtype=torch.int32, device=dev)
         nrr = [len(m.arrays["sa_x"]) for m in self.models]
         self.nrr = nrr
-        self.E_in = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
-        self.P_out = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
-        self.Rho_out = [torch.zeros(n, dtype=_CDT, device=dev) for n in nrr]
+        if len(set(nrr)) == 1:
+            # one block per quantity so the coupling kernels see [Nw][3][n] / [Nw][n]
+            self.E_all = torch.zeros((Nw, 3, nrr[0]), dtype=_CDT, device=dev)
+            self.P_all = torch.zeros((Nw, 3, nrr[0]), dtype=_CDT, device=dev)
+            self.Rho_all = torch.zeros((Nw, nrr[0]), dtype=_CDT, device=dev)
+            self.E_in = [self.E_all[i] for i in range(Nw)]
+            self.P_out = [self.P_all[i] for i in range(Nw)]
+            self.Rho_out = [self.Rho_all[i] for i in range(Nw)]
+        else:
+            self.E_all = self.P_all = self.Rho_all = None
+            self.E_in = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
+            self.P_out = [torch.zeros((3, n), dtype=_CDT, device=dev) for n in nrr]
+            self.Rho_out = [torch.zeros(n, dtype=_CDT, device=dev) for n in nrr]
         self.ws = []
         self.descs = (_lib.WireDesc * Nw)()
         w_index = w_index or list(range(1, Nw + 1))

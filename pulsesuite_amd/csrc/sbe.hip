@@ -16,7 +16,7 @@
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr double kHbar = 1.054571817e-34;    // scipy.constants.hbar
+constexpr double kHbar = 1.0545718176461565e-34;  // scipy.constants.hbar (h / 2 pi)
 constexpr double kE0 = 1.602176634e-19;      // scipy.constants.e
 
 // ---------------------------------------------------------------- workspace
@@ -593,7 +593,8 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 4; ++v) {
-        const int i = m0 + a * 16 + 4 * (lane >> 4) + v;
+        // v_mfma_f64_16x16x4f64 result: lane l, element v holds row (l/16) + 4v, column l%16
+        const int i = m0 + a * 16 + (lane >> 4) + 4 * v;
         const int j = n0 + b * 16 + (lane & 15);
         if (i >= Nk || j >= Nk) continue;
         cplx S = mk(accr[a][b][v], acci[a][b][v]);

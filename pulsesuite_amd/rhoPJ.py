@@ -1,0 +1,141 @@
+"""Wire <-> grid coupling on the GPU: the ``calc_j`` hook of the coupled step.
+
+Follows QWArray.QuantumWire (PSTD3D/rhoPJ.py:1060-1329) for the current path:
+the wire sees E sampled at one point (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085),
+each sub-band wire w = 1..Nw runs its SBE step on the x line, the current
+J(x) = sum_w dP_w/dt / Nw + CalcJfx (rhoPJ.py:561-592, 1183-1194) is deposited
+as J(x) g_y(y) g_z(z) (QWPlacement, rhoPJ.py:736-781) and enters the E update
+in k-space.  Because the deposit is separable, its 3-D FFT is the product of
+three 1-D FFTs: J_hat(kx,ky,kz) = FFT_x(J)(kx) FFT_y(g_y)(ky) FFT_z(g_z)(kz); the
+E-update kernel forms it on the fly and no J grid is stored or transformed.
+
+The previous polarisation is kept between calls, so J = (P - P_old)/dt as the
+reference's comment describes (its code resets P_old to zero every call,
+rhoPJ.py:1138-1147, which would make J = P/dt).  The longitudinal-field
+diagnostics (ElongfromRho, RhoB*) are not on the J path and are not computed.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from .SBEs import DeviceWires
+from .typespace import GetDx, GetKxArray, GetXArray, GetYArray, GetZArray, ss
+from .wire_model import build_model
+
+
+def _locator(x, x0):
+    i = np.searchsorted(x, x0, side="right") - 1
+    return int(max(0, min(i, len(x) - 2)))
+
+
+def trilinear(shape, rx, ry, rz, x0, y0, z0):
+    """Flat indices and weights of the 8 corners of EAtXYZ (usefulsubs.py:2016-2085)."""
+    axes = []
+    for a, a0 in ((rx, x0), (ry, y0), (rz, z0)):
+        if len(a) == 1:
+            axes.append((0, np.array([1.0, 0.0])))
+        else:
+            i = _locator(a, a0)
+            al = (a0 - a[i]) / (a[i + 1] - a[i])
+            axes.append((i, np.array([1.0 - al, al])))
+    Nx, Ny, Nz = shape
+    idx, w = [], []
+    for r in range(2):
+        for q in range(2):
+            for p in range(2):
+                ii, jj, kk = axes[0][0] + p, axes[1][0] + q, axes[2][0] + r
+                ww = axes[0][1][p] * axes[1][1][q] * axes[2][1][r]
+                if ii < Nx and jj < Ny and kk < Nz and ww != 0.0:
+                    idx.append((ii * Ny + jj) * Nz + kk)
+                    w.append(ww)
+                else:
+                    idx.append(-1)
+                    w.append(0.0)
+    return np.array(idx, dtype=np.int32), np.array(w)
+
+
+class QWArray:
+    """Nw sub-band wires along x through (x0, y0, z0), coupled to a PSTD3DPropagator.
+
+    qw_params / mb_flags configure the wires (see SBEs.SBESolver); Emax0 sets
+    the wireoff gate (1e-3 Emax0).
+    """
+
+    separable = True
+
+    def __init__(self, qw_params, mb_flags, Nw=1, Emax0=1.0, y0=0.0, z0=0.0, x0=0.0, ay=5e-9, az=5e-9,
+                 precision="fp64"):
+        self.qw, self.mb = qw_params, mb_flags
+        self.Nw = Nw
+        self.Emax0 = Emax0
+        self.x0, self.y0, self.z0, self.ay, self.az = x0, y0, z0, ay, az
+        self.precision = precision
+        self.prop = None
+
+    def bind(self, prop):
+        """Allocate device state for propagator ``prop`` (called by PSTD3DPropagator)."""
+        self.prop = prop
+        sp = prop.space
+        dev = prop.device
+        self.rx, self.ry, self.rz = GetXArray(sp), GetYArray(sp), GetZArray(sp)
+        Nx = sp.Nx
+        self.model = build_model(self.qw, self.mb, self.rx)
+        self.wires = DeviceWires(self.model, self.Nw, self.Emax0, prop.time.dt, self.precision, dev)
+        idx, w = trilinear((sp.Nx, sp.Ny, sp.Nz), self.rx, self.ry, self.rz, self.x0, self.y0, self.z0)
+        self.sidx = torch.as_tensor(idx, device=dev)
+        self.sw = torch.as_tensor(w, device=dev)
+        self.sample = torch.zeros(3, dtype=torch.complex128, device=dev)
+        self.P = self.wires.P_all  # (Nw, 3, Nx)
+        self.Rho = self.wires.Rho_all  # (Nw, Nx)
+        self.Pold = torch.zeros_like(self.P)
+        self.RhoOld = torch.zeros_like(self.Rho)
+        self.jl = torch.zeros((3, Nx), dtype=torch.complex128, device=dev)
+        gy = np.exp(-((self.ry - self.y0) ** 2) / (2.0 * self.ay**2)).astype(complex)
+        gz = np.exp(-((self.rz - self.z0) ** 2) / (2.0 * self.az**2)).astype(complex)
+        self.gy = torch.as_tensor(gy, device=dev)
+        self.gz = torch.as_tensor(gz, device=dev)
+        ctx = prop.ctx
+        st = _lib.stream_ptr()
+        # FFT of the transverse gates, once: passes along y and z of (1, Ny, 1) / (1, 1, Nz) views
+        self._gline_y = _line_ctx(1, sp.Ny, 1, dev)
+        self._gline_z = _line_ctx(1, 1, sp.Nz, dev)
+        self._gline_x = _line_ctx(Nx, 1, 1, dev)
+        if sp.Ny > 1:
+            _lib.check(_lib.lib.pss_fft_line(self._gline_y.gp, _lib.ptr(self.gy), _lib.ptr(self.gy), 1, -1, st), "fft gy")
+        if sp.Nz > 1:
+            _lib.check(_lib.lib.pss_fft_line(self._gline_z.gp, _lib.ptr(self.gz), _lib.ptr(self.gz), 2, -1, st), "fft gz")
+        self.dx = GetDx(sp)
+
+    def sample_args(self):
+        return (self.sidx, self.sw, self.sample)
+
+    def jsep(self):
+        return (self.jl, self.gy, self.gz)
+
+    def __call__(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
+        """calc_j hook: wires step on the sampled field, J line -> k-space factors."""
+        st = _lib.stream_ptr()
+        W = self.wires
+        for i in range(self.Nw):
+            _lib.check(_lib.lib.pss_fill_line(_lib.ptr(self.sample), _lib.ptr(W.E_in[i]), space.Nx, st), "fill")
+        W.set_dt(time.dt)
+        W.step()
+        _lib.check(_lib.lib.pss_jline(_lib.ptr(self.P), _lib.ptr(self.Rho), _lib.ptr(self.Pold), _lib.ptr(self.RhoOld),
+                                      self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), st), "jline")
+        if space.Nx > 1:
+            for c in range(3):
+                _lib.check(_lib.lib.pss_fft_line(self._gline_x.gp, _lib.ptr(self.jl[c]), _lib.ptr(self.jl[c]), 0, -1,
+                                                 st), "fft jline")
+
+
+_line_cache = {}
+
+
+def _line_ctx(Nx, Ny, Nz, dev):
+    from ._grid import GridContext
+
+    key = (Nx, Ny, Nz, str(dev))
+    if key not in _line_cache:
+        _line_cache[key] = GridContext(ss(3, Nx, Ny, Nz, 1.0, 1.0, 1.0, 1.0), 1.0, with_cpml=False, device=dev)
+    return _line_cache[key]

@@ -8,7 +8,7 @@ saves inputs/outputs as ``.npz`` fixtures next to this script.  The fixtures
 are data; the reference code itself never travels.
 
 Functions covered (none of them is fixed by a closed form alone):
-  cpml.CPML.ApplyCPML_E / ApplyCPML_B   (36x32x4 grid, seeded random k-space fields)
+  cpml.CPML.ApplyCPML_E / ApplyCPML_B   (32x64x4 grid, seeded random k-space fields)
   tfsf.InitializeTFSF / UpdateTFSC      (same grid)
   PSTD3D.SeedInitialCondition           (same grid, npml_x from CPML)
   PSTD3D.PSTD3DPropagator (cpml + soft) 3 time steps, no-op calc_j
@@ -36,7 +36,7 @@ def main(src):
     from pulsesuite.libpulsesuite import integrator, spliner
 
     rng = np.random.default_rng(1234)
-    Nx, Ny, Nz = 36, 32, 4  # x, y >= 32 get a PML from the reference rule; z stays periodic
+    Nx, Ny, Nz = 32, 64, 4  # x, y >= 32 get a PML from the reference rule; z stays periodic
     dx, dy, dz = 25e-9, 30e-9, 35e-9
     epsr = 2.25
     dt = 0.5 * min(dx, dy, dz) / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))

@@ -140,9 +140,14 @@ def test_two_subband_wires_and_gate():
             r = ref.QWCalculator(E, 0.5 * E, 0.3 * E, rr, 2e-17, w)
             gpu.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, w, *outs[w - 1], [False], [False])
             torch.cuda.synchronize()
-            for a, b in zip(outs[w - 1], r):
+            for k, (a, b) in enumerate(zip(outs[w - 1], r)):
                 if np.linalg.norm(b) == 0:
                     assert np.linalg.norm(a) == 0
+                elif k == 3:
+                    # Rho = RhoH - RhoE cancels to rounding noise while both populations are
+                    # the (identical) Fermi seeds: measure the difference against |RhoE| + |RhoH|
+                    scale = np.linalg.norm(ref.last_rho[0]) + np.linalg.norm(ref.last_rho[1])
+                    assert np.linalg.norm(a - b) <= TOL * scale
                 else:
                     assert rel(a, b) < TOL
     for w in (1, 2):
