@@ -19,6 +19,15 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kHbar = 1.0545718176461565e-34;  // scipy.constants.hbar (h / 2 pi)
 constexpr double kE0 = 1.602176634e-19;      // scipy.constants.e
 
+// n / d via the hardware reciprocal and Newton steps (within 1 ulp of IEEE division).
+__device__ inline double ddiv(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+
 // ---------------------------------------------------------------- workspace
 struct WS {
   int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, total;
@@ -77,41 +86,79 @@ __device__ inline bool wire_on(const pss_wire_desc& d) { return *d.on != 0; }
 // Second-derivative-like coefficients c[] of libpulsesuite/spliner.py:1287-1363
 // for real data y on knots with spacings h, given the y-independent elimination
 // factors t[] and pivots b[] (precomputed on the host with the same operations).
-// n >= 4.  Sequential recurrence: one thread per field.
-__device__ void spline_solve(const double* y, int ystride, int n, const double* h, const double* t,
-                             const double* b, const double* e, double* c) {
-  // interior second differences  c[i] = D_i - D_{i-1}
-  double Dprev = (y[ystride] - y[0]) / h[0];
-  double c1 = 0.0, c2 = 0.0, cm2 = 0.0, cm3 = 0.0;
-  for (int i = 1; i < n - 1; ++i) {
-    const double Di = (y[(int64_t)(i + 1) * ystride] - y[(int64_t)i * ystride]) / h[i];
-    const double ci = Di - Dprev;
-    c[i] = ci;
-    if (i == 1) c1 = ci;
-    if (i == 2) c2 = ci;
-    if (i == n - 2) cm2 = ci;
-    if (i == n - 3) cm3 = ci;
-    Dprev = Di;
+// n >= 4.  Run by one full wavefront: the two first-order recurrences (forward
+// elimination, back substitution) are split into per-lane chunks joined by an
+// affine-map scan across the 64 lanes; inside a chunk each value is computed with
+// the reference's own expression from its predecessor.
+__device__ inline double sdiff(const double* __restrict__ y, int ys, const double* __restrict__ h, int i) {
+  return (y[(int64_t)(i + 1) * ys] - y[(int64_t)i * ys]) / h[i];
+}
+
+__device__ void spline_solve_wave(const double* __restrict__ y, int ys, int n, const double* __restrict__ h,
+                                  const double* __restrict__ t, const double* __restrict__ b,
+                                  const double* __restrict__ e, double* __restrict__ c) {
+  const int lane = threadIdx.x & 63;
+  const int m = (n + 63) / 64;
+  const int lo = min(n, lane * m), hi = min(n, lo + m);
+  // right-hand side: interior second differences, FMM end conditions
+  // e = {x3-x1, x2-x0, h0^2, x3-x0, x[n-1]-x[n-3], x[n-2]-x[n-4], h[n-2]^2, x[n-1]-x[n-4]}
+  for (int i = lo; i < hi; ++i) {
+    double r;
+    if (i == 0) {
+      const double c1 = sdiff(y, ys, h, 1) - sdiff(y, ys, h, 0);
+      const double c2 = sdiff(y, ys, h, 2) - sdiff(y, ys, h, 1);
+      r = c2 / e[0] - c1 / e[1];
+      r = r * e[2] / e[3];
+    } else if (i == n - 1) {
+      const double cm2 = sdiff(y, ys, h, n - 2) - sdiff(y, ys, h, n - 3);
+      const double cm3 = sdiff(y, ys, h, n - 3) - sdiff(y, ys, h, n - 4);
+      r = cm2 / e[4] - cm3 / e[5];
+      r = -r * e[6] / e[7];
+    } else {
+      r = sdiff(y, ys, h, i) - sdiff(y, ys, h, i - 1);
+    }
+    c[i] = r;
   }
-  // end conditions: e = {x3-x1, x2-x0, h0^2, x3-x0, x[n-1]-x[n-3], x[n-2]-x[n-4], h[n-2]^2, x[n-1]-x[n-4]}
-  double c0 = c2 / e[0] - c1 / e[1];
-  double cn = cm2 / e[4] - cm3 / e[5];
-  c0 = c0 * e[2] / e[3];
-  cn = -cn * e[6] / e[7];
-  c[0] = c0;
-  c[n - 1] = cn;
-  // forward elimination
-  double cp = c0;
-  for (int i = 1; i < n; ++i) {
-    cp = c[i] - t[i] * cp;
-    c[i] = cp;
+  // forward elimination x_i = r_i - t_i x_{i-1} (x_0 = r_0)
+  double A = 1.0, B = 0.0;
+  for (int i = lo; i < hi; ++i) {
+    const double a = i == 0 ? 0.0 : -t[i];
+    B = a * B + c[i];
+    A = a * A;
   }
-  // back substitution
-  double cn1 = c[n - 1] / b[n - 1];
-  c[n - 1] = cn1;
-  for (int i = n - 2; i >= 0; --i) {
-    cn1 = (c[i] - h[i] * cn1) / b[i];
-    c[i] = cn1;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ap = __shfl_up(A, off, 64), Bp = __shfl_up(B, off, 64);
+    if (lane >= off) {
+      B = A * Bp + B;
+      A = A * Ap;
+    }
+  }
+  double x = __shfl_up(B, 1, 64);
+  if (lane == 0) x = 0.0;
+  for (int i = lo; i < hi; ++i) {
+    x = i == 0 ? c[0] : c[i] - t[i] * x;
+    c[i] = x;
+  }
+  // back substitution x_i = (x_i - h_i x_{i+1}) / b_i (x_{n-1} = x_{n-1} / b_{n-1})
+  A = 1.0;
+  B = 0.0;
+  for (int i = hi - 1; i >= lo; --i) {
+    const double a = i == n - 1 ? 0.0 : -h[i] / b[i];
+    B = a * B + c[i] / b[i];
+    A = a * A;
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ap = __shfl_down(A, off, 64), Bp = __shfl_down(B, off, 64);
+    if (lane + off < 64) {
+      B = A * Bp + B;
+      A = A * Ap;
+    }
+  }
+  x = __shfl_down(B, 1, 64);
+  if (lane == 63) x = 0.0;
+  for (int i = hi - 1; i >= lo; --i) {
+    x = i == n - 1 ? c[i] / b[i] : (c[i] - h[i] * x) / b[i];
+    c[i] = x;
   }
 }
 
@@ -156,9 +203,10 @@ __global__ void __launch_bounds__(256) k_prop_solve(const pss_wire_desc* __restr
   }
   __syncthreads();
   if (!wire_on(d)) return;
-  if (threadIdx.x < 3) {
-    const double* y = (threadIdx.x == 0 ? d.Exx : threadIdx.x == 1 ? d.Eyy : d.Ezz);
-    spline_solve(y, 2, d.nrr, d.sa_h, d.sa_t, d.sa_b, d.sa_end, ws + L.spa + threadIdx.x * d.nrr);
+  const int wv = threadIdx.x >> 6;
+  if (wv < 3) {
+    const double* y = (wv == 0 ? d.Exx : wv == 1 ? d.Eyy : d.Ezz);
+    spline_solve_wave(y, 2, d.nrr, d.sa_h, d.sa_t, d.sa_b, d.sa_end, ws + L.spa + wv * d.nrr);
   }
 }
 
@@ -337,21 +385,38 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
   const pss_wire_desc& d = descs[blockIdx.z];
   if (!wire_on(d) || !(d.flags & PSS_OFFDIAGDPH)) return;
   const int Nk = d.Nk;
-  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)(2 * Nk + 1) * Nk) return;
-  const int qi = (int)(idx / Nk), k = (int)(idx % Nk);
-  const int q = qi - Nk;
+  const int64_t idx0 = (int64_t)blockIdx.x * 256;
+  if (idx0 >= (int64_t)(2 * Nk + 1) * Nk) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const double* vc = ws + L.vc;
   const bool holes = blockIdx.y == 1;
-  const double* nA = ws + (holes ? L.nh2 : L.ne2);
-  const double* nB = ws + (holes ? L.ne2 : L.nh2);
-  const double* EA = holes ? d.Eh : d.Ee;
-  const double* EB = holes ? d.Ee : d.Eh;
+  extern __shared__ double dyn[];  // 4 * Nk doubles
+  double* sA = dyn;
+  double* sB = dyn + Nk;
+  double* sEA = dyn + 2 * Nk;
+  double* sEB = dyn + 3 * Nk;
+  {
+    const double* nA = ws + (holes ? L.nh2 : L.ne2);
+    const double* nB = ws + (holes ? L.ne2 : L.nh2);
+    const double* EA = holes ? d.Eh : d.Ee;
+    const double* EB = holes ? d.Ee : d.Eh;
+    for (int k = threadIdx.x; k < Nk; k += 256) {
+      sA[k] = cabs(mk(nA[2 * k], nA[2 * k + 1]));
+      sB[k] = cabs(mk(nB[2 * k], nB[2 * k + 1]));
+      sEA[k] = EA[k];
+      sEB[k] = EB[k];
+    }
+  }
+  __syncthreads();
+  const int64_t idx = idx0 + threadIdx.x;
+  if (idx >= (int64_t)(2 * Nk + 1) * Nk) return;
+  const int qi = (int)(idx / Nk), k = (int)(idx % Nk);
+  const int q = qi - Nk;
   // g = [gam_eh, gam_e, gam_h]; E2 gets (g0, g2), H2 gets (g1, g2)  (dephasing.py:735-736)
   const double bAA = kHbar * (holes ? d.gam_e : d.gam_eh);
   const double bAB = kHbar * d.gam_h;
+  const double pAA = bAA / kPi, pAB = bAB / kPi, bAA2 = bAA * bAA, bAB2 = bAB * bAB;
   const int aq = q < 0 ? -q : q;
   double vAA = 0.0, vAB = 0.0;
   if (aq >= 1 && aq <= Nk - 1) {
@@ -362,15 +427,15 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
   double acc = 0.0;
   const int kq = k + q;
   if (kq >= 0 && kq < Nk) {
-    const double nk = cabs(mk(nA[2 * k], nA[2 * k + 1]));
-    const double EAk = EA[k], EAkq = EA[kq];
+    const double nk = sA[k];
+    const double EAk = sEA[k], EAkq = sEA[kq];
     if (vAA != 0.0) {
       const int p0 = max(q + 1, 0), p1 = min(q + Nk - 1, Nk - 1);
       for (int p = p0; p <= p1; ++p) {
         const int pm = p - q;
-        const double npm = cabs(mk(nA[2 * pm], nA[2 * pm + 1])), np_ = cabs(mk(nA[2 * p], nA[2 * p + 1]));
-        const double Ed = EAkq + EA[pm] - EA[p] - EAk;
-        const double lr = (bAA / kPi) / (Ed * Ed + bAA * bAA);
+        const double npm = sA[pm], np_ = sA[p];
+        const double Ed = EAkq + sEA[pm] - sEA[p] - EAk;
+        const double lr = ddiv(pAA, Ed * Ed + bAA2);
         acc += vAA * lr * (npm * (1.0 - np_) * (1.0 - nk) + (1.0 - npm) * np_ * nk);
       }
     }
@@ -378,9 +443,9 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
       const int p0 = max(-q, 0), p1 = min(Nk - 1 - q, Nk - 1);
       for (int p = p0; p <= p1; ++p) {
         const int pp = p + q;
-        const double npp = cabs(mk(nB[2 * pp], nB[2 * pp + 1])), np_ = cabs(mk(nB[2 * p], nB[2 * p + 1]));
-        const double Ed = EAkq + EB[pp] - EB[p] - EAk;
-        const double lr = (bAB / kPi) / (Ed * Ed + bAB * bAB);
+        const double npp = sB[pp], np_ = sB[p];
+        const double Ed = EAkq + sEB[pp] - sEB[p] - EAk;
+        const double lr = ddiv(pAB, Ed * Ed + bAB2);
         acc += vAB * lr * (npp * (1.0 - np_) * (1.0 - nk) + (1.0 - npp) * np_ * nk);
       }
     }
@@ -514,7 +579,10 @@ __device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, cons
   }
 }
 
-__global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict__ descs) {
+// 4 wavefronts per 32x32 output tile; the (term, 16-wide k chunk) work items are
+// dealt round-robin to the waves, each with its own LDS staging, and the four
+// partial tiles are summed in a fixed order (deterministic) in the epilogue.
+__global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs) {
   const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d)) return;
@@ -528,9 +596,13 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
   const cplx* H = (const cplx*)(ws + L.H);
   const double* Dh = ws + L.Dh;
   const double* De = ws + L.De;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
-  __shared__ double Ar[16][32], Ai[16][32], Br[16][32], Bi[16][32];
+  __shared__ double stage[4][4][16][32];  // [wave][Ar, Ai, Br, Bi][k][m or n]
+  double (*Ar)[32] = stage[wv][0];
+  double (*Ai)[32] = stage[wv][1];
+  double (*Br)[32] = stage[wv][2];
+  double (*Bi)[32] = stage[wv][3];
   v4d accr[2][2], acci[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) {
@@ -538,14 +610,20 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
       acci[a][b] = v4d{0, 0, 0, 0};
     }
   const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
-  for (int t = 0; t < nterms; ++t) {
-    const Term T = kTerms[out][t];
-    const cplx* Ap = src_ptr(T.a, B, H, NN);
-    const cplx* Bp = src_ptr(T.b, B, H, NN);
-    const double sg = T.neg ? -1.0 : 1.0;
-    const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
-    const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
-    for (int k0 = 0; k0 < Nk; k0 += 16) {
+  const int nkc = (Nk + 15) / 16;
+  const int nitems = nterms * nkc;
+  const int niter = (nitems + 3) / 4;
+  for (int it = 0; it < niter; ++it) {
+    const int item = it * 4 + wv;
+    const bool has = item < nitems;
+    if (has) {
+      const int t = item / nkc, k0 = (item % nkc) * 16;
+      const Term T = kTerms[out][t];
+      const cplx* Ap = src_ptr(T.a, B, H, NN);
+      const cplx* Bp = src_ptr(T.b, B, H, NN);
+      const double sg = T.neg ? -1.0 : 1.0;
+      const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
+      const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
       // A tile: 32 (m) x 16 (k), coalesced along the contiguous global index
       for (int e = lane; e < 512; e += 64) {
         int mm, kk;
@@ -562,7 +640,9 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
         Br[kk][nn] = v.x;
         Bi[kk][nn] = v.y;
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    if (has) {
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
@@ -582,36 +662,49 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
             acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
           }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
-  // epilogue
-  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
-  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
-  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
-  const double* gam = ws + L.gam;
+  // partial tiles to LDS: wave wv writes its 32x32 complex tile into stage[wv]
+  double* P = &stage[wv][0][0][0];  // 2048 doubles = 32 x 32 complex
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 4; ++v) {
         // v_mfma_f64_16x16x4f64 result: lane l, element v holds row (l/16) + 4v, column l%16
-        const int i = m0 + a * 16 + (lane >> 4) + 4 * v;
-        const int j = n0 + b * 16 + (lane & 15);
-        if (i >= Nk || j >= Nk) continue;
-        cplx S = mk(accr[a][b][v], acci[a][b][v]);
-        const int64_t o = (int64_t)i * Nk + j;
-        if (out == 0) {
-          // + Heh^T[kh=i, ke=j] - i hbar (GamE[ke] + GamH[kh]) p[kh, ke]
-          const cplx hT = H[(int64_t)j * Nk + i];
-          const cplx pv = B.p2[o];
-          const double g = kHbar * (gam[j] + gam[Nk + i]);
-          S = S + hT - mk(-g * pv.y, g * pv.x);
-        }
-        const cplx R = div_ih(S, kHbar);
-        const cplx x1 = X1[o];
-        const cplx x3 = x1 + (R * d.dt) * 2.0;
-        Xn[o] = x3;
-        if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+        const int i = a * 16 + (lane >> 4) + 4 * v;
+        const int j = b * 16 + (lane & 15);
+        P[2 * (i * 32 + j)] = accr[a][b][v];
+        P[2 * (i * 32 + j) + 1] = acci[a][b][v];
       }
+  __syncthreads();
+  // epilogue: 4 elements per thread
+  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
+  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
+  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
+  const double* gam = ws + L.gam;
+  for (int q = threadIdx.x; q < 1024; q += 256) {
+    const int li = q >> 5, lj = q & 31;
+    const int i = m0 + li, j = n0 + lj;
+    if (i >= Nk || j >= Nk) continue;
+    cplx S = mk(0, 0);
+    for (int w = 0; w < 4; ++w) {
+      const double* Pw = &stage[w][0][0][0];
+      S = S + mk(Pw[2 * q], Pw[2 * q + 1]);
+    }
+    const int64_t o = (int64_t)i * Nk + j;
+    if (out == 0) {
+      // + Heh^T[kh=i, ke=j] - i hbar (GamE[ke] + GamH[kh]) p[kh, ke]
+      const cplx hT = H[(int64_t)j * Nk + i];
+      const cplx pv = B.p2[o];
+      const double g = kHbar * (gam[j] + gam[Nk + i]);
+      S = S + hT - mk(-g * pv.y, g * pv.x);
+    }
+    const cplx R = div_ih(S, kHbar);
+    const cplx x1 = X1[o];
+    const cplx x3 = x1 + (R * d.dt) * 2.0;
+    Xn[o] = x3;
+    if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+  }
 }
 
 // --------------------------- K7: carrier-carrier + phonon collision rates
@@ -622,27 +715,24 @@ __global__ void __launch_bounds__(64) k_rhs_gemm(const pss_wire_desc* __restrict
 struct CollCtx {
   const double *Ee, *Eh, *ne, *nh;  // ne/nh here: 1-based padded |Re n3|, index 0 = 0
   int Nk;
-  double heh, hee, hhh;             // hbar*gam_eh, hbar*gam_e, hbar*gam_h
-  double geh, gee, ghh;
+  double hw2[3], ihw[3], pre[3];    // per kind (eh, ee, hh): (hbar g)^2, 1/(hbar g), prefactor
   bool lorentz;
 };
 
 // value read as C_kind[a, b, c] (0-based read indices); kind 0 = eh, 1 = ee, 2 = hh
+// GaussDelta: (2 pi/hbar) exp(-(E/hw)^2) / (sqrt(pi) hw); Lorentz: 2 g / (E^2 + hw^2)
 __device__ inline double coll(const CollCtx& x, int kind, int a, int b, int c) {
   const int A = a - 1, Bq = b - 1, Cq = c - 1;
   if (A < 0 || Bq < 0 || Cq < 0) return 0.0;
-  int k30;
-  const double *E1, *E2;
-  double hw, g;
-  if (kind == 0) { k30 = Cq + Bq - A; E1 = x.Ee; E2 = x.Eh; hw = x.heh; g = x.geh; }
-  else if (kind == 1) { k30 = A + Bq - Cq; E1 = x.Ee; E2 = x.Ee; hw = x.hee; g = x.gee; }
-  else { k30 = A + Bq - Cq; E1 = x.Eh; E2 = x.Eh; hw = x.hhh; g = x.ghh; }
+  const int k30 = kind == 0 ? Cq + Bq - A : A + Bq - Cq;
   if (k30 < 0 || k30 >= x.Nk) return 0.0;
   if (A == Cq || Bq == k30) return 0.0;
+  const double* E1 = kind == 2 ? x.Eh : x.Ee;
+  const double* E2 = kind == 0 ? x.Eh : E1;
   const double Ed = E1[A] + E2[Bq] - E2[k30] - E1[Cq];
-  if (x.lorentz) return 2.0 * g / (Ed * Ed + hw * hw);
-  const double z = Ed / hw;
-  return (2.0 * kPi / kHbar) * (1.0 / (sqrt(kPi) * hw) * exp(-(z * z)));
+  if (x.lorentz) return ddiv(x.pre[kind], Ed * Ed + x.hw2[kind]);
+  const double z = Ed * x.ihw[kind];
+  return x.pre[kind] * exp(-(z * z));
 }
 
 __global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__ descs) {
@@ -653,22 +743,40 @@ __global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__
   if (kk >= Nk) return;
   const bool holes = blockIdx.y == 1;
   __shared__ double sh[16];
-  __shared__ double nsh[2][4097];
+  extern __shared__ double dyn[];  // 2 (Nk + 1) + 2 Nk doubles
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
   // populations from the new diagonal (C3, D3 before relaxation)
+  double* sne = dyn;
+  double* snh = dyn + (Nk + 1);
+  double* sEe = dyn + 2 * (Nk + 1);
+  double* sEh = sEe + Nk;
   for (int k = threadIdx.x; k <= Nk; k += 256) {
-    if (k == 0) { nsh[0][0] = 0.0; nsh[1][0] = 0.0; continue; }
-    nsh[0][k] = fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x);
-    nsh[1][k] = fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    if (k < Nk) {
+      sEe[k] = d.Ee[k];
+      sEh[k] = d.Eh[k];
+    }
+    if (k == 0) { sne[0] = 0.0; snh[0] = 0.0; continue; }
+    sne[k] = fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    snh[k] = fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x);
   }
   __syncthreads();
-  const double* ne = nsh[0];
-  const double* nh = nsh[1];
+  const double* ne = sne;
+  const double* nh = snh;
   const double* vc = ws + L.vc;
-  CollCtx x{d.Ee, d.Eh, ne, nh, Nk, kHbar * d.gam_eh, kHbar * d.gam_e, kHbar * d.gam_h,
-            d.gam_eh, d.gam_e, d.gam_h, (d.flags & PSS_LORENTZDELTA) != 0};
+  CollCtx x;
+  x.Ee = sEe; x.Eh = sEh; x.ne = ne; x.nh = nh; x.Nk = Nk;
+  x.lorentz = (d.flags & PSS_LORENTZDELTA) != 0;
+  {
+    const double g[3] = {d.gam_eh, d.gam_e, d.gam_h};
+    for (int c = 0; c < 3; ++c) {
+      const double hw = kHbar * g[c];
+      x.hw2[c] = hw * hw;
+      x.ihw[c] = 1.0 / hw;
+      x.pre[c] = x.lorentz ? 2.0 * g[c] : (2.0 * kPi / kHbar) * (1.0 / (sqrt(kPi) * hw));
+    }
+  }
   double win = 0.0, wout = 0.0;
   if (d.flags & PSS_EHS) {
     const int64_t npair = (int64_t)Nk * Nk;
@@ -849,45 +957,50 @@ __global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restri
 }
 
 // P(r), rho(r) on the wire grid (before the centred FFT pair, which cancels).
+// One wavefront per grid point r_j: lanes split the 2Nk-1 diagonals.
 __global__ void __launch_bounds__(256) k_pol_rho(const pss_wire_desc* __restrict__ descs) {
   const pss_wire_desc& d = descs[blockIdx.y];
   if (!wire_on(d)) return;
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= d.Nr) return;
+  const int lane = threadIdx.x & 63;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const cplx* ds = (const cplx*)(ws + L.ds);
   const double a = d.dkr * d.r[j];
-  cplx sx = mk(0, 0), sp = mk(0, 0), sc = mk(0, 0), sd = mk(0, 0);
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const bool lf = d.flags & PSS_LF, opt = d.flags & PSS_OPTICS;
-  for (int od = 0; od < nd; ++od) {
+  for (int od = lane; od < nd; od += 64) {
     double sn, cs;
     sincos((double)(od - (Nk - 1)) * a, &sn, &cs);
     const cplx e = mk(cs, sn);
     if (opt) {
-      sx = sx + ds[od] * e;
-      sp = sp + ds[nd + od] * e;
+      const cplx x = ds[od] * e, p = ds[nd + od] * e;
+      v[0] += x.x; v[1] += x.y; v[2] += p.x; v[3] += p.y;
     }
     if (lf) {
-      sc = sc + ds[2 * nd + od] * e;
-      sd = sd + ds[3 * nd + od] * e;
+      const cplx c = ds[2 * nd + od] * e, q = ds[3 * nd + od] * e;
+      v[4] += c.x; v[5] += c.y; v[6] += q.x; v[7] += q.y;
     }
   }
+  for (int q = 0; q < 8; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  if (lane != 0) return;
   const double w = d.win[j];
   const double f = 2 * d.ehint / d.area / d.L;
-  ws[L.pr + j] = opt ? (sx.x * d.dcv) * w * f : 0.0;
-  ws[L.pr + d.Nr + j] = opt ? (sp.x * d.dcv) * w * d.yw * f : 0.0;
-  ws[L.pr + 2 * d.Nr + j] = opt ? (sp.x * -d.dcv) * w * d.zsign * f : 0.0;
+  ws[L.pr + j] = opt ? (v[0] * d.dcv) * w * f : 0.0;
+  ws[L.pr + d.Nr + j] = opt ? (v[2] * d.dcv) * w * d.yw * f : 0.0;
+  ws[L.pr + 2 * d.Nr + j] = opt ? (v[2] * -d.dcv) * w * d.zsign * f : 0.0;
   cplx* rho = (cplx*)(ws + L.rho);
-  rho[j] = lf ? sc * (w / (2.0 * d.L)) : mk(0, 0);
-  rho[d.Nr + j] = lf ? sd * (w / (2.0 * d.L)) : mk(0, 0);
+  rho[j] = lf ? mk(v[4], v[5]) * (w / (2.0 * d.L)) : mk(0, 0);
+  rho[d.Nr + j] = lf ? mk(v[6], v[7]) * (w / (2.0 * d.L)) : mk(0, 0);
 }
 
 // QWRho5 edge removal + normalisation (qwoptics.py:1065-1077), the 2 e0 ehint/area
 // scale (SBEs.py:5764-5765), QW2Prop plasmonic normalisation (qwoptics.py:784-789)
 // and the five spline solves on r for QW2Prop.
-__global__ void __launch_bounds__(256) k_rho_spline(const pss_wire_desc* __restrict__ descs) {
+__global__ void __launch_bounds__(320) k_rho_spline(const pss_wire_desc* __restrict__ descs) {
   const pss_wire_desc& d = descs[blockIdx.x];
   if (!wire_on(d)) return;
   __shared__ double sh[16];
@@ -902,20 +1015,20 @@ __global__ void __launch_bounds__(256) k_rho_spline(const pss_wire_desc* __restr
     for (int s = 0; s < 2; ++s) {
       cplx* R = rho + s * Nr;
       double edge = 0.0;
-      for (int j = threadIdx.x; j < 10; j += 256) edge += R[j].x + R[Nr - 10 + j].x;
-      edge = block_sum<256>(edge, sh);
+      for (int j = threadIdx.x; j < 10; j += 320) edge += R[j].x + R[Nr - 10 + j].x;
+      edge = block_sum<320>(edge, sh);
       const double sub = edge / 20.0;
       double sa = 0.0;
-      for (int j = threadIdx.x; j < Nr; j += 256) {
+      for (int j = threadIdx.x; j < Nr; j += 320) {
         R[j].x -= sub;
         sa += cabs(R[j]);
       }
-      sa = block_sum<256>(sa, sh);
+      sa = block_sum<320>(sa, sh);
       double ntot = 0.0;
-      for (int k = threadIdx.x; k < Nk; k += 256) ntot += fabs(ws[L.n3 + s * Nk + k]);
-      ntot = block_sum<256>(ntot, sh) + 1e-50;
+      for (int k = threadIdx.x; k < Nk; k += 320) ntot += fabs(ws[L.n3 + s * Nk + k]);
+      ntot = block_sum<320>(ntot, sh) + 1e-50;
       const double sc = ntot / (sa * dr + 1e-100);
-      for (int j = threadIdx.x; j < Nr; j += 256) {
+      for (int j = threadIdx.x; j < Nr; j += 320) {
         cplx v = R[j] * sc;
         v = mk(2.0 * kE0 * v.x / d.area * d.ehint, 2.0 * kE0 * v.y / d.area * d.ehint);
         R[j] = v;
@@ -924,30 +1037,29 @@ __global__ void __launch_bounds__(256) k_rho_spline(const pss_wire_desc* __restr
     __syncthreads();
     // QW2Prop plasmonic step: re = |re|, normalise both to the common total
     double s0 = 0.0, s1 = 0.0;
-    for (int j = threadIdx.x; j < Nr; j += 256) {
+    for (int j = threadIdx.x; j < Nr; j += 320) {
       const double a = cabs(rho[j]), b = cabs(rho[Nr + j]);
       ws[L.spb + 3 * Nr + j] = a;  // temporary
       ws[L.spb + 4 * Nr + j] = b;
       s0 += a;
       s1 += b;
     }
-    s0 = block_sum<256>(s0, sh);
-    s1 = block_sum<256>(s1, sh);
+    s0 = block_sum<320>(s0, sh);
+    s1 = block_sum<320>(s1, sh);
     const double total = (s0 * dr + s1 * dr + 1e-100) / 2.0;
     const double f0 = total / (s0 * dr + 1e-100), f1 = total / (s1 * dr + 1e-100);
-    for (int j = threadIdx.x; j < Nr; j += 256) {
+    for (int j = threadIdx.x; j < Nr; j += 320) {
       rho[j] = mk(ws[L.spb + 3 * Nr + j] * f0, 0.0);
       rho[Nr + j] = mk(ws[L.spb + 4 * Nr + j] * f1, 0.0);
     }
     if (threadIdx.x == 0) ws[L.red + 1] = total;
   }
   __syncthreads();
-  if (threadIdx.x < 5) {
-    const int f = threadIdx.x;
-    if (f >= 3 && !lf) return;
+  const int f = threadIdx.x >> 6;  // one wavefront per field
+  if (f < 3 || lf) {
     const double* y = f < 3 ? (y5 + f * Nr) : (const double*)(rho + (f - 3) * Nr);
     const int ys = f < 3 ? 1 : 2;
-    spline_solve(y, ys, Nr, d.sb_h, d.sb_t, d.sb_b, d.sb_end, ws + L.spb + f * Nr);
+    spline_solve_wave(y, ys, Nr, d.sb_h, d.sb_t, d.sb_b, d.sb_end, ws + L.spb + f * Nr);
   }
 }
 
@@ -1064,6 +1176,12 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
     return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
   hipStream_t s = (hipStream_t)stream;
   const int nd = 2 * max_Nk - 1;
+  const size_t lds_off = 4 * (size_t)max_Nk * sizeof(double);
+  const size_t lds_rel = (4 * (size_t)max_Nk + 2) * sizeof(double);
+  if (lds_rel > 65536) {  // gfx950 gives a workgroup up to 160 KB of LDS on request
+    PSS_CHECK(hipFuncSetAttribute((const void*)k_relax, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
+    PSS_CHECK(hipFuncSetAttribute((const void*)k_offdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_off));
+  }
   const int nrb = (max_Nr + 255) / 256;
   hipLaunchKernelGGL(k_prop_solve, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prop_eval, dim3(nrb, nwires), dim3(256), 0, s, d_descs);
@@ -1071,16 +1189,16 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_prep, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
   const int64_t nq = (int64_t)(2 * max_Nk + 1) * max_Nk;
-  hipLaunchKernelGGL(k_offdiag, dim3((unsigned)((nq + 255) / 256), 2, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_offdiag, dim3((unsigned)((nq + 255) / 256), 2, nwires), dim3(256), lds_off, s, d_descs);
   const int t16 = (max_Nk + 15) / 16;
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   const int t32 = (max_Nk + 31) / 32;
-  hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(64), 0, s, d_descs);
-  hipLaunchKernelGGL(k_relax, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_relax, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
   hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_pol_rho, dim3(nrb, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_rho_spline, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_rho_spline, dim3(nwires), dim3(320), 0, s, d_descs);
   hipLaunchKernelGGL(k_qw2prop, dim3(nwires), dim3(256), 0, s, d_descs);
   PSS_CHECK(hipGetLastError());
   return 0;
