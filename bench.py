@@ -76,7 +76,7 @@ def time_steps(prop, steps, warmup):
 
 
 def fft_pass_timing(prop, reps=50):
-    """Average duration of one FP64 3-D FFT (3 LDS radix-2 passes) on this grid,
+    """Average duration of one FP64 3-D FFT (3 two-level register/LDS passes) on this grid,
     HIP events on the launch stream; algorithmic bytes = 3 passes x (read + write)."""
     import torch
 
@@ -160,7 +160,7 @@ def main():
     if not a.no_kernel_timing:
         ms, b = fft_pass_timing(r["prop"])
         ach = b / (ms * 1e-3)
-        line["roofline"] = {"bound": "hbm", "kernel": "k_fft_pass<128> x3 (one 3-D FFT)", "achieved": ach / 1e9,
+        line["roofline"] = {"bound": "hbm", "kernel": "k_fft2<8,16> x3 (one 3-D FFT)", "achieved": ach / 1e9,
                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
                             "avg_ms": ms}
     if a.cpu_steps > 0:

@@ -164,6 +164,154 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
   }
 }
 
+// In-register radix-2 DFT of R points (natural order in and out); twiddles
+// W_{2h}^j = tw[j * (N / 2h)] from the length-N table in LDS.
+template <int R, int N>
+__device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
+  constexpr int LOG = __builtin_ctz(R);
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int r = (int)(__builtin_bitreverse32((unsigned)i) >> (32 - LOG));
+    if (r > i) {
+      const cplx tmp = v[i];
+      v[i] = v[r];
+      v[r] = tmp;
+    }
+  }
+#pragma unroll
+  for (int h = 1; h < R; h <<= 1) {
+#pragma unroll
+    for (int g0 = 0; g0 < R; g0 += 2 * h) {
+#pragma unroll
+      for (int j = 0; j < h; ++j) {
+        const cplx w = tw[j * (N / (2 * h))];
+        const cplx x0 = v[g0 + j], x1 = v[g0 + j + h] * w;
+        v[g0 + j] = x0 + x1;
+        v[g0 + j + h] = x0 - x1;
+      }
+    }
+  }
+}
+
+// Two-level transform N = N1 x N2 along `axis`: each line is read straight from
+// HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
+// twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
+// per element instead of log2(N) staged radix-2 sweeps.
+template <int N1, int N2>
+__global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
+  constexpr int N = N1 * N2;
+  constexpr int G = N2;
+  constexpr int TB0 = 256 / G, TBL = 2048 / N;
+  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  __shared__ cplx Y[TB * N1 * (N2 + 1)];
+  __shared__ cplx tws[N / 2];
+  const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
+  int64_t base0, sa, st;
+  int nvalid;
+  int cx = 0, cy = 0, cz0 = 0;
+  if (a.axis == 2) {
+    const int64_t nlines = (int64_t)Nx * Ny;
+    const int64_t l0 = (int64_t)blockIdx.x * TB;
+    if (l0 >= nlines) return;
+    nvalid = (int)min((int64_t)TB, nlines - l0);
+    base0 = l0 * N;
+    sa = 1;
+    st = N;
+    cx = (int)(l0 / Ny);
+    cy = (int)(l0 % Ny);
+  } else {
+    const int nch = (Nz + TB - 1) / TB;
+    const int other = (int)(blockIdx.x / nch);
+    cz0 = (int)(blockIdx.x % nch) * TB;
+    if (a.axis == 1) {
+      if (other >= Nx) return;
+      cx = other;
+      base0 = (int64_t)other * Ny * Nz + cz0;
+      sa = Nz;
+    } else {
+      if (other >= Ny) return;
+      cy = other;
+      base0 = (int64_t)other * Nz + cz0;
+      sa = (int64_t)Ny * Nz;
+    }
+    st = 1;
+    nvalid = min(TB, Nz - cz0);
+  }
+  for (int m = threadIdx.x; m < N / 2; m += TB * G) {
+    const double wr = a.tw[2 * m], wi = a.tw[2 * m + 1];
+    tws[m] = mk(wr, a.dir > 0 ? -wi : wi);
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  // ---- level 1: thread (t, n2) transforms x[n1 N2 + n2], n1 < N1
+  {
+    int t, n2;
+    if (a.axis == 2) { n2 = tid % G; t = tid / G; } else { t = tid % TB; n2 = tid / TB; }
+    cplx v[N1];
+    double kf = 0.0;
+    bool kconst = false;
+    if (a.kmul && a.mul_axis != a.axis) {
+      int coord;
+      if (a.axis == 2) {
+        const int64_t l = (int64_t)cx * Ny + cy + t;
+        coord = a.mul_axis == 0 ? (int)(l / Ny) : (int)(l % Ny);
+      } else if (a.mul_axis == 2) coord = cz0 + t;
+      else coord = a.mul_axis == 0 ? cx : cy;
+      kf = a.kmul[min(coord, a.mul_axis == 0 ? Nx - 1 : a.mul_axis == 1 ? Ny - 1 : Nz - 1)];
+      kconst = true;
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      cplx x = mk(0, 0);
+      if (t < nvalid) {
+        const int idx = n1 * N2 + n2;
+        x = a.in[base0 + (int64_t)t * st + (int64_t)idx * sa];
+        if (a.kmul) {
+          const double k = kconst ? kf : a.kmul[idx];
+          x = mk(-k * x.y, k * x.x);  // i k x
+        }
+      }
+      v[n1] = x;
+    }
+    dft_reg<N1, N>(v, tws);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int m = (n2 * k1) % N;
+      const cplx w = m < N / 2 ? tws[m] : mk(-tws[m - N / 2].x, -tws[m - N / 2].y);
+      Y[(t * N1 + k1) * (N2 + 1) + n2] = v[k1] * w;
+    }
+  }
+  __syncthreads();
+  // ---- level 2: thread (t, k1) transforms over n2 -> X[k1 + N1 k2]
+  {
+    int t, k1;
+    if (a.axis == 2) { k1 = tid % G; t = tid / G; } else { t = tid % TB; k1 = tid / TB; }
+    if (k1 < N1 && t < nvalid) {
+      cplx u[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) u[n2] = Y[(t * N1 + k1) * (N2 + 1) + n2];
+      dft_reg<N2, N>(u, tws);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int k = k1 + N1 * k2;
+        a.out[base0 + (int64_t)t * st + (int64_t)k * sa] = u[k2] * a.scale;
+      }
+    }
+  }
+}
+
+template <int N1, int N2>
+void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
+  constexpr int N = N1 * N2;
+  constexpr int G = N2;
+  constexpr int TB0 = 256 / G, TBL = 2048 / N;
+  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  int64_t blocks;
+  if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
+  else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
+  hipLaunchKernelGGL((k_fft2<N1, N2>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+}
+
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
                 hipStream_t s) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
@@ -181,6 +329,14 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   a.mul_axis = mul_axis;
   a.dir = dir;
   a.scale = dir > 0 ? 1.0 / N : 1.0;
+  switch (N) {
+    case 64: launch_fft2<8, 8>(a, g, s); return 0;
+    case 128: launch_fft2<8, 16>(a, g, s); return 0;
+    case 256: launch_fft2<16, 16>(a, g, s); return 0;
+    case 512: launch_fft2<16, 32>(a, g, s); return 0;
+    case 1024: launch_fft2<32, 32>(a, g, s); return 0;
+    default: break;
+  }
   const int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   int64_t blocks;
   if (axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
