@@ -294,6 +294,7 @@ __global__ void __launch_bounds__(256) k_prep(const pss_wire_desc* __restrict__ 
     for (int n = 0; n < 3; ++n) ws[L.vc + n * Nk + q] = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
   }
   if (threadIdx.x == 0) ws[L.red + 0] = n1;
+  if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
 }
 
 // ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H
@@ -417,6 +418,9 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
   const double bAA = kHbar * (holes ? d.gam_e : d.gam_eh);
   const double bAB = kHbar * d.gam_h;
   const double pAA = bAA / kPi, pAB = bAB / kPi, bAA2 = bAA * bAA, bAB2 = bAB * bAB;
+  // mixed precision: (b/pi)/(E^2+b^2) = (1/(pi b)) / ((E/b)^2 + 1) with the reciprocal in FP32
+  const bool fast = d.precision == PSS_MIXED;
+  const double ipAA = 1.0 / (kPi * bAA), ipAB = 1.0 / (kPi * bAB), ibAA = 1.0 / bAA, ibAB = 1.0 / bAB;
   const int aq = q < 0 ? -q : q;
   double vAA = 0.0, vAB = 0.0;
   if (aq >= 1 && aq <= Nk - 1) {
@@ -435,7 +439,8 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
         const int pm = p - q;
         const double npm = sA[pm], np_ = sA[p];
         const double Ed = EAkq + sEA[pm] - sEA[p] - EAk;
-        const double lr = ddiv(pAA, Ed * Ed + bAA2);
+        const double lr = fast ? ipAA * (double)__frcp_rn((float)((Ed * ibAA) * (Ed * ibAA) + 1.0))
+                               : ddiv(pAA, Ed * Ed + bAA2);
         acc += vAA * lr * (npm * (1.0 - np_) * (1.0 - nk) + (1.0 - npm) * np_ * nk);
       }
     }
@@ -445,7 +450,8 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
         const int pp = p + q;
         const double npp = sB[pp], np_ = sB[p];
         const double Ed = EAkq + sEB[pp] - sEB[p] - EAk;
-        const double lr = ddiv(pAB, Ed * Ed + bAB2);
+        const double lr = fast ? ipAB * (double)__frcp_rn((float)((Ed * ibAB) * (Ed * ibAB) + 1.0))
+                               : ddiv(pAB, Ed * Ed + bAB2);
         acc += vAB * lr * (npp * (1.0 - np_) * (1.0 - nk) + (1.0 - npp) * np_ * nk);
       }
     }
@@ -585,7 +591,7 @@ __device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, cons
 __global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs) {
   const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
   const pss_wire_desc& d = descs[wire];
-  if (!wire_on(d)) return;
+  if (!wire_on(d) || d.precision == PSS_MIXED) return;
   const int Nk = d.Nk;
   const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   if (m0 >= Nk || n0 >= Nk) return;
@@ -707,6 +713,254 @@ __global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restric
   }
 }
 
+// ------------------------------- K6m: mixed-precision RHS contractions
+// Same products as k_rhs_gemm, with each complex operand split per real
+// component into FP16 hi + lo (x s = hi + lo, s a power of two per matrix from
+// its max |x|), three MFMA products hi*hi + hi*lo + lo*hi accumulated in FP32,
+// folded into FP64 per term.  The diagonals of Hee and Hhh (single-particle
+// energies + exchange diagonal, which set the optical phase) are kept out of the
+// split products and applied exactly in FP64 in the epilogue, as is the
+// leapfrog accumulation.  Max-abs slots: red[8 + m], m = p, C, D, Heh, Hee, Hhh,
+// hbar Dh, hbar De.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void atomic_max_pos(double* addr, double v) {
+  atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256) k_maxabs(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  const int Nk = d.Nk;
+  const int m = blockIdx.y;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int64_t NN = (int64_t)Nk * Nk;
+  const cplx* H = (const cplx*)(ws + L.H);
+  __shared__ double sh[16];
+  double v = 0.0;
+  if (m < 6) {
+    const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : m == 2 ? B.D2 : H + (m - 3) * NN;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < NN; o += (int64_t)gridDim.x * 256) {
+      if ((m == 4 || m == 5) && (o / Nk == o % Nk)) continue;  // diagonal handled in FP64
+      v = fmax(v, fmax(fabs(X[o].x), fabs(X[o].y)));
+    }
+  } else if (d.flags & PSS_OFFDIAGDPH) {
+    const double* D = ws + (m == 6 ? L.Dh : L.De);
+    const int64_t n = (2 * (int64_t)Nk + 1) * Nk;
+    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256)
+      v = fmax(v, fabs(kHbar * D[o]));
+  }
+  v = block_max<256>(v, sh);
+  if (threadIdx.x == 0 && v > 0.0) atomic_max_pos(ws + L.red + 8 + m, v);
+}
+
+__device__ inline double pow2_scale(double mx) {
+  if (!(mx > 0.0)) return 1.0;
+  int e;
+  frexp(mx, &e);
+  return ldexp(1.0, 14 - e);
+}
+
+// matrix index (red slot) of a term operand
+__device__ inline int slot_of(int src, int op) {
+  if (op == OP_MH) return 6;
+  if (op == OP_MET) return 7;
+  switch (src) {
+    case S_P: return 0;
+    case S_C: return 1;
+    case S_D: return 2;
+    case S_HEH: return 3;
+    case S_HEE: return 4;
+    default: return 5;  // S_HHH
+  }
+}
+
+__device__ inline void split16(double x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (double)hi);
+}
+
+__global__ void __launch_bounds__(256) k_rhs_gemm_mixed(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  const int Nk = d.Nk;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  if (m0 >= Nk || n0 >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int64_t NN = (int64_t)Nk * Nk;
+  const cplx* H = (const cplx*)(ws + L.H);
+  const double* Dh = ws + L.Dh;
+  const double* De = ws + L.De;
+  const double* mx = ws + L.red + 8;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+  __shared__ double lds_raw[8192];  // 64 KB: fp16 staging during the loop, FP64 partial tiles after
+  // per wave staging: A[hr, lr, hi, li][32][16], B[hr, lr, hi, li][32][16] halfs = 8 KB
+  _Float16* stg = (_Float16*)lds_raw + wv * 4096;
+  _Float16* As = stg;          // [4][32][16]
+  _Float16* Bs = stg + 2048;   // [4][32][16]
+
+  f4 ar_[2][2], ai_[2][2];
+  double pr[2][2][4], pi_[2][2][4];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) {
+        ar_[a][b][v] = 0.f; ai_[a][b][v] = 0.f;
+        pr[a][b][v] = 0.0; pi_[a][b][v] = 0.0;
+      }
+  const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
+  const int nkc = (Nk + 15) / 16;
+  const int nitems = nterms * nkc;
+  const int niter = (nitems + 3) / 4;
+  int cur_t = -1;
+  double cur_inv = 0.0;
+  for (int it = 0; it < niter; ++it) {
+    const int item = it * 4 + wv;
+    const bool has = item < nitems;
+    const int t = has ? item / nkc : cur_t;
+    if (has && t != cur_t) {
+      // fold the FP32 accumulators of the previous term into FP64
+      if (cur_t >= 0)
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) {
+              pr[a][b][v] += (double)ar_[a][b][v] * cur_inv;
+              pi_[a][b][v] += (double)ai_[a][b][v] * cur_inv;
+              ar_[a][b][v] = 0.f;
+              ai_[a][b][v] = 0.f;
+            }
+      const Term T = kTerms[out][t];
+      const double sa = pow2_scale(mx[slot_of(T.a, T.aop)]), sb = pow2_scale(mx[slot_of(T.b, T.bop)]);
+      cur_inv = 1.0 / (sa * sb);
+      cur_t = t;
+    }
+    if (has) {
+      const int k0 = (item % nkc) * 16;
+      const Term T = kTerms[out][t];
+      const cplx* Ap = src_ptr(T.a, B, H, NN);
+      const cplx* Bp = src_ptr(T.b, B, H, NN);
+      const double sg = T.neg ? -1.0 : 1.0;
+      const double sa = pow2_scale(mx[slot_of(T.a, T.aop)]) * sg, sb = pow2_scale(mx[slot_of(T.b, T.bop)]);
+      const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
+      const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
+      const bool adiag0 = (T.a == S_HEE || T.a == S_HHH), bdiag0 = (T.b == S_HEE || T.b == S_HHH);
+      for (int e = lane; e < 512; e += 64) {
+        int mm, kk;
+        if (!aT) { mm = e >> 4; kk = e & 15; } else { kk = e >> 5; mm = e & 31; }
+        cplx v = load_op(Ap, T.aop, m0 + mm, k0 + kk, Nk, Dh, De);
+        if (adiag0 && m0 + mm == k0 + kk) v = mk(0, 0);
+        _Float16 h, l;
+        split16(v.x * sa, h, l);
+        As[(0 * 32 + mm) * 16 + kk] = h;
+        As[(1 * 32 + mm) * 16 + kk] = l;
+        split16(v.y * sa, h, l);
+        As[(2 * 32 + mm) * 16 + kk] = h;
+        As[(3 * 32 + mm) * 16 + kk] = l;
+      }
+      for (int e = lane; e < 512; e += 64) {
+        int kk, nn;
+        if (!bT) { kk = e >> 5; nn = e & 31; } else { nn = e >> 4; kk = e & 15; }
+        cplx v = load_op(Bp, T.bop, k0 + kk, n0 + nn, Nk, Dh, De);
+        if (bdiag0 && k0 + kk == n0 + nn) v = mk(0, 0);
+        _Float16 h, l;
+        split16(v.x * sb, h, l);
+        Bs[(0 * 32 + nn) * 16 + kk] = h;
+        Bs[(1 * 32 + nn) * 16 + kk] = l;
+        split16(v.y * sb, h, l);
+        Bs[(2 * 32 + nn) * 16 + kk] = h;
+        Bs[(3 * 32 + nn) * 16 + kk] = l;
+      }
+    }
+    __syncthreads();
+    if (has) {
+      const int li = lane & 15, kq = 4 * (lane >> 4);
+      h4 fa[2][4], fb[2][4];
+      for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 4; ++c) {
+          fa[a][c] = *(const h4*)&As[(c * 32 + a * 16 + li) * 16 + kq];
+          fb[a][c] = *(const h4*)&Bs[(c * 32 + a * 16 + li) * 16 + kq];
+        }
+      for (int a = 0; a < 2; ++a) {
+        const h4 nhi = -fa[a][2], nli = -fa[a][3];
+        for (int b = 0; b < 2; ++b) {
+          f4 r = ar_[a][b], i = ai_[a][b];
+          // re*re - im*im
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][0], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][1], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][1], fb[b][0], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nhi, fb[b][2], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nhi, fb[b][3], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nli, fb[b][2], r, 0, 0, 0);
+          // re*im + im*re
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][2], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][3], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][1], fb[b][2], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][2], fb[b][0], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][2], fb[b][1], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][3], fb[b][0], i, 0, 0, 0);
+          ar_[a][b] = r;
+          ai_[a][b] = i;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (cur_t >= 0)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 4; ++v) {
+          pr[a][b][v] += (double)ar_[a][b][v] * cur_inv;
+          pi_[a][b][v] += (double)ai_[a][b][v] * cur_inv;
+        }
+  // partial tiles: v_mfma_f32_16x16x16f16 result lane l, element v -> row 4(l/16)+v, col l%16
+  double* P = lds_raw + wv * 2048;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) {
+        const int i = a * 16 + 4 * (lane >> 4) + v;
+        const int j = b * 16 + (lane & 15);
+        P[2 * (i * 32 + j)] = pr[a][b][v];
+        P[2 * (i * 32 + j) + 1] = pi_[a][b][v];
+      }
+  __syncthreads();
+  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
+  const cplx* X2 = out == 0 ? B.p2 : out == 1 ? B.C2 : B.D2;
+  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
+  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
+  const double* gam = ws + L.gam;
+  const cplx* Hee = H + NN;
+  const cplx* Hhh = H + 2 * NN;
+  for (int q = threadIdx.x; q < 1024; q += 256) {
+    const int li = q >> 5, lj = q & 31;
+    const int i = m0 + li, j = n0 + lj;
+    if (i >= Nk || j >= Nk) continue;
+    cplx S = mk(0, 0);
+    for (int w = 0; w < 4; ++w) S = S + mk(lds_raw[w * 2048 + 2 * q], lds_raw[w * 2048 + 2 * q + 1]);
+    const int64_t o = (int64_t)i * Nk + j;
+    const cplx x2 = X2[o];
+    if (out == 0) {
+      // exact diagonal parts: Hhh[kh,kh] p[kh,ke] + p[kh,ke] Hee[ke,ke]; + Heh^T - i hbar gamma p
+      const cplx hT = H[(int64_t)j * Nk + i];
+      const double g = kHbar * (gam[j] + gam[Nk + i]);
+      S = S + Hhh[(int64_t)i * Nk + i] * x2 + x2 * Hee[(int64_t)j * Nk + j] + hT - mk(-g * x2.y, g * x2.x);
+    } else {
+      const cplx* Hd = out == 1 ? Hee : Hhh;
+      S = S + x2 * (Hd[(int64_t)j * Nk + j] - Hd[(int64_t)i * Nk + i]);
+    }
+    const cplx R = div_ih(S, kHbar);
+    const cplx x1 = X1[o];
+    const cplx x3 = x1 + (R * d.dt) * 2.0;
+    Xn[o] = x3;
+    if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+  }
+}
+
 // --------------------------- K7: carrier-carrier + phonon collision rates
 // MBCE/MBCH (PSTD3D/coulomb.py:957-1071) with the Ceh/Cee/Chh tables generated
 // on the fly and read with the reference's one-slot index shift (slot (a,b,c)
@@ -717,6 +971,7 @@ struct CollCtx {
   int Nk;
   double hw2[3], ihw[3], pre[3];    // per kind (eh, ee, hh): (hbar g)^2, 1/(hbar g), prefactor
   bool lorentz;
+  bool fast;                        // mixed precision: exp / reciprocal of the line shape in FP32
 };
 
 // value read as C_kind[a, b, c] (0-based read indices); kind 0 = eh, 1 = ee, 2 = hh
@@ -730,8 +985,15 @@ __device__ inline double coll(const CollCtx& x, int kind, int a, int b, int c) {
   const double* E1 = kind == 2 ? x.Eh : x.Ee;
   const double* E2 = kind == 0 ? x.Eh : E1;
   const double Ed = E1[A] + E2[Bq] - E2[k30] - E1[Cq];
-  if (x.lorentz) return ddiv(x.pre[kind], Ed * Ed + x.hw2[kind]);
+  if (x.lorentz) {
+    if (x.fast) {  // 2 g / (E^2 + hw^2) = (2 g / hw^2) / (z^2 + 1)
+      const double z = Ed * x.ihw[kind];
+      return x.pre[kind] / x.hw2[kind] * (double)__frcp_rn((float)(z * z + 1.0));
+    }
+    return ddiv(x.pre[kind], Ed * Ed + x.hw2[kind]);
+  }
   const double z = Ed * x.ihw[kind];
+  if (x.fast) return x.pre[kind] * (double)__expf(-(float)(z * z));
   return x.pre[kind] * exp(-(z * z));
 }
 
@@ -768,6 +1030,7 @@ __global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__
   CollCtx x;
   x.Ee = sEe; x.Eh = sEh; x.ne = ne; x.nh = nh; x.Nk = Nk;
   x.lorentz = (d.flags & PSS_LORENTZDELTA) != 0;
+  x.fast = d.precision == PSS_MIXED;
   {
     const double g[3] = {d.gam_eh, d.gam_e, d.gam_h};
     for (int c = 0; c < 3; ++c) {
@@ -1194,6 +1457,8 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   const int t32 = (max_Nk + 31) / 32;
   hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_rhs_gemm_mixed, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_relax, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
   hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
