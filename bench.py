@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--nk", type=int, default=256)
     ap.add_argument("--npml", type=int, default=12)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     return ap.parse_args()
@@ -99,6 +99,32 @@ def fft_pass_timing(prop, reps=50):
     return ms, bytes_
 
 
+def rel_err(a, b):
+    import torch
+
+    return float(torch.linalg.vector_norm(a - b) / torch.clamp(torch.linalg.vector_norm(b), min=1e-300))
+
+
+def precision_errors(p64, q64, pmx, qmx):
+    """Relative L2 of the mixed run against the FP64 run after the same steps:
+    fields (E, B as vector fields), wire polarisation state p, populations (diagonals
+    of C, D), and the coherences C, D."""
+    import torch
+
+    out = {}
+    for tag, names in (("E", ("Ex", "Ey", "Ez")), ("B", ("Bx", "By", "Bz"))):
+        num = sum(float(torch.linalg.vector_norm(getattr(pmx, n) - getattr(p64, n))) ** 2 for n in names)
+        den = sum(float(torch.linalg.vector_norm(getattr(p64, n))) ** 2 for n in names)
+        out[tag] = (num / max(den, 1e-300)) ** 0.5
+    a, b = q64.wires.get_state(0), qmx.wires.get_state(0)
+    out["p"] = float(np.linalg.norm(b["p2"] - a["p2"]) / max(np.linalg.norm(a["p2"]), 1e-300))
+    for k in ("C2", "D2"):
+        da, db = np.diag(a[k]).real, np.diag(b[k]).real
+        out["n_" + k[0].lower()] = float(np.linalg.norm(db - da) / max(np.linalg.norm(da), 1e-300))
+        out[k[0]] = float(np.linalg.norm(b[k] - a[k]) / max(np.linalg.norm(a[k]), 1e-300))
+    return out
+
+
 def cpu_baseline(grid, nk, npml, steps):
     """The FP64 NumPy oracle on the same workload for `steps` steps (bounded sample)."""
     from oracle import pstd as OP
@@ -134,9 +160,13 @@ def main():
     for prec in precs:
         prop, qa = setup(a.grid, a.nk, a.npml, prec)
         el = time_steps(prop, a.steps, a.warmup)
-        results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop)
+        results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop, qa=qa)
     main_prec = precs[-1]
     r = results[main_prec]
+    errs = None
+    if len(precs) == 2:
+        errs = precision_errors(results["fp64"]["prop"], results["fp64"]["qa"], results["mixed"]["prop"],
+                                results["mixed"]["qa"])
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -148,15 +178,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128/f64" if main_prec == "fp64" else "mixed",
+        "dtype": "c128/f64" if main_prec == "fp64" else
+                 "mixed (SBE products fp16 hi/lo MFMA, fp32 accumulate; fields, populations, time step fp64)",
         "data": "synthetic (seeded soft-source pulse, reference initial wire state)",
         "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
                                f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,
                    "wires": 1, "precision": main_prec},
     }
-    if "fp64" in results and "mixed" in results:
+    if errs is not None:
         line["fp64_value"] = results["fp64"]["value"]
+        line["fp64_ms_per_step"] = results["fp64"]["ms"]
         line["mixed_over_fp64"] = results["mixed"]["value"] / results["fp64"]["value"]
+        line["mixed_rel_err_vs_fp64"] = errs
+        line["mixed_within_1e-6"] = all(v <= 1e-6 for v in errs.values())
     if not a.no_kernel_timing:
         ms, b = fft_pass_timing(r["prop"])
         ach = b / (ms * 1e-3)

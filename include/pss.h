@@ -91,6 +91,15 @@ int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, i
                  int32_t max_nrr, void* stream);
 
 /*
+ * Uniform drive of batched wires: Exx/Eyy/Ezz of every wire are set along the wire to
+ * E0_c exp(-u^2/(w0 tw)^2) cos(u) exp(-u^20/(2 w0 tw)^20), u = w0 (t - tp), with
+ * t = t0 + ctl[1] dt and ctl[1] advanced by one (the Gaussian pulse of
+ * docs/source/examples/sbe_wire_1d.py:45-54 applied as at :99-100).
+ * d_pulses: device [nwires][8] = {E0x, E0y, E0z, w0, tw, tp, t0, unused}.
+ */
+int pss_drive_uniform(const pss_wire_desc* d_descs, const double* d_pulses, int32_t nwires, void* stream);
+
+/*
  * Coulomb integral V(q) = prefactor * Vint(q) for a list of q (PSTD3D/coulomb.py:70-104,
  * 502-577), with the reference's series/asymptotic K0 (PSTD3D/coulomb.py:39-66).
  * y: [Ny] wire grid; q: [nq]; out: [nq].
@@ -158,6 +167,18 @@ int pss_fill_line(const double* sample, double* lines, int32_t n, void* stream);
  * then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx], jline: [3][Nx]. */
 int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
               double dt, double dx, double* jline, void* stream);
+
+/*
+ * Fused stage combine of the classic RK4 step (libpulsesuite/integrator.py:143-157,
+ * Numerical Recipes stage order) on n doubles (a complex array is 2n doubles):
+ *   stage 0: ytmp = y + (h/2) dydt
+ *   stage 1: ytmp = y + (h/2) dyt
+ *   stage 2: ytmp = y + h dym;  dym = dym + dyt
+ *   stage 3: y = y + (h/6) (dydt + dyt + 2 dym)
+ * Separately rounded FP64 operations in that order (bit-identical to the expressions).
+ */
+int pss_rk4_stage(int32_t stage, int64_t n, double h, double* y, const double* dydt, const double* dyt,
+                  double* dym, double* ytmp, void* stream);
 
 /* Message of the last failing call (host string, valid until the next call). */
 const char* pss_last_error(void);

@@ -799,6 +799,7 @@ class OracleWire:
                 self.MBCH(np.real(ne3), np.real(nh3), VC, WinH, WoutH)
             if mb.Phonon:
                 self.MBPH(np.real(nh3), VC, E1D, WinH, WoutH)
+            self.last_W = (WinE, WoutE, WinH, WoutH)  # collision rates, for tests
             We = WinE + WoutE
             Wh = WinH + WoutH
             ne3 = np.exp(-We * dt) * ((-1 + np.exp(We * dt) + ne3) * WinE + ne3 * WoutE) / (We + self.small)

@@ -225,7 +225,8 @@ def ws_layout(Nk, Nr, nrr):
     segs = [("er", 3 * Nr), ("spa", 3 * nrr), ("eq", 3 * nd * 2), ("vc", 3 * Nk), ("e1d", Nk), ("ne2", 2 * Nk),
             ("nh2", 2 * Nk), ("tr", 4 * nd * 2), ("gam", 2 * Nk), ("De", (2 * Nk + 1) * Nk),
             ("Dh", (2 * Nk + 1) * Nk), ("H", 3 * Nk * Nk * 2), ("W", 4 * Nk), ("n3", 4 * Nk), ("red", 32),
-            ("ds", 4 * nd * 2), ("pr", 3 * Nr), ("rho", 2 * Nr * 2), ("spb", 5 * Nr), ("outr", 5 * nrr)]
+            ("ds", 4 * nd * 2), ("pr", 3 * Nr), ("rho", 2 * Nr * 2), ("spb", 5 * Nr), ("outr", 5 * nrr),
+            ("cp", 2 * Nk * Nk)]
     out = {}
     o = 0
     for n, sz in segs:

@@ -7,7 +7,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpss.so")
-SOURCES = ["capi.hip", "sbe.hip", "pstd.hip", "sbe_mixed.hip"]
+SOURCES = ["capi.hip", "sbe.hip", "pstd.hip", "integrator.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

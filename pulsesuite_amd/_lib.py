@@ -73,6 +73,8 @@ def _load():
                                     C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
+        "pss_drive_uniform": (C.c_int, [_vp, _vp, _i, _vp]),
+        "pss_rk4_stage": (C.c_int, [_i, C.c_int64, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_last_error": (C.c_char_p, []),
         "pss_version": (C.c_char_p, []),
     }
@@ -86,8 +88,8 @@ def _load():
 lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_last_error",
-           "pss_version")
+           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform",
+           "pss_last_error", "pss_version")
 
 
 def check(rc, what):

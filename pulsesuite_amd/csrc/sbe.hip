@@ -10,6 +10,8 @@
 // State rotation: prev (X1) lives in buffer 0 and is averaged in place; cur (X2)
 // and next (X3) swap between buffers 1 and 2 by the parity of the wire's step
 // counter ctl[0]; *on is the solver-wide latch of the reference's wireoff gate.
+#include <algorithm>
+
 #include "common.h"
 #include "../../include/pss.h"
 
@@ -30,7 +32,7 @@ __device__ inline double ddiv(double n, double d) {
 
 // ---------------------------------------------------------------- workspace
 struct WS {
-  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, total;
+  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, total;
 };
 
 __host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
@@ -59,6 +61,7 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.rho = o;  o += al(2 * (int64_t)Nr * 2);
   w.spb = o;  o += al(5 * (int64_t)Nr);
   w.outr = o; o += al(5 * (int64_t)nrr);
+  w.cp = o;   o += al(2 * (int64_t)Nk * Nk);
   w.total = o;
   return w;
 }
@@ -966,156 +969,168 @@ __global__ void __launch_bounds__(256) k_rhs_gemm_mixed(const pss_wire_desc* __r
 // on the fly and read with the reference's one-slot index shift (slot (a,b,c)
 // holds the entry built for (a-1,b-1,c-1); slot 0 is zero); MBPE/MBPH
 // (PSTD3D/phonons.py:140-268).  n = |Re n3| for MBC*, Re n3 for MBP*.
+//
+// Regrouping of the eight reference sums (1-based momenta, n padded with n[0]=0).
+// Every term is X(a,b,c) * (population factors) with, for e-h scattering,
+// X = V_eh(|a-c|)^2 C_eh[a,b,c] and the fourth momentum d = b + c - a, and the
+// shifted table read C_eh[a,b,c] = delta(Ee[a-2] + Eh[b-2] - Eh[d-2] - Ee[c-2]),
+// nonzero only for a,b,c,d >= 2 and a != c:
+//   Win_e[a]  += X (1-nh[b]) nh[d] ne[c]      (MBCE in,  coulomb.py:975)
+//   Wout_e[c] += X (1-ne[a]) (1-nh[b]) nh[d]  (MBCE out, coulomb.py:986)
+//   Win_h[b]  += X (1-ne[a]) nh[d] ne[c]      (MBCH in)
+//   Wout_h[d] += X (1-ne[a]) (1-nh[b]) ne[c]  (MBCH out)
+// and for e-e (h-h alike) with d = a + b - c and C_ee[a,b,c] =
+// delta(Ee[a-2] + Ee[b-2] - Ee[d-2] - Ee[c-2]):
+//   Win_e[a]  += X (1-ne[b]) ne[d] ne[c],   Wout_e[c] += X (1-ne[a]) (1-ne[b]) ne[d].
+// A workgroup owns one output momentum o, a thread one partner momentum t:
+//   electrons (a = o, c = t): S = sum_b delta(..) (1-nh[b]) nh[b+t-o]   (e-h)
+//                                 + sum_b delta(..) (1-ne[b]) ne[o+b-t] (e-e, own V)
+//     Win_e[o] += ne[t] V^2 S,  Wout_e[t] += (1-ne[o]) V^2 S
+//   holes (e-h: b = o, d = t; h-h: a = o, c = t):
+//     S = sum_a delta(..) (1-ne[a]) ne[t+a-o]   (e-h),  sum_b delta(..) (1-nh[b]) nh[o+b-t]  (h-h)
+//     Win_h[o] += nh[t] V^2 S,  Wout_h[t] += (1-nh[o]) V^2 S
+// so each line shape is evaluated once per (species, kind) instead of once per
+// output term.  Wout partial sums go to a per-wire (species, t, o) array and are
+// reduced over o by k_wsum, in a fixed order.
 struct CollCtx {
-  const double *Ee, *Eh, *ne, *nh;  // ne/nh here: 1-based padded |Re n3|, index 0 = 0
-  int Nk;
   double hw2[3], ihw[3], pre[3];    // per kind (eh, ee, hh): (hbar g)^2, 1/(hbar g), prefactor
   bool lorentz;
   bool fast;                        // mixed precision: exp / reciprocal of the line shape in FP32
 };
 
-// value read as C_kind[a, b, c] (0-based read indices); kind 0 = eh, 1 = ee, 2 = hh
-// GaussDelta: (2 pi/hbar) exp(-(E/hw)^2) / (sqrt(pi) hw); Lorentz: 2 g / (E^2 + hw^2)
-__device__ inline double coll(const CollCtx& x, int kind, int a, int b, int c) {
-  const int A = a - 1, Bq = b - 1, Cq = c - 1;
-  if (A < 0 || Bq < 0 || Cq < 0) return 0.0;
-  const int k30 = kind == 0 ? Cq + Bq - A : A + Bq - Cq;
-  if (k30 < 0 || k30 >= x.Nk) return 0.0;
-  if (A == Cq || Bq == k30) return 0.0;
-  const double* E1 = kind == 2 ? x.Eh : x.Ee;
-  const double* E2 = kind == 0 ? x.Eh : E1;
-  const double Ed = E1[A] + E2[Bq] - E2[k30] - E1[Cq];
-  if (x.lorentz) {
-    if (x.fast) {  // 2 g / (E^2 + hw^2) = (2 g / hw^2) / (z^2 + 1)
-      const double z = Ed * x.ihw[kind];
-      return x.pre[kind] / x.hw2[kind] * (double)__frcp_rn((float)(z * z + 1.0));
-    }
+// GaussDelta: (2 pi/hbar) exp(-(E/hw)^2) / (sqrt(pi) hw)  (coulomb.py:245-263);
+// Lorentz: 2 g / (E^2 + hw^2).  FAST (mixed precision): the exponential, or the
+// reciprocal (2 g / hw^2) / (z^2 + 1) of the Lorentzian, in FP32.
+template <bool LORENTZ, bool FAST>
+__device__ inline double line_shape(const CollCtx& x, int kind, double Ed) {
+  const double z = Ed * x.ihw[kind];
+  if (LORENTZ) {
+    if (FAST) return x.pre[kind] / x.hw2[kind] * (double)__frcp_rn((float)(z * z + 1.0));
     return ddiv(x.pre[kind], Ed * Ed + x.hw2[kind]);
   }
-  const double z = Ed * x.ihw[kind];
-  if (x.fast) return x.pre[kind] * (double)__expf(-(float)(z * z));
+  if (FAST) return x.pre[kind] * (double)__expf(-(float)(z * z));
   return x.pre[kind] * exp(-(z * z));
 }
 
-__global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__ descs) {
+__device__ inline CollCtx coll_ctx(const pss_wire_desc& d) {
+  CollCtx x;
+  x.lorentz = (d.flags & PSS_LORENTZDELTA) != 0;
+  x.fast = d.precision == PSS_MIXED;
+  const double g[3] = {d.gam_eh, d.gam_e, d.gam_h};
+  for (int c = 0; c < 3; ++c) {
+    const double hw = kHbar * g[c];
+    x.hw2[c] = hw * hw;
+    x.ihw[c] = 1.0 / hw;
+    x.pre[c] = x.lorentz ? 2.0 * g[c] : (2.0 * kPi / kHbar) * (1.0 / (sqrt(kPi) * hw));
+  }
+  return x;
+}
+
+// sum over j in [lo, hi] of shape(base + P[j].E - Q[j + s].E) (1 - P[j].n) Q[j + s].n
+// where P / Q are (energy, population) pairs indexed by 1-based momentum
+template <bool LORENTZ, bool FAST>
+__device__ inline double coll_line(const CollCtx& x, int kind, double base, const double2* __restrict__ P,
+                                   const double2* __restrict__ Q, int s, int lo, int hi) {
+  double acc = 0.0;
+  for (int j = lo; j <= hi; ++j) {
+    const double2 pj = P[j], qj = Q[j + s];
+    acc += line_shape<LORENTZ, FAST>(x, kind, base + pj.x - qj.x) * ((1.0 - pj.y) * qj.y);
+  }
+  return acc;
+}
+
+template <bool LORENTZ, bool FAST>
+__device__ inline void coll_pair(const CollCtx& x, bool holes, int Nk, int o, int t, const double2* Pe,
+                                 const double2* Ph, double& Sx, double& Ss) {
+  if (!holes) {
+    // e-h: a = o, c = t, loop b, d = b + t - o: Ed = Ee[o-2] - Ee[t-2] + Eh[b-2] - Eh[d-2]
+    Sx = coll_line<LORENTZ, FAST>(x, 0, Pe[o].x - Pe[t].x, Ph, Ph, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
+    // e-e: a = o, c = t, loop b, d = o + b - t
+    Ss = coll_line<LORENTZ, FAST>(x, 1, Pe[o].x - Pe[t].x, Pe, Pe, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
+  } else {
+    // e-h: b = o, d = t, loop a, c = t + a - o: Ed = Ee[a-2] - Ee[c-2] + Eh[o-2] - Eh[t-2]
+    Sx = coll_line<LORENTZ, FAST>(x, 0, Ph[o].x - Ph[t].x, Pe, Pe, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
+    // h-h: a = o, c = t, loop b, d = o + b - t
+    Ss = coll_line<LORENTZ, FAST>(x, 2, Ph[o].x - Ph[t].x, Ph, Ph, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_coll(const pss_wire_desc* __restrict__ descs) {
   const pss_wire_desc& d = descs[blockIdx.z];
-  if (!wire_on(d) || !(d.flags & (PSS_EHS | PSS_PHONON))) return;
+  if (!wire_on(d) || !(d.flags & PSS_EHS)) return;
   const int Nk = d.Nk;
-  const int kk = blockIdx.x;  // 0-based output index
-  if (kk >= Nk) return;
+  const int o = (int)blockIdx.x + 1;  // 1-based output momentum
+  if (o > Nk) return;
   const bool holes = blockIdx.y == 1;
   __shared__ double sh[16];
-  extern __shared__ double dyn[];  // 2 (Nk + 1) + 2 Nk doubles
+  extern __shared__ double2 pairs[];  // (E[k-2], n[k]) for k = 0..Nk, electrons then holes
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
-  // populations from the new diagonal (C3, D3 before relaxation)
-  double* sne = dyn;
-  double* snh = dyn + (Nk + 1);
-  double* sEe = dyn + 2 * (Nk + 1);
-  double* sEh = sEe + Nk;
+  double2* Pe = pairs;
+  double2* Ph = pairs + (Nk + 1);
   for (int k = threadIdx.x; k <= Nk; k += 256) {
-    if (k < Nk) {
-      sEe[k] = d.Ee[k];
-      sEh[k] = d.Eh[k];
-    }
-    if (k == 0) { sne[0] = 0.0; snh[0] = 0.0; continue; }
-    sne[k] = fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x);
-    snh[k] = fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    const bool in = k >= 2;
+    Pe[k] = make_double2(in ? d.Ee[k - 2] : 0.0, k == 0 ? 0.0 : fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x));
+    Ph[k] = make_double2(in ? d.Eh[k - 2] : 0.0, k == 0 ? 0.0 : fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x));
   }
   __syncthreads();
-  const double* ne = sne;
-  const double* nh = snh;
+  const CollCtx x = coll_ctx(d);
   const double* vc = ws + L.vc;
-  CollCtx x;
-  x.Ee = sEe; x.Eh = sEh; x.ne = ne; x.nh = nh; x.Nk = Nk;
-  x.lorentz = (d.flags & PSS_LORENTZDELTA) != 0;
-  x.fast = d.precision == PSS_MIXED;
-  {
-    const double g[3] = {d.gam_eh, d.gam_e, d.gam_h};
-    for (int c = 0; c < 3; ++c) {
-      const double hw = kHbar * g[c];
-      x.hw2[c] = hw * hw;
-      x.ihw[c] = 1.0 / hw;
-      x.pre[c] = x.lorentz ? 2.0 * g[c] : (2.0 * kPi / kHbar) * (1.0 / (sqrt(kPi) * hw));
+  const double2* Po = holes ? Ph : Pe;  // output species
+  const double no = Po[o].y;
+  double win = 0.0;
+  double* pout = ws + L.cp + (holes ? (int64_t)Nk * Nk : 0);
+  for (int t = (int)threadIdx.x + 1; t <= Nk; t += 256) {
+    double Sx = 0.0, Ss = 0.0;  // cross-species (e-h) and same-species (e-e / h-h) sums
+    if (o >= 2 && t >= 2 && t != o) {
+      if (x.lorentz) {
+        if (x.fast) coll_pair<true, true>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+        else coll_pair<true, false>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+      } else {
+        if (x.fast) coll_pair<false, true>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+        else coll_pair<false, false>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+      }
+      const int q = abs(o - t);
+      const double vx = vc[q], vs = vc[(holes ? 2 : 1) * Nk + q];
+      Sx = vx * vx * Sx + vs * vs * Ss;
     }
+    win += Po[t].y * Sx;
+    pout[(int64_t)(t - 1) * Nk + (o - 1)] = (1.0 - no) * Sx;
   }
+  win = block_sum<256>(win, sh);
+  if (threadIdx.x == 0) ws[L.W + (holes ? 2 : 0) * Nk + (o - 1)] = win;
+}
+
+// Win / Wout per momentum: collision in-rate from k_coll, out-rate reduced from
+// its partials, plus MBPE / MBPH (phonons.py:140-268):
+//   MBPE: Win[k] += sum_i Vep[k,i] n[i] EP[k,i];  Wout[k] += sum_i Vep[k,i] (1-n[i]) EP[i,k]
+//   MBPH: Win[k] += sum_i Vhp[i,k] n[i] HP[k,i];  Wout[k] += sum_i Vhp[i,k] (1-n[i]) HP[i,k]
+//   EP[a,b] = 2 Gph (N0/((E[a]-E[b]-hO)^2+hG^2) + (N0+1)/((E[a]-E[b]+hO)^2+hG^2))
+__global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || !(d.flags & (PSS_EHS | PSS_PHONON))) return;
+  const int Nk = d.Nk;
+  const int kk = blockIdx.x;
+  if (kk >= Nk) return;
+  const bool holes = blockIdx.y == 1;
+  __shared__ double sh[16];
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
   double win = 0.0, wout = 0.0;
   if (d.flags & PSS_EHS) {
-    const int64_t npair = (int64_t)Nk * Nk;
-    for (int64_t pr = threadIdx.x; pr < npair; pr += 256) {
-      const int q1 = (int)(pr / Nk) + 1, q2 = (int)(pr % Nk) + 1;
-      if (!holes) {
-        const int k = kk + 1;
-        {  // e-h in/out
-          const int kp = q1, k1 = q2;
-          int v = kp + k1 - k;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[abs(k - k1)];
-            win += V * V * (1.0 - nh[kp]) * nh[v] * ne[k1] * coll(x, 0, k - 1, kp - 1, k1 - 1);
-          }
-          v = kp + k - k1;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[abs(k1 - k)];
-            wout += V * V * (1.0 - ne[k1]) * (1.0 - nh[kp]) * nh[v] * coll(x, 0, k1 - 1, kp - 1, k - 1);
-          }
-        }
-        {  // e-e in/out
-          const int k2 = q1, k4 = q2;
-          int v = k + k2 - k4;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[Nk + abs(k - k4)];
-            win += V * V * (1.0 - ne[k2]) * ne[v] * ne[k4] * coll(x, 1, k - 1, k2 - 1, k4 - 1);
-          }
-          v = k4 + k2 - k;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[Nk + abs(k4 - k)];
-            wout += V * V * (1.0 - ne[k4]) * (1.0 - ne[k2]) * ne[v] * coll(x, 1, k4 - 1, k2 - 1, k - 1);
-          }
-        }
-      } else {
-        const int kp = kk + 1;
-        {
-          const int k = q1, k1 = q2;
-          int v = kp + k1 - k;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[abs(k - k1)];
-            win += V * V * (1.0 - ne[k]) * nh[v] * ne[k1] * coll(x, 0, k - 1, kp - 1, k1 - 1);
-          }
-          v = kp + k - k1;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[abs(k1 - k)];
-            wout += V * V * (1.0 - ne[k]) * (1.0 - nh[v]) * ne[k1] * coll(x, 0, k - 1, v - 1, k1 - 1);
-          }
-        }
-        {
-          const int k2p = q1, k4p = q2;
-          int v = kp + k2p - k4p;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[2 * Nk + abs(kp - k4p)];
-            win += V * V * (1.0 - nh[k2p]) * nh[v] * nh[k4p] * coll(x, 2, kp - 1, k2p - 1, k4p - 1);
-          }
-          v = k4p + k2p - kp;
-          if (v >= 1 && v <= Nk) {
-            const double V = vc[2 * Nk + abs(k4p - kp)];
-            wout += V * V * (1.0 - nh[k4p]) * (1.0 - nh[k2p]) * nh[v] * coll(x, 2, k4p - 1, k2p - 1, kp - 1);
-          }
-        }
-      }
-    }
+    const double* pout = ws + L.cp + (holes ? (int64_t)Nk * Nk : 0) + (int64_t)kk * Nk;
+    for (int i = threadIdx.x; i < Nk; i += 256) wout += pout[i];
   }
   if (d.flags & PSS_PHONON) {
-    // MBPE: Win[k] += sum_i Vep[k,i] n[i] EP[k,i];  Wout[k] += sum_i Vep[k,i] (1-n[i]) EP[i,k]
-    // MBPH: Win[k] += sum_i Vhp[i,k] n[i] HP[k,i];  Wout[k] += sum_i Vhp[i,k] (1-n[i]) HP[i,k]
-    // EP[a,b] = 2 Gph idel (N0/((E[a]-E[b]-hO)^2+hG^2) + (N0+1)/((E[a]-E[b]+hO)^2+hG^2))
     const double* E = holes ? d.Eh : d.Ee;
     const double* e1d = ws + L.e1d;
-    const double* vrow = vc + (holes ? 2 : 1) * Nk;
-    const Bufs& Bq = B;
+    const double* vrow = ws + L.vc + (holes ? 2 : 1) * Nk;
     const double N0 = d.N0, hO = d.hOph, hG2 = d.hGph * d.hGph, G2 = 2.0 * d.Gph;
     for (int i = threadIdx.x; i < Nk; i += 256) {
       if (i == kk) continue;
-      const cplx ni_c = holes ? Bq.Dn[(int64_t)i * Nk + i] : Bq.Cn[(int64_t)i * Nk + i];
+      const cplx ni_c = holes ? B.Dn[(int64_t)i * Nk + i] : B.Cn[(int64_t)i * Nk + i];
       const double ni = ni_c.x;
       const int dq = abs(kk - i);
       const double V = vrow[dq] / e1d[dq] * d.Vscale;
@@ -1130,8 +1145,10 @@ __global__ void __launch_bounds__(256) k_relax(const pss_wire_desc* __restrict__
   win = block_sum<256>(win, sh);
   wout = block_sum<256>(wout, sh);
   if (threadIdx.x == 0) {
-    ws[L.W + (holes ? 2 : 0) * Nk + kk] = win;
-    ws[L.W + (holes ? 3 : 1) * Nk + kk] = wout;
+    double* W = ws + L.W;
+    const int iw = (holes ? 2 : 0) * Nk + kk, iwo = (holes ? 3 : 1) * Nk + kk;
+    W[iw] = ((d.flags & PSS_EHS) ? W[iw] : 0.0) + win;
+    W[iwo] = wout;
   }
 }
 
@@ -1425,6 +1442,39 @@ __global__ void __launch_bounds__(256) k_vint(const double* __restrict__ y, int 
   if (threadIdx.x == 0) out[iq] = pref * acc;
 }
 
+// ------------------------------------------- uniform drive of batched wires
+// Exx/Eyy/Ezz[:] = E0_c exp(-u^2/(w0 tw)^2) cos(u) exp(-u^20/(2 w0 tw)^20),
+// u = w0 (t - tp), t = t0 + n dt with n the wire's drive counter ctl[1]: the
+// Gaussian pulse with super-Gaussian cut-off of docs/source/examples/
+// sbe_wire_1d.py:45-54, applied uniformly along the wire (:99-100).  The counter
+// lives on the device, so a captured graph replays successive times.
+// pulse record (8 doubles): E0x, E0y, E0z, w0, tw, tp, t0, unused.
+__global__ void __launch_bounds__(256) k_drive(const pss_wire_desc* __restrict__ descs,
+                                               const double* __restrict__ pulses) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  const double* pp = pulses + 8 * (int64_t)blockIdx.x;
+  const int n = d.ctl[1];
+  const double t = pp[6] + n * d.dt;
+  const double w0 = pp[3], tw = pp[4], tp = pp[5];
+  const double u = w0 * (t - tp);
+  const double w0tw = w0 * tw;
+  const double u2 = u * u;
+  double u20 = u2 * u2;        // u^4
+  u20 = u20 * u20 * u2 * u2 * u2 * u2 * u2 * u2;  // u^8 * u^12 = u^20
+  double s = 2.0 * w0tw;
+  double s2 = s * s, s4 = s2 * s2;
+  const double s20 = s4 * s4 * s4 * s4 * s4;
+  const double env = exp(-u2 / (w0tw * w0tw)) * cos(u) * exp(-u20 / s20);
+  double* E[3] = {(double*)d.Exx, (double*)d.Eyy, (double*)d.Ezz};
+  for (int i = threadIdx.x; i < 3 * d.nrr; i += 256) {
+    const int c = i / d.nrr, j = i - c * d.nrr;
+    E[c][2 * j] = pp[c] * env;
+    E[c][2 * j + 1] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) d.ctl[1] = n + 1;
+}
+
 }  // namespace
 
 // ================================================================== C ABI
@@ -1440,10 +1490,15 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipStream_t s = (hipStream_t)stream;
   const int nd = 2 * max_Nk - 1;
   const size_t lds_off = 4 * (size_t)max_Nk * sizeof(double);
-  const size_t lds_rel = (4 * (size_t)max_Nk + 2) * sizeof(double);
-  if (lds_rel > 65536) {  // gfx950 gives a workgroup up to 160 KB of LDS on request
-    PSS_CHECK(hipFuncSetAttribute((const void*)k_relax, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
+  const size_t lds_rel = 2 * ((size_t)max_Nk + 1) * sizeof(double2);
+  // gfx950 gives a workgroup up to 160 KB of LDS on request; raised once per process
+  // for the largest size seen (outside any stream work, so a captured step is unaffected)
+  static size_t lds_granted = 65536;
+  const size_t lds_need = std::max(lds_rel, lds_off);
+  if (lds_need > lds_granted) {
+    PSS_CHECK(hipFuncSetAttribute((const void*)k_coll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
     PSS_CHECK(hipFuncSetAttribute((const void*)k_offdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_off));
+    lds_granted = lds_need;
   }
   const int nrb = (max_Nr + 255) / 256;
   hipLaunchKernelGGL(k_prop_solve, dim3(nwires), dim3(256), 0, s, d_descs);
@@ -1459,12 +1514,22 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_rhs_gemm_mixed, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_relax, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
+  hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
+  hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_rho_spline, dim3(nwires), dim3(320), 0, s, d_descs);
   hipLaunchKernelGGL(k_qw2prop, dim3(nwires), dim3(256), 0, s, d_descs);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_drive_uniform(const pss_wire_desc* d_descs, const double* d_pulses, int32_t nwires,
+                                 void* stream) {
+  if (nwires <= 0) return 0;
+  if (!d_descs || !d_pulses) return pss_set_msg(2, "pss_drive_uniform: null pointer");
+  hipLaunchKernelGGL(k_drive, dim3(nwires), dim3(256), 0, (hipStream_t)stream, d_descs, d_pulses);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

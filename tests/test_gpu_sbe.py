@@ -101,6 +101,10 @@ def test_intermediates_one_step():
     assert rel(gam[:nk], GE) < 1e-12 and rel(gam[nk:], GH) < 1e-12
     assert rel(dev.workspace_view(0, "De").cpu().numpy().reshape(2 * nk + 1, nk), De) < 1e-12
     assert rel(dev.workspace_view(0, "Dh").cpu().numpy().reshape(2 * nk + 1, nk), Dh) < 1e-12
+    # carrier-carrier + phonon rates of the step (regrouped sums on the device)
+    W = dev.workspace_view(0, "W").cpu().numpy().reshape(4, nk)
+    for a, b in zip(W, ref.last_W):
+        assert rel(a, b) < 1e-12
 
 
 @pytest.mark.parametrize("mb_kw", [

@@ -647,3 +647,46 @@ def test_obe_mode_is_two_level_bloch():
         e2 = _obe_rel_err(0.5e-17, 80, k)
         assert e1 < 0.03
         assert 1.7 < e1 / e2 < 2.3
+
+
+@pytest.mark.parametrize("lorentz", [False, True])
+def test_collision_regrouping_matches_mbce_mbch(lorentz):
+    """The device evaluates each carrier-carrier line shape once per (species, kind)
+    and spreads it over Win[o] and Wout[t] (o = output momentum, t = partner):
+    S(o, t) = sum_j C[o,j,t] (1-n[j]) n[j+t-o]  etc.  Check that regrouping against
+    the loop-form MBCE/MBCH (PSTD3D/coulomb.py:957-1071) on random populations."""
+    w, _ = make_wire(nk=12, mb=O.MBFlags(LorentzDelta=lorentz))
+    Nk = w.Nk
+    rng = np.random.default_rng(3)
+    ne0, nh0 = rng.uniform(0, 1, Nk), rng.uniform(0, 1, Nk)
+    v = rng.uniform(0.5, 2, (3, Nk))
+    idx = np.abs(np.arange(Nk)[:, None] - np.arange(Nk)[None, :])
+    VC = np.stack([v[c][idx] for c in range(3)], axis=2)
+    Wi, Wo, Hi, Ho = (np.zeros(Nk) for _ in range(4))
+    w.MBCE(ne0, nh0, VC, Wi, Wo)
+    w.MBCH(ne0, nh0, VC, Hi, Ho)
+    ne, nh = np.r_[0, ne0], np.r_[0, nh0]
+
+    def C(kind, a, b, c):  # read C_kind[a-1, b-1, c-1] for 1-based a, b, c
+        return float(w._ceh_read(np.array(a - 1), np.array(b - 1), np.array(c - 1), kind))
+
+    R = np.zeros((4, Nk))
+    for o in range(1, Nk + 1):
+        for t in range(1, Nk + 1):
+            Sx = Ss = Hx = Hs = 0.0
+            for j in range(1, Nk + 1):
+                if 1 <= j + t - o <= Nk:
+                    Sx += C("eh", o, j, t) * (1 - nh[j]) * nh[j + t - o]
+                    Hx += C("eh", j, o, j + t - o) * (1 - ne[j]) * ne[j + t - o]
+                if 1 <= o + j - t <= Nk:
+                    Ss += C("ee", o, j, t) * (1 - ne[j]) * ne[o + j - t]
+                    Hs += C("hh", o, j, t) * (1 - nh[j]) * nh[o + j - t]
+            q = abs(o - t)
+            S = v[0][q] ** 2 * Sx + v[1][q] ** 2 * Ss
+            H = v[0][q] ** 2 * Hx + v[2][q] ** 2 * Hs
+            R[0, o - 1] += ne[t] * S
+            R[1, t - 1] += (1 - ne[o]) * S
+            R[2, o - 1] += nh[t] * H
+            R[3, t - 1] += (1 - nh[o]) * H
+    for a, b in zip(R, (Wi, Wo, Hi, Ho)):
+        assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
