@@ -3,7 +3,14 @@
 Workload (default): 3-D PSTD on a 128^3 complex grid with a 12-cell CPML on all
 faces, additive soft plane source, spectral curl via 3-D FFTs, and one
 embedded quantum wire (Nk = 256, GaAs parameters, screened = 0 "bare") driven
-by the point field and feeding its current back -- 1000 leapfrog steps.
+by the point field, its current line formed, transformed and added to the E
+update every step -- 1000 leapfrog steps.
+
+The wire runs with the reference's ignorewire switch (PSTD3D/SBEs.py:5993-5997):
+every kernel of the step runs, and the wire's P and rho outputs are zeroed before
+they form the current.  With the back-action on, the point-current loop of this
+configuration grows exponentially (about 1.5e16 1/s, the same at dt/2 and dt/10,
+the same in the FP64 oracle) and overflows after ~45 steps; --feedback runs it so.
 
 Metric: k-point.steps/s = (sum over wires of Nk) x executed steps / wall time,
 inputs resident in HBM, timed region bracketed by synchronize on both sides.
@@ -39,10 +46,14 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--feedback", action="store_true", help="keep the wire's P/rho in the current (diverges)")
     return ap.parse_args()
 
 
-def setup(grid, nk, npml, precision):
+AMP = 1e7  # soft-source amplitude (V/m)
+
+
+def setup(grid, nk, npml, precision, feedback=False):
     from pulsesuite_amd.PSTD3D import PSTD3DPropagator
     from pulsesuite_amd.rhoPJ import QWArray
     from pulsesuite_amd.typepulse import ps
@@ -54,8 +65,8 @@ def setup(grid, nk, npml, precision):
     epsr = 9.1
     dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
     sp = ss(3, grid, grid, grid, dx, dx, dx, epsr)
-    pulse = ps(800e-9, 1e8, 100e-15, 50e-15, 0.0)
-    mb = MBFlags(Screened=False)  # bare Coulomb for this config
+    pulse = ps(800e-9, AMP, 100e-15, 50e-15, 0.0)
+    mb = MBFlags(Screened=False, ignorewire=not feedback)  # bare Coulomb for this config
     qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, precision=precision)
     prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type="cpml",
                             calc_j=qa, npml=npml, check_every=0)
@@ -125,7 +136,7 @@ def precision_errors(p64, q64, pmx, qmx):
     return out
 
 
-def cpu_baseline(grid, nk, npml, steps):
+def cpu_baseline(grid, nk, npml, steps, feedback=False):
     """The FP64 NumPy oracle on the same workload for `steps` steps (bounded sample)."""
     from oracle import pstd as OP
     from oracle.coupling import OracleCoupling
@@ -135,8 +146,9 @@ def cpu_baseline(grid, nk, npml, steps):
     epsr = 9.1
     dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
     sp = OP.Space(3, grid, grid, grid, dx, dx, dx, epsr)
-    oc = OracleCoupling(sp, 800e-9, 0.0, Nw=1, qw=QWParams(L=L_for_nk(nk)), mb=MBFlags(Screened=False))
-    prop = OP.Propagator(sp, OP.Time(0.0, 1.0, dt, 0), OP.Pulse(800e-9, 1e8, 100e-15, 50e-15, 0.0), "soft", "cpml",
+    oc = OracleCoupling(sp, 800e-9, 0.0, Nw=1, qw=QWParams(L=L_for_nk(nk)),
+                        mb=MBFlags(Screened=False, ignorewire=not feedback))
+    prop = OP.Propagator(sp, OP.Time(0.0, 1.0, dt, 0), OP.Pulse(800e-9, AMP, 100e-15, 50e-15, 0.0), "soft", "cpml",
                          calc_j=oc.calc_j, npml=npml)
     prop.step()  # warm-up
     t0 = time.perf_counter()
@@ -158,7 +170,7 @@ def main():
     results = {}
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     for prec in precs:
-        prop, qa = setup(a.grid, a.nk, a.npml, prec)
+        prop, qa = setup(a.grid, a.nk, a.npml, prec, a.feedback)
         el = time_steps(prop, a.steps, a.warmup)
         results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop, qa=qa)
     main_prec = precs[-1]
@@ -198,7 +210,7 @@ def main():
                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
                             "avg_ms": ms}
     if a.cpu_steps > 0:
-        v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps)
+        v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps, a.feedback)
         line["cpu_baseline"] = {"value": v, "unit": "k-point*steps/s", "cores": len(os.sched_getaffinity(0)),
                                 "kind": "port", "sample": f"{a.cpu_steps} steps of the same workload, "
                                 f"{el:.1f} s, NumPy/SciPy FP64 oracle"}

@@ -141,7 +141,15 @@ class PSTD3DPropagator:
 
     def step(self, stream=None):
         """One leapfrog step (PSTD3D/PSTD3D.py:523-627)."""
-        sp, tm = self.space, self.time
+        self.step_e_half(stream)
+        if self.calc_j is not None:
+            self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+        self.step_b_half(stream)
+
+    def step_e_half(self, stream=None):
+        """E update + CPML-E half step (with the soft source on Ey and the wire's point
+        sample), then t += dt/2: the part of a step before the calc_j hook."""
+        tm = self.time
         g = self.ctx.g
         g.dt = tm.dt
         g.v2 = self.v2
@@ -157,8 +165,13 @@ class PSTD3DPropagator:
         self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
                               src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
         tm.t += tm.dt / 2.0
-        if self.calc_j is not None:
-            self.calc_j(sp, tm, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+
+    def step_b_half(self, stream=None):
+        """B update + CPML-B half step (soft source on Bz), then t += dt/2: the part of a
+        step after the calc_j hook."""
+        tm = self.time
+        P = _lib.ptr
+        st = _lib.stream_ptr(stream)
         _lib.check(_lib.lib.pss_update_b3d(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(self.Bx), P(self.By),
                                            P(self.Bz), st), "pss_update_b3d")
         self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, dt=tm.dt,

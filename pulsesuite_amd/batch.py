@@ -119,15 +119,7 @@ class WireBatch:
     # ------------------------------------------------------------ planning
     def ranges(self, nb):
         """Contiguous slot ranges of roughly equal work (sum Nk^3), at most nb."""
-        w = self.nk.astype(np.float64) ** 3
-        cum = np.cumsum(w)
-        cuts = [0]
-        for b in range(1, nb):
-            c = int(np.searchsorted(cum, cum[-1] * b / nb, side="left")) + 1
-            if cuts[-1] < c < self.N:
-                cuts.append(c)
-        cuts.append(self.N)
-        return [(a, b) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+        return balanced_ranges(self.nk, nb)
 
     def _ensure_streams(self, ns):
         import torch
@@ -258,6 +250,22 @@ class WireBatch:
 
     def memory_bytes(self):
         return batch_bytes([s.Nk for s in self.specs], nrr=self.dev.nrr[0])
+
+
+def balanced_ranges(nks, nb):
+    """Cut a list of wires (sorted by Nk) into at most nb contiguous ranges of roughly
+    equal work, sum of Nk^3."""
+    n = len(nks)
+    if n == 0:
+        return []
+    cum = np.cumsum(np.asarray(nks, dtype=np.float64) ** 3)
+    cuts = [0]
+    for b in range(1, nb):
+        c = int(np.searchsorted(cum, cum[-1] * b / nb, side="left")) + 1
+        if cuts[-1] < c < n:
+            cuts.append(c)
+    cuts.append(n)
+    return [(a, b) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
 
 def wire_bytes(nk, nrr=16):

@@ -115,12 +115,21 @@ class QWArray:
 
     def __call__(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
         """calc_j hook: wires step on the sampled field, J line -> k-space factors."""
+        self.drive(space, time)
+        self.wires.step()
+        self.deposit(space, time)
+
+    def drive(self, space, time):
+        """Wire lines <- the point sample taken during the E half step."""
         st = _lib.stream_ptr()
         W = self.wires
         for i in range(self.Nw):
             _lib.check(_lib.lib.pss_fill_line(_lib.ptr(self.sample), _lib.ptr(W.E_in[i]), space.Nx, st), "fill")
         W.set_dt(time.dt)
-        W.step()
+
+    def deposit(self, space, time):
+        """J line from the wires' new P and rho, then its FFT along x."""
+        st = _lib.stream_ptr()
         _lib.check(_lib.lib.pss_jline(_lib.ptr(self.P), _lib.ptr(self.Rho), _lib.ptr(self.Pold), _lib.ptr(self.RhoOld),
                                       self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), st), "jline")
         if space.Nx > 1:
