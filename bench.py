@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--feedback", action="store_true", help="keep the wire's P/rho in the current (diverges)")
+    ap.add_argument("--batched-wires", type=int, default=64, help="wires of the batched run (0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--batched-nk-max", type=int, default=1024)
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (1/s)")
     return ap.parse_args()
 
 
@@ -110,6 +115,63 @@ def fft_pass_timing(prop, reps=50):
     return ms, bytes_
 
 
+def batched(nw, steps, nk_max=1024, seed=0):
+    """Batched wires of uneven Nk (uniform over even values in [64, nk_max]), pump
+    log-uniform over 3 decades, detuning +-50 meV; one captured launch per step with
+    the autotuned plan; FP64 and mixed on the same batch, errors of mixed vs FP64."""
+    import torch
+
+    from pulsesuite_amd.batch import WireBatch, sample_specs
+    from pulsesuite_amd.wire_model import MBFlags
+
+    specs = sample_specs(nw, seed=seed, nk_lo=64, nk_hi=nk_max, e_lo=1e6, e_hi=1e9)
+    out = {"workload": f"{nw} wires, Nk uniform even [64,{nk_max}], E0 log-uniform [1e6,1e9] V/m, detuning "
+                       f"+-50 meV, screened, dt 10 as, seed {seed}", "wires": nw, "steps": steps,
+           "kpoints": int(sum(s.Nk for s in specs))}
+    states = {}
+    for prec in ("fp64", "mixed"):
+        b = WireBatch(specs, precision=prec, dt=10e-18, mb_flags=MBFlags(Screened=True), nrr=16)
+        print(f"[bench] batched {prec}: autotune", file=sys.stderr, flush=True)
+        best, times = b.autotune(reps=1)
+        b.run(1)  # capture with the tuned plan (one eager step)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[prec] = {"value": b.kpoint_steps(steps) / el, "ms_per_step": 1e3 * el / steps,
+                     "plan": f"buckets={best.buckets} streams={best.streams}"}
+        states[prec] = [b.get_state(i) for i in range(0, nw, max(1, nw // 8))]
+        del b
+        torch.cuda.empty_cache()
+    errs = {}
+    for k in ("p2", "C2", "D2"):
+        num = sum(np.linalg.norm(m[k] - f[k]) ** 2 for m, f in zip(states["mixed"], states["fp64"]))
+        den = sum(np.linalg.norm(f[k]) ** 2 for f in states["fp64"])
+        errs[k[0]] = float(np.sqrt(num / max(den, 1e-300)))
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed_rel_err_vs_fp64"] = errs
+    return out
+
+
+SERVE = dict(nk_lo=64, nk_hi=512, st_lo=200, st_hi=2000, e_lo=1e6, e_hi=1e8, seed=0)
+
+
+def serving(nreq, rate, precision="mixed"):
+    """Continuous-batching run: a seeded Poisson stream of wire + 1-D PSTD requests
+    (1024 cells, 16-cell CPML), admitted by HBM budget; throughput and latency."""
+    from pulsesuite_amd.serve import Server, poisson_requests
+
+    reqs = poisson_requests(nreq, rate, **SERVE)
+    srv = Server(hbm_budget=64 << 30, max_slots=64, precision=precision)
+    srv.run(reqs)
+    out = srv.summary()
+    out.update(requests=nreq, rate_per_s=rate, precision=precision, max_in_flight=srv.max_active,
+               workload=f"Poisson {rate}/s, Nk log-uniform [{SERVE['nk_lo']},{SERVE['nk_hi']}], steps log-uniform "
+                        f"[{SERVE['st_lo']},{SERVE['st_hi']}], wire + 1-D PSTD 1024 cells npml 16, seed {SERVE['seed']}")
+    return out
+
+
 def rel_err(a, b):
     import torch
 
@@ -170,6 +232,7 @@ def main():
     results = {}
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     for prec in precs:
+        print(f"[bench] coupled step {prec}", file=sys.stderr, flush=True)
         prop, qa = setup(a.grid, a.nk, a.npml, prec, a.feedback)
         el = time_steps(prop, a.steps, a.warmup)
         results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop, qa=qa)
@@ -209,6 +272,11 @@ def main():
         line["roofline"] = {"bound": "hbm", "kernel": "k_fft2<8,16> x3 (one 3-D FFT)", "achieved": ach / 1e9,
                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
                             "avg_ms": ms}
+    if a.batched_wires > 0:
+        line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max)
+    if a.serve_requests > 0:
+        print("[bench] serving", file=sys.stderr, flush=True)
+        line["serving"] = serving(a.serve_requests, a.serve_rate)
     if a.cpu_steps > 0:
         v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps, a.feedback)
         line["cpu_baseline"] = {"value": v, "unit": "k-point*steps/s", "cores": len(os.sched_getaffinity(0)),

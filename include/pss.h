@@ -74,12 +74,17 @@ typedef struct pss_wire_desc {
   /* per-step input / output (complex [nrr]) */
   const double *Exx, *Eyy, *Ezz;
   double *Pxx, *Pyy, *Pzz, *Rho;
-  /* workspace: pss_sbe_workspace_bytes(Nk, Nr, nrr) bytes, 256-byte aligned */
+  /* workspace: pss_sbe_workspace_bytes(Nk, Nr, nrr, precision) bytes, 256-byte aligned */
   double *ws;
 } pss_wire_desc;
 
 /* Bytes of workspace one wire needs. */
-int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr);
+int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr, int32_t precision);
+
+/* Split-K factor of the mixed-precision contraction launch (power of two <= 16;
+ * 0 = chosen per launch from the tile count).  Process-wide; a tuning knob with no
+ * reference counterpart, set by the batch autotuner. */
+int pss_sbe_tune(int32_t ksplit);
 
 /*
  * One SBE time step of `nwires` wires whose descriptors sit in device memory

@@ -131,7 +131,7 @@ class DeviceWires:
             st["C"][b].diagonal().copy_(fe)
             st["D"][b].diagonal().copy_(fh)
         self.state.append(st)
-        wsb = _lib.lib.pss_sbe_workspace_bytes(Nk, Nr, nrr)
+        wsb = _lib.lib.pss_sbe_workspace_bytes(Nk, Nr, nrr, self.prec_code)
         ws = torch.zeros(wsb // 8, dtype=torch.float64, device=dev)
         self.ws.append(ws)
         d = self.descs[i]
@@ -227,9 +227,16 @@ def ws_layout(Nk, Nr, nrr):
             ("Dh", (2 * Nk + 1) * Nk), ("H", 3 * Nk * Nk * 2), ("W", 4 * Nk), ("n3", 4 * Nk), ("red", 32),
             ("ds", 4 * nd * 2), ("pr", 3 * Nr), ("rho", 2 * Nr * 2), ("spb", 5 * Nr), ("outr", 5 * nrr),
             ("cp", 2 * Nk * Nk)]
+    np64 = (Nk + 63) // 64 * 64
+    ksm = 16 if np64 <= 128 else 8 if np64 <= 256 else 4 if np64 <= 512 else 1
     out = {}
     o = 0
     for n, sz in segs:
+        out[n] = o
+        o += al(sz)
+    # mixed precision only: FP16 operand planes, split-K partial tiles
+    out["total_fp64"] = o
+    for n, sz in (("sp", 16 * np64 * np64), ("pk", 6 * ksm * Nk * Nk if ksm > 1 else 0)):
         out[n] = o
         o += al(sz)
     out["total"] = o

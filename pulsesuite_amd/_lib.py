@@ -62,7 +62,8 @@ def _load():
             "(or __graft_entry__.build()); there is no CPU fallback")
     lib = C.CDLL(LIB_PATH)
     sig = {
-        "pss_sbe_workspace_bytes": (C.c_int64, [_i, _i, _i]),
+        "pss_sbe_workspace_bytes": (C.c_int64, [_i, _i, _i, _i]),
+        "pss_sbe_tune": (C.c_int, [_i]),
         "pss_sbe_step": (C.c_int, [_vp, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
@@ -87,7 +88,7 @@ def _load():
 
 lib = _load()
 
-EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
+EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
            "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform",
            "pss_last_error", "pss_version")
 

@@ -11,8 +11,8 @@ graph (torch.cuda.CUDAGraph) and replayed; the drive time and the state
 rotation live in device counters, so replays need no host work.
 
 The plan (bucket count, stream count) is chosen by :meth:`WireBatch.autotune`,
-which times each candidate's captured step on the batch itself and restores the
-state afterwards.  A wire's arithmetic does not depend on the plan (every kernel
+which times each candidate's step on the batch itself and restores the state
+afterwards.  A wire's arithmetic does not depend on the plan (every kernel
 indexes a wire's work by its own Nk), so results are identical under every plan.
 """
 
@@ -211,18 +211,24 @@ class WireBatch:
                     out.append(Plan(nb, ns))
         return out
 
-    def autotune(self, reps=3, candidates=None):
-        """Time each plan's captured step on this batch, keep the fastest, restore the state."""
+    def autotune(self, reps=3, candidates=None, graph=False):
+        """Time each plan's step on this batch (eager launches, or captured graphs with
+        graph=True), keep the fastest, restore the state."""
         import torch
 
         snap = self.snapshot()
         times = {}
         for plan in candidates or self.candidates():
-            g = self.capture(plan)
+            if graph:
+                g = self.capture(plan)
+                run = g.replay
+            else:
+                self._launch(plan)  # warm-up
+                run = lambda p=plan: self._launch(p)  # noqa: E731
             torch.cuda.synchronize(self.device)
             t0 = time.perf_counter()
             for _ in range(reps):
-                g.replay()
+                run()
             torch.cuda.synchronize(self.device)
             times[plan] = (time.perf_counter() - t0) / reps
             self.restore(snap)
@@ -268,10 +274,11 @@ def balanced_ranges(nks, nb):
     return [(a, b) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
 
 
-def wire_bytes(nk, nrr=16):
+def wire_bytes(nk, nrr=16, precision="mixed"):
     """Device bytes one wire holds: 9 Nk x Nk complex state matrices plus its workspace."""
     nr = 2 * nk + 2
-    return 9 * 16 * nk * nk + int(_lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr))
+    code = _lib.FP64 if precision == "fp64" else _lib.MIXED
+    return 9 * 16 * nk * nk + int(_lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr, code))
 
 
 def batch_bytes(nks, nrr=16):

@@ -32,7 +32,8 @@ __device__ inline double ddiv(double n, double d) {
 
 // ---------------------------------------------------------------- workspace
 struct WS {
-  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, total;
+  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, sp, pk,
+      total_fp64, total;
 };
 
 __host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
@@ -62,6 +63,12 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.spb = o;  o += al(5 * (int64_t)Nr);
   w.outr = o; o += al(5 * (int64_t)nrr);
   w.cp = o;   o += al(2 * (int64_t)Nk * Nk);
+  // mixed precision only: FP16 operand planes and split-K partial tiles
+  w.total_fp64 = o;
+  const int64_t np = (Nk + 63) / 64 * 64;
+  const int ksm = np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
+  w.sp = o;   o += al(16 * np * np);
+  w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);
   w.total = o;
   return w;
 }
@@ -786,182 +793,275 @@ __device__ inline void split16(double x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - (double)hi);
 }
 
-__global__ void __launch_bounds__(256) k_rhs_gemm_mixed(const pss_wire_desc* __restrict__ descs) {
-  const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
+// ---- operand planes.  k_split writes, for every source matrix (slot m: p, C, D,
+// Heh, Hee with zero diagonal, Hhh with zero diagonal, Mh = i hbar Dh-matrix,
+// Q = Me^T = i hbar De-matrix), the scaled FP16 hi/lo planes (re hi, re lo, im hi,
+// im lo) in two orientations: o = 0 row-major X[i][k], o = 1 transposed X[k][i].
+// Planes are NP x NP (NP = Nk rounded up to 64) with zeros outside Nk, so the
+// contraction kernel loads whole 16-wide k segments without bounds checks.
+__host__ __device__ inline int np64(int Nk) { return (Nk + 63) / 64 * 64; }
+
+__host__ __device__ inline int ksplit_max(int Nk) {
+  const int np = np64(Nk);
+  return np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
+}
+
+__device__ inline _Float16* plane(double* ws, const WS& L, int NP, int m, int o, int q) {
+  return (_Float16*)(ws + L.sp) + ((int64_t)((m * 2 + o) * 4 + q)) * NP * NP;
+}
+
+// element of source slot m at (i, k), i, k < Nk, as a complex value (before scaling)
+__device__ inline cplx slot_value(int m, int i, int k, int Nk, const Bufs& B, const cplx* H, const double* Dh,
+                                  const double* De) {
+  const int64_t NN = (int64_t)Nk * Nk;
+  const int64_t o = (int64_t)i * Nk + k;
+  switch (m) {
+    case 0: return B.p2[o];
+    case 1: return B.C2[o];
+    case 2: return B.D2[o];
+    case 3: return H[o];
+    case 4: return i == k ? mk(0, 0) : H[NN + o];
+    case 5: return i == k ? mk(0, 0) : H[2 * NN + o];
+    case 6: return i == k ? mk(0, 0) : mk(0.0, kHbar * Dh[(int64_t)(k - i + Nk) * Nk + i]);  // Mh[i][k]
+    default: return i == k ? mk(0, 0) : mk(0.0, kHbar * De[(int64_t)(i - k + Nk) * Nk + k]);  // Me^T[i][k]
+  }
+}
+
+// grid (NP/32, NP/32, 8 * nwires): one 32 x 32 tile of one slot, both orientations
+__global__ void __launch_bounds__(256) k_split(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z >> 3, m = blockIdx.z & 7;
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || d.precision != PSS_MIXED) return;
-  const int Nk = d.Nk;
-  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
-  if (m0 >= Nk || n0 >= Nk) return;
+  if (m >= 6 && !(d.flags & PSS_OFFDIAGDPH)) return;
+  const int Nk = d.Nk, NP = np64(Nk);
+  const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+  if (k0 >= NP || i0 >= NP) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
-  const int64_t NN = (int64_t)Nk * Nk;
   const cplx* H = (const cplx*)(ws + L.H);
-  const double* Dh = ws + L.Dh;
-  const double* De = ws + L.De;
-  const double* mx = ws + L.red + 8;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-
-  __shared__ double lds_raw[8192];  // 64 KB: fp16 staging during the loop, FP64 partial tiles after
-  // per wave staging: A[hr, lr, hi, li][32][16], B[hr, lr, hi, li][32][16] halfs = 8 KB
-  _Float16* stg = (_Float16*)lds_raw + wv * 4096;
-  _Float16* As = stg;          // [4][32][16]
-  _Float16* Bs = stg + 2048;   // [4][32][16]
-
-  f4 ar_[2][2], ai_[2][2];
-  double pr[2][2][4], pi_[2][2][4];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 4; ++v) {
-        ar_[a][b][v] = 0.f; ai_[a][b][v] = 0.f;
-        pr[a][b][v] = 0.0; pi_[a][b][v] = 0.0;
-      }
-  const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
-  const int nkc = (Nk + 15) / 16;
-  const int nitems = nterms * nkc;
-  const int niter = (nitems + 3) / 4;
-  int cur_t = -1;
-  double cur_inv = 0.0;
-  for (int it = 0; it < niter; ++it) {
-    const int item = it * 4 + wv;
-    const bool has = item < nitems;
-    const int t = has ? item / nkc : cur_t;
-    if (has && t != cur_t) {
-      // fold the FP32 accumulators of the previous term into FP64
-      if (cur_t >= 0)
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) {
-              pr[a][b][v] += (double)ar_[a][b][v] * cur_inv;
-              pi_[a][b][v] += (double)ai_[a][b][v] * cur_inv;
-              ar_[a][b][v] = 0.f;
-              ai_[a][b][v] = 0.f;
-            }
-      const Term T = kTerms[out][t];
-      const double sa = pow2_scale(mx[slot_of(T.a, T.aop)]), sb = pow2_scale(mx[slot_of(T.b, T.bop)]);
-      cur_inv = 1.0 / (sa * sb);
-      cur_t = t;
+  const double sc = pow2_scale(ws[L.red + 8 + m]);
+  __shared__ _Float16 T[4][32][33];
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int ii = e >> 5, kk = e & 31;
+    const int i = i0 + ii, k = k0 + kk;
+    cplx v = mk(0, 0);
+    if (i < Nk && k < Nk) v = slot_value(m, i, k, Nk, B, H, ws + L.Dh, ws + L.De);
+    _Float16 h[4];
+    split16(v.x * sc, h[0], h[1]);
+    split16(v.y * sc, h[2], h[3]);
+    for (int q = 0; q < 4; ++q) {
+      plane(ws, L, NP, m, 0, q)[(int64_t)i * NP + k] = h[q];
+      T[q][kk][ii] = h[q];
     }
-    if (has) {
-      const int k0 = (item % nkc) * 16;
-      const Term T = kTerms[out][t];
-      const cplx* Ap = src_ptr(T.a, B, H, NN);
-      const cplx* Bp = src_ptr(T.b, B, H, NN);
-      const double sg = T.neg ? -1.0 : 1.0;
-      const double sa = pow2_scale(mx[slot_of(T.a, T.aop)]) * sg, sb = pow2_scale(mx[slot_of(T.b, T.bop)]);
-      const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
-      const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
-      const bool adiag0 = (T.a == S_HEE || T.a == S_HHH), bdiag0 = (T.b == S_HEE || T.b == S_HHH);
-      for (int e = lane; e < 512; e += 64) {
-        int mm, kk;
-        if (!aT) { mm = e >> 4; kk = e & 15; } else { kk = e >> 5; mm = e & 31; }
-        cplx v = load_op(Ap, T.aop, m0 + mm, k0 + kk, Nk, Dh, De);
-        if (adiag0 && m0 + mm == k0 + kk) v = mk(0, 0);
-        _Float16 h, l;
-        split16(v.x * sa, h, l);
-        As[(0 * 32 + mm) * 16 + kk] = h;
-        As[(1 * 32 + mm) * 16 + kk] = l;
-        split16(v.y * sa, h, l);
-        As[(2 * 32 + mm) * 16 + kk] = h;
-        As[(3 * 32 + mm) * 16 + kk] = l;
-      }
-      for (int e = lane; e < 512; e += 64) {
-        int kk, nn;
-        if (!bT) { kk = e >> 5; nn = e & 31; } else { nn = e >> 4; kk = e & 15; }
-        cplx v = load_op(Bp, T.bop, k0 + kk, n0 + nn, Nk, Dh, De);
-        if (bdiag0 && k0 + kk == n0 + nn) v = mk(0, 0);
-        _Float16 h, l;
-        split16(v.x * sb, h, l);
-        Bs[(0 * 32 + nn) * 16 + kk] = h;
-        Bs[(1 * 32 + nn) * 16 + kk] = l;
-        split16(v.y * sb, h, l);
-        Bs[(2 * 32 + nn) * 16 + kk] = h;
-        Bs[(3 * 32 + nn) * 16 + kk] = l;
-      }
-    }
-    __syncthreads();
-    if (has) {
-      const int li = lane & 15, kq = 4 * (lane >> 4);
-      h4 fa[2][4], fb[2][4];
-      for (int a = 0; a < 2; ++a)
-        for (int c = 0; c < 4; ++c) {
-          fa[a][c] = *(const h4*)&As[(c * 32 + a * 16 + li) * 16 + kq];
-          fb[a][c] = *(const h4*)&Bs[(c * 32 + a * 16 + li) * 16 + kq];
-        }
-      for (int a = 0; a < 2; ++a) {
-        const h4 nhi = -fa[a][2], nli = -fa[a][3];
-        for (int b = 0; b < 2; ++b) {
-          f4 r = ar_[a][b], i = ai_[a][b];
-          // re*re - im*im
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][0], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][1], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][1], fb[b][0], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nhi, fb[b][2], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nhi, fb[b][3], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nli, fb[b][2], r, 0, 0, 0);
-          // re*im + im*re
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][2], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][0], fb[b][3], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][1], fb[b][2], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][2], fb[b][0], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][2], fb[b][1], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a][3], fb[b][0], i, 0, 0, 0);
-          ar_[a][b] = r;
-          ai_[a][b] = i;
-        }
-      }
-    }
-    __syncthreads();
   }
-  if (cur_t >= 0)
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b)
-        for (int v = 0; v < 4; ++v) {
-          pr[a][b][v] += (double)ar_[a][b][v] * cur_inv;
-          pi_[a][b][v] += (double)ai_[a][b][v] * cur_inv;
-        }
-  // partial tiles: v_mfma_f32_16x16x16f16 result lane l, element v -> row 4(l/16)+v, col l%16
-  double* P = lds_raw + wv * 2048;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 4; ++v) {
-        const int i = a * 16 + 4 * (lane >> 4) + v;
-        const int j = b * 16 + (lane & 15);
-        P[2 * (i * 32 + j)] = pr[a][b][v];
-        P[2 * (i * 32 + j) + 1] = pi_[a][b][v];
-      }
   __syncthreads();
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int kk = e >> 5, ii = e & 31;
+    for (int q = 0; q < 4; ++q) plane(ws, L, NP, m, 1, q)[(int64_t)(k0 + kk) * NP + i0 + ii] = T[q][kk][ii];
+  }
+}
+
+// operand mapping of a term: A side needs op(A)[i][k] with k contiguous, B side
+// op(B)[k][j] with k contiguous for fixed j
+__device__ inline void a_side(int src, int op, int& m, int& o, bool& cj) {
+  m = slot_of(src, op);
+  o = (op == OP_T || op == OP_H) ? 1 : 0;
+  cj = (op == OP_C || op == OP_H);
+}
+__device__ inline void b_side(int src, int op, int& m, int& o, bool& cj) {
+  m = slot_of(src, op);
+  o = (op == OP_N || op == OP_C || op == OP_MET) ? 1 : 0;
+  cj = (op == OP_C || op == OP_H);
+}
+
+// leapfrog epilogue of one element with the products' sum S (mixed path): exact
+// diagonal parts Hhh[kh,kh] p + p Hee[ke,ke] (+ Heh^T - i hbar gamma p) for dp,
+// X (Hd[j,j] - Hd[i,i]) for dC / dD; R = S / (i hbar); X3 = X1 + 2 dt R.
+__device__ inline void mixed_epilogue(const pss_wire_desc& d, const Bufs& B, const cplx* H, const double* gam,
+                                      int out, int i, int j, cplx S) {
+  const int Nk = d.Nk;
+  const int64_t NN = (int64_t)Nk * Nk;
+  const int64_t o = (int64_t)i * Nk + j;
   const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
   const cplx* X2 = out == 0 ? B.p2 : out == 1 ? B.C2 : B.D2;
   cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
   cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
-  const double* gam = ws + L.gam;
   const cplx* Hee = H + NN;
   const cplx* Hhh = H + 2 * NN;
-  for (int q = threadIdx.x; q < 1024; q += 256) {
-    const int li = q >> 5, lj = q & 31;
-    const int i = m0 + li, j = n0 + lj;
-    if (i >= Nk || j >= Nk) continue;
-    cplx S = mk(0, 0);
-    for (int w = 0; w < 4; ++w) S = S + mk(lds_raw[w * 2048 + 2 * q], lds_raw[w * 2048 + 2 * q + 1]);
-    const int64_t o = (int64_t)i * Nk + j;
-    const cplx x2 = X2[o];
-    if (out == 0) {
-      // exact diagonal parts: Hhh[kh,kh] p[kh,ke] + p[kh,ke] Hee[ke,ke]; + Heh^T - i hbar gamma p
-      const cplx hT = H[(int64_t)j * Nk + i];
-      const double g = kHbar * (gam[j] + gam[Nk + i]);
-      S = S + Hhh[(int64_t)i * Nk + i] * x2 + x2 * Hee[(int64_t)j * Nk + j] + hT - mk(-g * x2.y, g * x2.x);
-    } else {
-      const cplx* Hd = out == 1 ? Hee : Hhh;
-      S = S + x2 * (Hd[(int64_t)j * Nk + j] - Hd[(int64_t)i * Nk + i]);
-    }
-    const cplx R = div_ih(S, kHbar);
-    const cplx x1 = X1[o];
-    const cplx x3 = x1 + (R * d.dt) * 2.0;
-    Xn[o] = x3;
-    if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+  const cplx x2 = X2[o];
+  if (out == 0) {
+    const cplx hT = H[(int64_t)j * Nk + i];
+    const double g = kHbar * (gam[j] + gam[Nk + i]);
+    S = S + Hhh[(int64_t)i * Nk + i] * x2 + x2 * Hee[(int64_t)j * Nk + j] + hT - mk(-g * x2.y, g * x2.x);
+  } else {
+    const cplx* Hd = out == 1 ? Hee : Hhh;
+    S = S + x2 * (Hd[(int64_t)j * Nk + j] - Hd[(int64_t)i * Nk + i]);
   }
+  const cplx R = div_ih(S, kHbar);
+  const cplx x1 = X1[o];
+  const cplx x3 = x1 + (R * d.dt) * 2.0;
+  Xn[o] = x3;
+  if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+}
+
+// ---- contraction: one 64 x 64 output tile per workgroup, 4 waves (32 x 32 each,
+// 2 x 2 blocks of 16 x 16), A (64 x 16) and B (16 x 64) FP16 planes of one k chunk
+// staged in LDS and shared by the waves.  12 v_mfma_f32_16x16x16f16 per block and
+// chunk: re += ar br - ai bi, im += ar bi + ai br, each product hi*hi + hi*lo + lo*hi.
+// FP32 partials are folded into FP64 (times 1/(sa sb)) every 8 chunks and at term
+// ends.  The (term, chunk) list is cut into `ks` contiguous slices (blockIdx.z);
+// with ks == 1 the epilogue runs here, else FP64 partial tiles go to the pk buffer.
+constexpr int kLdk = 20;  // padded k stride of a staged row (halves): conflict-free fragment reads
+
+__global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict__ descs, int ks) {
+  const int zz = blockIdx.z;
+  const int slice = zz % ks, out = (zz / ks) % 3, wire = zz / (3 * ks);
+  const pss_wire_desc& d = descs[wire];
+  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  const int Nk = d.Nk, NP = np64(Nk);
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  if (m0 >= NP || n0 >= NP) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const double* mx = ws + L.red + 8;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+
+  __shared__ _Float16 As[4][64][kLdk];
+  __shared__ _Float16 Bs[4][64][kLdk];
+
+  double accr[2][2][4], acci[2][2][4];
+  f4 fr[2][2], fi[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      fr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      fi[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int v = 0; v < 4; ++v) { accr[a][b][v] = 0.0; acci[a][b][v] = 0.0; }
+    }
+  const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
+  const int nkc = NP / 16;
+  const int nitems = nterms * nkc;
+  const int it0 = (int)((int64_t)nitems * slice / ks), it1 = (int)((int64_t)nitems * (slice + 1) / ks);
+  // loader role: thread -> (operand, plane, row) with two 16-byte loads of 8 halves
+  const int lop = threadIdx.x >> 7;         // 0 = A, 1 = B
+  const int lq = (threadIdx.x >> 5) & 3;    // plane
+  const int lr = threadIdx.x & 31;          // row pair base: rows lr and lr + 32
+  int cur_t = -1, nacc = 0;
+  double cur_f = 0.0;
+  bool acj = false, bcj = false;
+  const _Float16 *pa = nullptr, *pb = nullptr;
+  auto fold = [&]() {
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 4; ++v) {
+          accr[a][b][v] += (double)fr[a][b][v] * cur_f;
+          acci[a][b][v] += (double)fi[a][b][v] * cur_f;
+          fr[a][b][v] = 0.f;
+          fi[a][b][v] = 0.f;
+        }
+    nacc = 0;
+  };
+  for (int it = it0; it < it1; ++it) {
+    const int t = it / nkc, k0 = (it % nkc) * 16;
+    if (t != cur_t) {
+      if (cur_t >= 0) fold();
+      const Term T = kTerms[out][t];
+      int ma, oa, mb, ob;
+      a_side(T.a, T.aop, ma, oa, acj);
+      b_side(T.b, T.bop, mb, ob, bcj);
+      const double sa = pow2_scale(mx[ma]), sb = pow2_scale(mx[mb]);
+      cur_f = (T.neg ? -1.0 : 1.0) / (sa * sb);
+      cur_t = t;
+      pa = (const _Float16*)(ws + L.sp) + (int64_t)((ma * 2 + oa) * 4) * NP * NP;
+      pb = (const _Float16*)(ws + L.sp) + (int64_t)((mb * 2 + ob) * 4) * NP * NP;
+    }
+    {  // stage A rows m0.., B rows (= output columns) n0.., 16 k values, 4 planes
+      const _Float16* src = (lop == 0 ? pa : pb) + (int64_t)lq * NP * NP;
+      const int r0 = (lop == 0 ? m0 : n0);
+      for (int h = 0; h < 2; ++h) {
+        const int r = lr + 32 * h;
+        const _Float16* g = src + (int64_t)(r0 + r) * NP + k0;
+        const uint4 v0 = *(const uint4*)g;
+        const uint4 v1 = *(const uint4*)(g + 8);
+        _Float16* dst = lop == 0 ? &As[lq][r][0] : &Bs[lq][r][0];
+        *(uint2*)(dst) = make_uint2(v0.x, v0.y);
+        *(uint2*)(dst + 4) = make_uint2(v0.z, v0.w);
+        *(uint2*)(dst + 8) = make_uint2(v1.x, v1.y);
+        *(uint2*)(dst + 12) = make_uint2(v1.z, v1.w);
+      }
+    }
+    __syncthreads();
+    {
+      const int li = lane & 15, kq = (lane >> 4) * 4;
+      h4 a[2][4], bq[2][4];
+      for (int x = 0; x < 2; ++x)
+        for (int q = 0; q < 4; ++q) {
+          a[x][q] = *(const h4*)&As[q][wr * 32 + x * 16 + li][kq];
+          bq[x][q] = *(const h4*)&Bs[q][wc * 32 + x * 16 + li][kq];
+        }
+      if (acj)
+        for (int x = 0; x < 2; ++x) { a[x][2] = -a[x][2]; a[x][3] = -a[x][3]; }
+      if (bcj)
+        for (int x = 0; x < 2; ++x) { bq[x][2] = -bq[x][2]; bq[x][3] = -bq[x][3]; }
+      for (int x = 0; x < 2; ++x) {
+        const h4 nih = -a[x][2], nil = -a[x][3];
+        for (int y = 0; y < 2; ++y) {
+          f4 r = fr[x][y], im = fi[x][y];
+          // re: ar br - ai bi
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][0], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][1], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][1], bq[y][0], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nih, bq[y][2], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nih, bq[y][3], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nil, bq[y][2], r, 0, 0, 0);
+          // im: ar bi + ai br
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][2], im, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][3], im, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][1], bq[y][2], im, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][2], bq[y][0], im, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][2], bq[y][1], im, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][3], bq[y][0], im, 0, 0, 0);
+          fr[x][y] = r;
+          fi[x][y] = im;
+        }
+      }
+    }
+    if (++nacc == 8) fold();
+    __syncthreads();
+  }
+  if (cur_t >= 0) fold();
+  // result element (x, y, v): row m0 + 32 wr + 16 x + 4 (lane / 16) + v, column n0 + 32 wc + 16 y + lane % 16
+  const Bufs B = bufs(d);
+  const cplx* H = (const cplx*)(ws + L.H);
+  const double* gam = ws + L.gam;
+  cplx* pk = (cplx*)(ws + L.pk) + ((int64_t)slice * 3 + out) * Nk * Nk;
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y)
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const int j = n0 + wc * 32 + y * 16 + (lane & 15);
+        if (i >= Nk || j >= Nk) continue;
+        const cplx S = mk(accr[x][y][v], acci[x][y][v]);
+        if (ks == 1) mixed_epilogue(d, B, H, gam, out, i, j, S);
+        else pk[(int64_t)i * Nk + j] = S;
+      }
+}
+
+// sum of the ks partial tiles (fixed order) + epilogue; grid (ceil(NN/256), 3, nwires)
+__global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict__ descs, int ks) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  const int Nk = d.Nk, out = blockIdx.y;
+  const int64_t NN = (int64_t)Nk * Nk;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= NN) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const cplx* pk = (const cplx*)(ws + L.pk);
+  cplx S = mk(0, 0);
+  for (int s = 0; s < ks; ++s) S = S + pk[((int64_t)s * 3 + out) * NN + o];
+  mixed_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
 }
 
 // --------------------------- K7: carrier-carrier + phonon collision rates
@@ -1478,8 +1578,20 @@ __global__ void __launch_bounds__(256) k_drive(const pss_wire_desc* __restrict__
 }  // namespace
 
 // ================================================================== C ABI
-extern "C" int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr) {
-  return ws_layout(Nk, Nr, nrr).total * (int64_t)sizeof(double);
+extern "C" int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr, int32_t precision) {
+  const WS w = ws_layout(Nk, Nr, nrr);
+  return (precision == PSS_MIXED ? w.total : w.total_fp64) * (int64_t)sizeof(double);
+}
+
+namespace {
+int g_ksplit = 0;  // 0: choose per launch
+}
+
+extern "C" int pss_sbe_tune(int32_t ksplit) {
+  if (ksplit < 0 || ksplit > 16 || (ksplit & (ksplit - 1)))
+    return pss_set_msg(2, "pss_sbe_tune: ksplit must be 0 (auto) or a power of two <= 16");
+  g_ksplit = ksplit;
+  return 0;
 }
 
 extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
@@ -1513,7 +1625,23 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   const int t32 = (max_Nk + 31) / 32;
   hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_rhs_gemm_mixed, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
+  {
+    const int np = np64(max_Nk), t64 = np / 64;
+    hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
+    // split-K: enough workgroups to fill the chip, within every wire's partial buffer
+    int ks = g_ksplit;
+    if (ks == 0) {
+      ks = 1;
+      const int64_t tiles = (int64_t)t64 * t64 * 3 * nwires;
+      while (ks < ksplit_max(max_Nk) && tiles * ks < 1024) ks *= 2;
+    }
+    ks = std::min(ks, ksplit_max(max_Nk));
+    hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    if (ks > 1) {
+      const int64_t nn = (int64_t)max_Nk * max_Nk;
+      hipLaunchKernelGGL(k_rhs_epi, dim3((unsigned)((nn + 255) / 256), 3, nwires), dim3(256), 0, s, d_descs, ks);
+    }
+  }
   hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
   hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);

@@ -35,7 +35,9 @@ def test_workspace_layout_matches_python_mirror():
     from pulsesuite_amd.SBEs import ws_layout
 
     for nk, nr, nrr in ((16, 34, 48), (128, 258, 1024), (1024, 2050, 128)):
-        assert _lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr) == 8 * ws_layout(nk, nr, nrr)["total"]
+        lay = ws_layout(nk, nr, nrr)
+        assert _lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr, _lib.FP64) == 8 * lay["total_fp64"]
+        assert _lib.lib.pss_sbe_workspace_bytes(nk, nr, nrr, _lib.MIXED) == 8 * lay["total"]
 
 
 @pytest.mark.parametrize("struct", ["pss_wire_desc", "pss_grid"])

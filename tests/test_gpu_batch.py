@@ -95,7 +95,8 @@ def test_plans_and_graph_replays_identical():
                     assert ok, (plan, n, i, k)
 
 
-def test_autotune_restores_state_and_picks_a_plan():
+@pytest.mark.parametrize("graph", [False, True])
+def test_autotune_restores_state_and_picks_a_plan(graph):
     import torch
 
     from pulsesuite_amd.batch import WireBatch
@@ -105,7 +106,7 @@ def test_autotune_restores_state_and_picks_a_plan():
     torch.cuda.synchronize()
     before = [b.get_state(i) for i in range(b.N)]
     ctl = b.dev.ctl.clone()
-    best, times = b.autotune(reps=2)
+    best, times = b.autotune(reps=2, graph=graph)
     assert best in times and len(times) >= 3
     assert torch.equal(ctl, b.dev.ctl)
     for i in range(b.N):
