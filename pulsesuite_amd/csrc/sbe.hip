@@ -469,6 +469,8 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
   ws[(holes ? L.Dh : L.De) + idx] = acc * kPi / kHbar;
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 // ------------------------------------------------- K5: Hamiltonians (CalcH)
 // PSTD3D/SBEs.py:820-935 and _calc_Meh :5165-5175.  Exchange terms are Toeplitz
 // products along diagonals; direct terms use the diagonal traces of K3.
@@ -501,21 +503,20 @@ __global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict_
   }
   if (d.flags & PSS_EXCITONS) {
     const bool lf = d.flags & PSS_LF;
-    const int qmin = max(-k1, -k2), qmax = min(Nk - 1 - k1, Nk - 1 - k2);
-    cplx a = mk(0, 0), b = mk(0, 0), c = mk(0, 0);
-    for (int q = qmin; q <= qmax; ++q) {
-      if (q == 0) continue;
-      const int aq = q < 0 ? -q : q;
-      const int64_t t = (int64_t)(k2 + q) * Nk + (k1 + q);  // X^T[k1+q, k2+q]
-      a = a + vc[aq] * B.p2[t];
-      if (lf || k1 == k2) {
+    // the exchange sums sum_{q != 0} V(|q|) X^T[k1+q, k2+q] of Heh (and of Hee, Hhh with
+    // LF) are added by k_exch; without LF Hee / Hhh keep only their diagonal, summed here
+    if (!lf && k1 == k2) {
+      cplx b = mk(0, 0), c = mk(0, 0);
+      for (int q = -k1; q <= Nk - 1 - k1; ++q) {
+        if (q == 0) continue;
+        const int aq = q < 0 ? -q : q;
+        const int64_t t = (int64_t)(k2 + q) * Nk + (k1 + q);
         b = b + vc[Nk + aq] * B.C2[t];
         c = c + vc[2 * Nk + aq] * B.D2[t];
       }
+      hee = hee - b;
+      hhh = hhh - c;
     }
-    heh = heh + a;
-    hee = hee - b;
-    hhh = hhh - c;
     if (lf && k1 != k2) {
       const int qd = k1 - k2;
       const int aq = qd < 0 ? -qd : qd;
@@ -533,6 +534,100 @@ __global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict_
   H[o] = heh;
   H[NN + o] = hee;
   H[2 * NN + o] = hhh;
+}
+
+// ------------------------------- K5b: exchange sums as Toeplitz MFMA products
+// For a matrix X (p, C or D) and potential row V, the exchange term of CalcH
+// (PSTD3D/SBEs.py:820-907) is  E[k1][k2] = sum_{q != 0} V(|q|) X[k2+q][k1+q].
+// Along a diagonal d = k1 - k2 it is a Toeplitz product: with x_d[u] = X[u-d][u]
+// (zero where u - d leaves [0, Nk)),  E[k1][k1-d] = sum_u T[k1][u] x_d[u],
+// T[k1][u] = V(|u - k1|) (u != k1).  Every diagonal shares T, so the diagonals
+// are the columns of one matrix and E is the real x complex product
+//   Y[k1][dd] = sum_u T[k1][u] X~[u][dd],  X~[u][dd] = X[u-d][u], d = dd - (Nk-1),
+// on v_mfma_f64_16x16x4f64.  A workgroup owns 64 k1 x 64 dd; T fragments come
+// from the potential row in LDS, X~ is gathered row by row of X (contiguous u)
+// into LDS per 16-wide u chunk, and chunks where the parallelogram is empty
+// are skipped.  m = 0: Heh += Y (from p, V_eh); with LF m = 1: Hee -= Y (C,
+// V_ee), m = 2: Hhh -= Y (D, V_hh).  grid (ceil((2Nk-1)/64), ceil(Nk/64), 3 nwires).
+__global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!wire_on(d) || !(d.flags & PSS_EXCITONS)) return;
+  if (m > 0 && !(d.flags & PSS_LF)) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1;
+  const int dd0 = blockIdx.x * 64, k10 = blockIdx.y * 64;
+  if (dd0 >= nd || k10 >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : B.D2;
+  extern __shared__ double vrow[];  // Nk doubles: V(|q|) of this matrix
+  __shared__ double Br[16][65], Bi[16][65];
+  for (int i = threadIdx.x; i < Nk; i += 256) vrow[i] = ws[L.vc + (int64_t)m * Nk + i];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+  v4d accr[2][2], acci[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) { accr[a][b] = v4d{0, 0, 0, 0}; acci[a][b] = v4d{0, 0, 0, 0}; }
+  // u range with any nonzero X~[u][dd], dd in [dd0, dd0 + 64): u - d in [0, Nk)
+  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + 63, nd - 1) - (Nk - 1);
+  const int ulo = max(0, dlo), uhi = min(Nk - 1, Nk - 1 + dhi);
+  // ... and with T nonzero: u within [0, Nk) (all k1 rows of the tile are in range or masked)
+  for (int u0 = (ulo / 16) * 16; u0 <= uhi; u0 += 16) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      Br[e >> 6][e & 63] = 0.0;
+      Bi[e >> 6][e & 63] = 0.0;
+    }
+    __syncthreads();
+    // rows r = u - d of X touched by this chunk: r in [u0 - dhi, u0 + 15 - dlo]
+    const int r_lo = max(0, u0 - dhi), r_hi = min(Nk - 1, u0 + 15 - dlo);
+    const int nslots = (r_hi - r_lo + 1) * 16;
+    for (int sidx = threadIdx.x; sidx < nslots; sidx += 256) {
+      const int r = r_lo + (sidx >> 4), u = u0 + (sidx & 15);
+      if (u >= Nk) continue;
+      const int ddl = u - r + (Nk - 1) - dd0;
+      if (ddl < 0 || ddl >= 64 || dd0 + ddl >= nd) continue;
+      const cplx v = X[(int64_t)r * Nk + u];
+      Br[u - u0][ddl] = v.x;
+      Bi[u - u0][ddl] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int uu = u0 + ks * 4 + (lane >> 4);  // A: lane -> T[k1 = row l%16][u = l/16]
+      double ta[2];
+      for (int a = 0; a < 2; ++a) {
+        const int k1 = k10 + wr * 32 + a * 16 + (lane & 15);
+        const int q = uu - k1;
+        ta[a] = (q == 0 || uu >= Nk || k1 >= Nk) ? 0.0 : vrow[q < 0 ? -q : q];
+      }
+      double br[2], bi[2];
+      for (int b = 0; b < 2; ++b) {  // B: lane -> X~[u = l/16][dd = l%16]
+        br[b] = Br[ks * 4 + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
+        bi[b] = Bi[ks * 4 + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
+      }
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], br[b], accr[a][b], 0, 0, 0);
+          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], bi[b], acci[a][b], 0, 0, 0);
+        }
+    }
+  }
+  // D element v of lane l: row (l >> 4) + 4 v, column l & 15
+  cplx* H = (cplx*)(ws + L.H) + (int64_t)m * Nk * Nk;
+  const double sg = m == 0 ? 1.0 : -1.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 4; ++v) {
+        const int k1 = k10 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
+        const int dd = dd0 + wc * 32 + b * 16 + (lane & 15);
+        if (k1 >= Nk || dd >= nd) continue;
+        const int k2 = k1 - (dd - (Nk - 1));
+        if (k2 < 0 || k2 >= Nk) continue;
+        cplx& h = H[(int64_t)k1 * Nk + k2];
+        h = h + mk(sg * accr[a][b][v], sg * acci[a][b][v]);
+      }
 }
 
 // ------------------------------------------- K6: RHS GEMMs + leapfrog epilogue
@@ -561,7 +656,6 @@ __constant__ Term kTerms[3][6] = {
      {S_HEH, OP_H, S_P, OP_T, 1}, {S_NONE, 0, S_NONE, 0, 0}, {S_NONE, 0, S_NONE, 0, 0}},
 };
 
-typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ inline const cplx* src_ptr(int s, const Bufs& B, const cplx* H, int64_t NN) {
   switch (s) {
@@ -1622,6 +1716,10 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_offdiag, dim3((unsigned)((nq + 255) / 256), 2, nwires), dim3(256), lds_off, s, d_descs);
   const int t16 = (max_Nk + 15) / 16;
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
+  {
+    const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
+    hipLaunchKernelGGL(k_exch, dim3(tn, tm, 3 * nwires), dim3(256), (size_t)max_Nk * sizeof(double), s, d_descs);
+  }
   const int t32 = (max_Nk + 31) / 32;
   hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
