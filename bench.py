@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--batched-wires", type=int, default=64, help="wires of the batched run (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--batched-nk-max", type=int, default=1024)
+    ap.add_argument("--batched-eager", action="store_true", help="batched run without autotune or graph capture")
+    ap.add_argument("--workloads", default="coupled,batched,serving",
+                    help="comma list of coupled, batched, serving (the JSON line needs coupled)")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (1/s)")
     return ap.parse_args()
@@ -115,7 +118,7 @@ def fft_pass_timing(prop, reps=50):
     return ms, bytes_
 
 
-def batched(nw, steps, nk_max=1024, seed=0):
+def batched(nw, steps, nk_max=1024, seed=0, eager=False):
     """Batched wires of uneven Nk (uniform over even values in [64, nk_max]), pump
     log-uniform over 3 decades, detuning +-50 meV; one captured launch per step with
     the autotuned plan; FP64 and mixed on the same batch, errors of mixed vs FP64."""
@@ -125,22 +128,29 @@ def batched(nw, steps, nk_max=1024, seed=0):
     from pulsesuite_amd.wire_model import MBFlags
 
     specs = sample_specs(nw, seed=seed, nk_lo=64, nk_hi=nk_max, e_lo=1e6, e_hi=1e9)
+    for sp in specs:  # start 2 fs before the pulse peak so the timed steps are driven
+        sp.t0 = sp.tp - 2e-15
     out = {"workload": f"{nw} wires, Nk uniform even [64,{nk_max}], E0 log-uniform [1e6,1e9] V/m, detuning "
-                       f"+-50 meV, screened, dt 10 as, seed {seed}", "wires": nw, "steps": steps,
+                       f"+-50 meV, screened, dt 10 as, from 2 fs before the pulse peak, seed {seed}",
+           "wires": nw, "steps": steps,
            "kpoints": int(sum(s.Nk for s in specs))}
     states = {}
     for prec in ("fp64", "mixed"):
         b = WireBatch(specs, precision=prec, dt=10e-18, mb_flags=MBFlags(Screened=True), nrr=16)
-        print(f"[bench] batched {prec}: autotune", file=sys.stderr, flush=True)
-        best, times = b.autotune(reps=1)
-        b.run(1)  # capture with the tuned plan (one eager step)
+        if eager:
+            b.run(1, graph=False)
+        else:
+            print(f"[bench] batched {prec}: autotune", file=sys.stderr, flush=True)
+            b.autotune(reps=1)
+            b.run(1)  # capture with the tuned plan (one eager step)
+        best = b.plan
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        b.run(steps)
+        b.run(steps, graph=not eager)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         out[prec] = {"value": b.kpoint_steps(steps) / el, "ms_per_step": 1e3 * el / steps,
-                     "plan": f"buckets={best.buckets} streams={best.streams}"}
+                     "plan": f"buckets={best.buckets} streams={best.streams}" + (" eager" if eager else " graph")}
         states[prec] = [b.get_state(i) for i in range(0, nw, max(1, nw // 8))]
         del b
         torch.cuda.empty_cache()
@@ -229,6 +239,15 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    wl = set(a.workloads.split(","))
+    if "coupled" not in wl:  # partial runs (profiling): no headline value
+        line = {"metric": METRIC, "value": None, "note": "coupled workload skipped"}
+        if "batched" in wl and a.batched_wires > 0:
+            line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max, eager=a.batched_eager)
+        if "serving" in wl and a.serve_requests > 0:
+            line["serving"] = serving(a.serve_requests, a.serve_rate)
+        print(json.dumps(line))
+        return
     results = {}
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     for prec in precs:
@@ -272,9 +291,9 @@ def main():
         line["roofline"] = {"bound": "hbm", "kernel": "k_fft2<8,16> x3 (one 3-D FFT)", "achieved": ach / 1e9,
                             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
                             "avg_ms": ms}
-    if a.batched_wires > 0:
-        line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max)
-    if a.serve_requests > 0:
+    if "batched" in wl and a.batched_wires > 0:
+        line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max, eager=a.batched_eager)
+    if "serving" in wl and a.serve_requests > 0:
         print("[bench] serving", file=sys.stderr, flush=True)
         line["serving"] = serving(a.serve_requests, a.serve_rate)
     if a.cpu_steps > 0:

@@ -34,6 +34,10 @@ def _dev(device):
     return torch.device(device)
 
 
+# read-only device tables per (model, name, device), shared by every DeviceWires
+_DEVICE_TABLES = {}
+
+
 class DeviceWires:
     """Device-resident state, tables, workspace and descriptors of Nw wires sharing one model.
 
@@ -90,9 +94,19 @@ class DeviceWires:
         self.max_nrr = max(nrr)
 
     # ------------------------------------------------------------ set-up
+    def _cached(self, m, name):
+        e = _DEVICE_TABLES.get((id(m), name, str(self.device)))
+        return e[1] if e is not None and e[0] is m else None
+
+    def _store(self, m, name, t):
+        if len(_DEVICE_TABLES) > 65536:
+            _DEVICE_TABLES.clear()
+        _DEVICE_TABLES[(id(m), name, str(self.device))] = (m, t)
+        return t
+
     def _table(self, m, name):
-        key = (id(m), name)
-        if key not in self._tables:
+        t = self._cached(m, name)
+        if t is None:
             a = m.arrays[name]
             if np.iscomplexobj(a):
                 t = torch.as_tensor(np.ascontiguousarray(a), dtype=_CDT).view(torch.float64)
@@ -100,12 +114,13 @@ class DeviceWires:
                 t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32))
             else:
                 t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64))
-            self._tables[key] = t.to(self.device)
-        return self._tables[key]
+            t = self._store(m, name, t.to(self.device))
+        return t
 
     def _v0(self, m):
-        key = (id(m), "V0")
-        if key not in self._tables:
+        """V(q) rows (eh, ee, hh) of the model on the device, computed once per model."""
+        V0 = self._cached(m, "V0")
+        if V0 is None:
             s = m.scalars
             y = self._table(m, "r")
             q = torch.as_tensor(m.arrays["Vq"], dtype=torch.float64, device=self.device)
@@ -114,8 +129,8 @@ class DeviceWires:
             for n, (a1, a2) in enumerate(((s["ae"], s["ah"]), (s["ae"], s["ae"]), (s["ah"], s["ah"]))):
                 _lib.check(_lib.lib.pss_vint(_lib.ptr(y), m.Nr, _lib.ptr(q), m.Nk, a1, a2, s["Delta0"], s["vpref"],
                                              _lib.ptr(V0[n]), st), "pss_vint")
-            self._tables[key] = V0
-        return self._tables[key]
+            self._store(m, "V0", V0)
+        return V0
 
     def _setup_wire(self, i, m, w):
         dev = self.device
@@ -234,11 +249,11 @@ def ws_layout(Nk, Nr, nrr):
     for n, sz in segs:
         out[n] = o
         o += al(sz)
-    # mixed precision only: FP16 operand planes, split-K partial tiles
+    out["pk"] = o  # split-K partial tiles
+    o += al(6 * ksm * Nk * Nk if ksm > 1 else 0)
     out["total_fp64"] = o
-    for n, sz in (("sp", 16 * np64 * np64), ("pk", 6 * ksm * Nk * Nk if ksm > 1 else 0)):
-        out[n] = o
-        o += al(sz)
+    out["sp"] = o  # mixed precision only: FP16 operand planes
+    o += al(16 * np64 * np64)
     out["total"] = o
     return out
 

@@ -63,12 +63,11 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.spb = o;  o += al(5 * (int64_t)Nr);
   w.outr = o; o += al(5 * (int64_t)nrr);
   w.cp = o;   o += al(2 * (int64_t)Nk * Nk);
-  // mixed precision only: FP16 operand planes and split-K partial tiles
-  w.total_fp64 = o;
   const int64_t np = (Nk + 63) / 64 * 64;
   const int ksm = np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
-  w.sp = o;   o += al(16 * np * np);
-  w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);
+  w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);  // split-K partial tiles
+  w.total_fp64 = o;
+  w.sp = o;   o += al(16 * np * np);  // mixed precision only: FP16 operand planes
   w.total = o;
   return w;
 }
@@ -390,83 +389,128 @@ __global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __res
 }
 
 // ------------------------------- K4: off-diagonal dephasing tables De, Dh
-// PSTD3D/dephasing.py:227-317 (CalcOffDiagDeph_E2/H2) with n = |n2|.
-// Thread per (q, k) output; blockIdx.y: 0 = electrons, 1 = holes.
+// PSTD3D/dephasing.py:227-317 (CalcOffDiagDeph_E2/H2) with n = |n2|:
+//   D[q, k] = pi/hbar sum_p V^2 L(Ed) (n_pm (1-n_p)(1-n_k) + (1-n_pm) n_p n_k),
+//   Ed = E[k+q] - E[k] + E[p-q] - E[p]   (same species, p - q in [1, Nk-1]),
+//   Ed = E[k+q] - E[k] + E'[p+q] - E'[p] (other species E', n'),
+// L the Lorentzian of width b.  Per q the p-dependent parts G_p = E[p-q] - E[p],
+// a_p = n_pm (1-n_p), b_p = (1-n_pm) n_p are shared by every k, so a workgroup
+// owns one q and 256 k, stages (G, a, b) of a p chunk in LDS (broadcast reads)
+// and accumulates Sa = sum L(F + G_p) a_p, Sb = sum L(F + G_p) b_p with F =
+// E[k+q] - E[k]; D = pi/hbar V^2 ((1-n_k) Sa + n_k Sb).
+// grid (ceil(Nk/256), 2Nk+1 (q + Nk), 2 nwires (species));  electrons -> De.
+__device__ inline void offdiag_sum64(const double* __restrict__ G, const double* __restrict__ A,
+                                     const double* __restrict__ Bv, int n, double F, double pre, double b2,
+                                     double& Sa, double& Sb) {
+  for (int j = 0; j < n; ++j) {
+    const double Ed = F + G[j];
+    const double l = ddiv(pre, Ed * Ed + b2);
+    Sa = fma(l, A[j], Sa);
+    Sb = fma(l, Bv[j], Sb);
+  }
+}
+
+// mixed precision: P[j] = (G_j / b, a_j, b_j), z = F / b + G_j / b and 1 / (1 + z^2)
+// in FP32, partial sums of 64 terms in FP32 folded into FP64; the caller applies
+// the factor 1 / (pi b) of (b/pi)/(E^2+b^2) = (1/(pi b)) / (1 + (E/b)^2).
+__device__ inline void offdiag_sum32(const float4* __restrict__ P, int n, float Fs, double& Sa, double& Sb) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j1 = min(n, j0 + 64);
+    float sa = 0.f, sb = 0.f;
+    for (int j = j0; j < j1; ++j) {
+      const float4 v = P[j];
+      const float z = Fs + v.x;
+      const float l = __frcp_rn(fmaf(z, z, 1.0f));
+      sa = fmaf(l, v.y, sa);
+      sb = fmaf(l, v.z, sb);
+    }
+    Sa += (double)sa;
+    Sb += (double)sb;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.z];
+  const pss_wire_desc& d = descs[blockIdx.z >> 1];
   if (!wire_on(d) || !(d.flags & PSS_OFFDIAGDPH)) return;
   const int Nk = d.Nk;
-  const int64_t idx0 = (int64_t)blockIdx.x * 256;
-  if (idx0 >= (int64_t)(2 * Nk + 1) * Nk) return;
+  const int q = (int)blockIdx.y - Nk;
+  if (q > Nk) return;
+  if (blockIdx.x * 256 >= Nk) return;
+  const bool holes = blockIdx.z & 1;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
-  const double* vc = ws + L.vc;
-  const bool holes = blockIdx.y == 1;
-  extern __shared__ double dyn[];  // 4 * Nk doubles
-  double* sA = dyn;
-  double* sB = dyn + Nk;
-  double* sEA = dyn + 2 * Nk;
-  double* sEB = dyn + 3 * Nk;
-  {
-    const double* nA = ws + (holes ? L.nh2 : L.ne2);
-    const double* nB = ws + (holes ? L.ne2 : L.nh2);
-    const double* EA = holes ? d.Eh : d.Ee;
-    const double* EB = holes ? d.Ee : d.Eh;
-    for (int k = threadIdx.x; k < Nk; k += 256) {
-      sA[k] = cabs(mk(nA[2 * k], nA[2 * k + 1]));
-      sB[k] = cabs(mk(nB[2 * k], nB[2 * k + 1]));
-      sEA[k] = EA[k];
-      sEB[k] = EB[k];
-    }
+  // outputs with k + q outside [0, Nk) are zero; a block with none inside only writes zeros
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int klo = max(0, -q), khi = min(Nk, Nk - q);  // valid k in [klo, khi)
+  const int kb = blockIdx.x * 256;
+  if (kb + 256 <= klo || kb >= khi) {
+    if (k < Nk) ws[(holes ? L.Dh : L.De) + (int64_t)(q + Nk) * Nk + k] = 0.0;
+    return;
   }
-  __syncthreads();
-  const int64_t idx = idx0 + threadIdx.x;
-  if (idx >= (int64_t)(2 * Nk + 1) * Nk) return;
-  const int qi = (int)(idx / Nk), k = (int)(idx % Nk);
-  const int q = qi - Nk;
+  const double* vc = ws + L.vc;
+  const double* nA = ws + (holes ? L.nh2 : L.ne2);
+  const double* nB = ws + (holes ? L.ne2 : L.nh2);
+  const double* EA = holes ? d.Eh : d.Ee;
+  const double* EB = holes ? d.Ee : d.Eh;
+  constexpr int CH = 512;
+  __shared__ double sG[CH], sa[CH], sb[CH];
+  __shared__ float4 sP[CH];
   // g = [gam_eh, gam_e, gam_h]; E2 gets (g0, g2), H2 gets (g1, g2)  (dephasing.py:735-736)
   const double bAA = kHbar * (holes ? d.gam_e : d.gam_eh);
   const double bAB = kHbar * d.gam_h;
-  const double pAA = bAA / kPi, pAB = bAB / kPi, bAA2 = bAA * bAA, bAB2 = bAB * bAB;
-  // mixed precision: (b/pi)/(E^2+b^2) = (1/(pi b)) / ((E/b)^2 + 1) with the reciprocal in FP32
   const bool fast = d.precision == PSS_MIXED;
-  const double ipAA = 1.0 / (kPi * bAA), ipAB = 1.0 / (kPi * bAB), ibAA = 1.0 / bAA, ibAB = 1.0 / bAB;
   const int aq = q < 0 ? -q : q;
   double vAA = 0.0, vAB = 0.0;
   if (aq >= 1 && aq <= Nk - 1) {
-    const double a = vc[(holes ? 2 : 1) * Nk + aq], b = vc[0 * Nk + aq];
-    vAA = a * a;
-    vAB = b * b;
+    const double x = vc[(holes ? 2 : 1) * Nk + aq], y = vc[aq];
+    vAA = x * x;
+    vAB = y * y;
   }
-  double acc = 0.0;
   const int kq = k + q;
-  if (kq >= 0 && kq < Nk) {
-    const double nk = sA[k];
-    const double EAk = sEA[k], EAkq = sEA[kq];
-    if (vAA != 0.0) {
-      const int p0 = max(q + 1, 0), p1 = min(q + Nk - 1, Nk - 1);
-      for (int p = p0; p <= p1; ++p) {
-        const int pm = p - q;
-        const double npm = sA[pm], np_ = sA[p];
-        const double Ed = EAkq + sEA[pm] - sEA[p] - EAk;
-        const double lr = fast ? ipAA * (double)__frcp_rn((float)((Ed * ibAA) * (Ed * ibAA) + 1.0))
-                               : ddiv(pAA, Ed * Ed + bAA2);
-        acc += vAA * lr * (npm * (1.0 - np_) * (1.0 - nk) + (1.0 - npm) * np_ * nk);
+  const bool valid = k >= klo && k < khi;
+  const double F = valid ? EA[kq] - EA[k] : 0.0;
+  double acc = 0.0;
+  for (int term = 0; term < 2; ++term) {
+    const double v2 = term == 0 ? vAA : vAB;
+    if (v2 == 0.0) continue;  // uniform over the block
+    const double bw = term == 0 ? bAA : bAB;
+    const double pre = bw / kPi, b2 = bw * bw, ib = 1.0 / bw;
+    const double* E = term == 0 ? EA : EB;
+    const double* nn = term == 0 ? nA : nB;
+    // same species: partner p - q in [1, Nk-1]; other species: p + q in [0, Nk-1]
+    const int s = term == 0 ? -q : q;
+    const int p0 = term == 0 ? max(q + 1, 0) : max(-q, 0);
+    const int p1 = term == 0 ? min(q + Nk - 1, Nk - 1) : min(Nk - 1 - q, Nk - 1);
+    double Sa = 0.0, Sb = 0.0;
+    for (int c0 = p0; c0 <= p1; c0 += CH) {
+      const int n = min(CH, p1 - c0 + 1);
+      __syncthreads();
+      for (int j = threadIdx.x; j < n; j += 256) {
+        const int p = c0 + j, pr = p + s;
+        const double np_ = cabs(mk(nn[2 * p], nn[2 * p + 1]));
+        const double npr = cabs(mk(nn[2 * pr], nn[2 * pr + 1]));
+        const double G = E[pr] - E[p], av = npr * (1.0 - np_), bv = (1.0 - npr) * np_;
+        if (fast) {
+          sP[j] = make_float4((float)(G * ib), (float)av, (float)bv, 0.f);
+        } else {
+          sG[j] = G;
+          sa[j] = av;
+          sb[j] = bv;
+        }
+      }
+      __syncthreads();
+      if (valid) {
+        if (fast) offdiag_sum32(sP, n, (float)(F * ib), Sa, Sb);
+        else offdiag_sum64(sG, sa, sb, n, F, pre, b2, Sa, Sb);
       }
     }
-    if (vAB != 0.0) {
-      const int p0 = max(-q, 0), p1 = min(Nk - 1 - q, Nk - 1);
-      for (int p = p0; p <= p1; ++p) {
-        const int pp = p + q;
-        const double npp = sB[pp], np_ = sB[p];
-        const double Ed = EAkq + sEB[pp] - sEB[p] - EAk;
-        const double lr = fast ? ipAB * (double)__frcp_rn((float)((Ed * ibAB) * (Ed * ibAB) + 1.0))
-                               : ddiv(pAB, Ed * Ed + bAB2);
-        acc += vAB * lr * (npp * (1.0 - np_) * (1.0 - nk) + (1.0 - npp) * np_ * nk);
-      }
+    if (valid) {
+      const double nk = cabs(mk(nA[2 * k], nA[2 * k + 1]));
+      const double f = fast ? 1.0 / (kPi * bw) : 1.0;
+      acc += v2 * f * ((1.0 - nk) * Sa + nk * Sb);
     }
   }
-  ws[(holes ? L.Dh : L.De) + idx] = acc * kPi / kHbar;
+  if (k < Nk) ws[(holes ? L.Dh : L.De) + (int64_t)(q + Nk) * Nk + k] = acc * kPi / kHbar;
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -689,15 +733,41 @@ __device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, cons
   }
 }
 
-// 4 wavefronts per 32x32 output tile; the (term, 16-wide k chunk) work items are
-// dealt round-robin to the waves, each with its own LDS staging, and the four
-// partial tiles are summed in a fixed order (deterministic) in the epilogue.
-__global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs) {
-  const int wire = blockIdx.z / 3, out = blockIdx.z % 3;
+// FP64 leapfrog epilogue of one element with the products' sum S (the Hee / Hhh
+// diagonals are inside the products): + Heh^T - i hbar gamma p for dp.
+__device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, const cplx* H, const double* gam,
+                                     int out, int i, int j, cplx S) {
+  const int Nk = d.Nk;
+  const int64_t o = (int64_t)i * Nk + j;
+  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
+  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
+  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
+  if (out == 0) {
+    // + Heh^T[kh=i, ke=j] - i hbar (GamE[ke] + GamH[kh]) p[kh, ke]
+    const cplx hT = H[(int64_t)j * Nk + i];
+    const cplx pv = B.p2[o];
+    const double g = kHbar * (gam[j] + gam[Nk + i]);
+    S = S + hT - mk(-g * pv.y, g * pv.x);
+  }
+  const cplx R = div_ih(S, kHbar);
+  const cplx x1 = X1[o];
+  const cplx x3 = x1 + (R * d.dt) * 2.0;
+  Xn[o] = x3;
+  if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+}
+
+// One 64 x 64 output tile per workgroup, 4 waves (32 x 32 quadrants, 2 x 2 blocks of
+// 16 x 16, v_mfma_f64_16x16x4f64); A (64 x 16) and B (16 x 64) complex operands of one
+// k chunk staged in LDS, shared by the waves; coalesced loads along the contiguous
+// index of op(X).  The (term, chunk) list is cut into `ks` slices (blockIdx.z); with
+// ks == 1 the epilogue runs here, else FP64 partial tiles go to the pk buffer.
+__global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs, int ks) {
+  const int zz = blockIdx.z;
+  const int slice = zz % ks, out = (zz / ks) % 3, wire = zz / (3 * ks);
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || d.precision == PSS_MIXED) return;
   const int Nk = d.Nk;
-  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   if (m0 >= Nk || n0 >= Nk) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
@@ -707,12 +777,9 @@ __global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restric
   const double* Dh = ws + L.Dh;
   const double* De = ws + L.De;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
 
-  __shared__ double stage[4][4][16][32];  // [wave][Ar, Ai, Br, Bi][k][m or n]
-  double (*Ar)[32] = stage[wv][0];
-  double (*Ai)[32] = stage[wv][1];
-  double (*Br)[32] = stage[wv][2];
-  double (*Bi)[32] = stage[wv][3];
+  __shared__ double Ar[16][64], Ai[16][64], Br[16][64], Bi[16][64];
   v4d accr[2][2], acci[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) {
@@ -722,99 +789,65 @@ __global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restric
   const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
   const int nkc = (Nk + 15) / 16;
   const int nitems = nterms * nkc;
-  const int niter = (nitems + 3) / 4;
-  for (int it = 0; it < niter; ++it) {
-    const int item = it * 4 + wv;
-    const bool has = item < nitems;
-    if (has) {
-      const int t = item / nkc, k0 = (item % nkc) * 16;
-      const Term T = kTerms[out][t];
-      const cplx* Ap = src_ptr(T.a, B, H, NN);
-      const cplx* Bp = src_ptr(T.b, B, H, NN);
-      const double sg = T.neg ? -1.0 : 1.0;
-      const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
-      const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
-      // A tile: 32 (m) x 16 (k), coalesced along the contiguous global index
-      for (int e = lane; e < 512; e += 64) {
-        int mm, kk;
-        if (!aT) { mm = e >> 4; kk = e & 15; } else { kk = e >> 5; mm = e & 31; }
-        const cplx v = load_op(Ap, T.aop, m0 + mm, k0 + kk, Nk, Dh, De);
-        Ar[kk][mm] = sg * v.x;
-        Ai[kk][mm] = sg * v.y;
-      }
-      for (int e = lane; e < 512; e += 64) {
-        int kk, nn;
-        if (!bT) { kk = e >> 5; nn = e & 31; } else { nn = e >> 4; kk = e & 15; }
-        // op(B)[k, n]: for transposed ops load_op's (i, k) = (k, n) of op(B)
-        const cplx v = load_op(Bp, T.bop, k0 + kk, n0 + nn, Nk, Dh, De);
-        Br[kk][nn] = v.x;
-        Bi[kk][nn] = v.y;
-      }
+  const int it0 = (int)((int64_t)nitems * slice / ks), it1 = (int)((int64_t)nitems * (slice + 1) / ks);
+  for (int it = it0; it < it1; ++it) {
+    const int t = it / nkc, k0 = (it % nkc) * 16;
+    const Term T = kTerms[out][t];
+    const cplx* Ap = src_ptr(T.a, B, H, NN);
+    const cplx* Bp = src_ptr(T.b, B, H, NN);
+    const double sg = T.neg ? -1.0 : 1.0;
+    const bool aT = (T.aop == OP_T || T.aop == OP_H || T.aop == OP_MET);
+    const bool bT = (T.bop == OP_T || T.bop == OP_H || T.bop == OP_MET);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      int mm, kk;
+      if (!aT) { mm = e >> 4; kk = e & 15; } else { kk = e >> 6; mm = e & 63; }
+      const cplx v = load_op(Ap, T.aop, m0 + mm, k0 + kk, Nk, Dh, De);
+      Ar[kk][mm] = sg * v.x;
+      Ai[kk][mm] = sg * v.y;
+    }
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      int kk, nn;
+      if (!bT) { kk = e >> 6; nn = e & 63; } else { nn = e >> 4; kk = e & 15; }
+      // op(B)[k, n]: for transposed ops load_op's (i, k) = (k, n) of op(B)
+      const cplx v = load_op(Bp, T.bop, k0 + kk, n0 + nn, Nk, Dh, De);
+      Br[kk][nn] = v.x;
+      Bi[kk][nn] = v.y;
     }
     __syncthreads();
-    if (has) {
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        const int li = lane & 15;
-        double ar[2], ai[2], br[2], bi[2];
-        for (int a = 0; a < 2; ++a) {
-          ar[a] = Ar[kr][a * 16 + li];
-          ai[a] = Ai[kr][a * 16 + li];
-          br[a] = Br[kr][a * 16 + li];
-          bi[a] = Bi[kr][a * 16 + li];
+    for (int kq = 0; kq < 4; ++kq) {
+      const int kr = kq * 4 + (lane >> 4);
+      const int li = lane & 15;
+      double ar[2], ai[2], br[2], bi[2];
+      for (int x = 0; x < 2; ++x) {
+        ar[x] = Ar[kr][wr * 32 + x * 16 + li];
+        ai[x] = Ai[kr][wr * 32 + x * 16 + li];
+        br[x] = Br[kr][wc * 32 + x * 16 + li];
+        bi[x] = Bi[kr][wc * 32 + x * 16 + li];
+      }
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], accr[x][y], 0, 0, 0);
+          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], accr[x][y], 0, 0, 0);
+          acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acci[x][y], 0, 0, 0);
+          acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acci[x][y], 0, 0, 0);
         }
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b) {
-            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
-            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], accr[a][b], 0, 0, 0);
-            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acci[a][b], 0, 0, 0);
-            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
-          }
-      }
     }
-    __syncthreads();
   }
-  // partial tiles to LDS: wave wv writes its 32x32 complex tile into stage[wv]
-  double* P = &stage[wv][0][0][0];  // 2048 doubles = 32 x 32 complex
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 4; ++v) {
-        // v_mfma_f64_16x16x4f64 result: lane l, element v holds row (l/16) + 4v, column l%16
-        const int i = a * 16 + (lane >> 4) + 4 * v;
-        const int j = b * 16 + (lane & 15);
-        P[2 * (i * 32 + j)] = accr[a][b][v];
-        P[2 * (i * 32 + j) + 1] = acci[a][b][v];
-      }
-  __syncthreads();
-  // epilogue: 4 elements per thread
-  const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
-  cplx* Xp = out == 0 ? B.pp : out == 1 ? B.Cp : B.Dp;
-  cplx* Xn = out == 0 ? B.pn : out == 1 ? B.Cn : B.Dn;
+  // v_mfma_f64_16x16x4f64 result: lane l, element v holds row (l/16) + 4v, column l%16
   const double* gam = ws + L.gam;
-  for (int q = threadIdx.x; q < 1024; q += 256) {
-    const int li = q >> 5, lj = q & 31;
-    const int i = m0 + li, j = n0 + lj;
-    if (i >= Nk || j >= Nk) continue;
-    cplx S = mk(0, 0);
-    for (int w = 0; w < 4; ++w) {
-      const double* Pw = &stage[w][0][0][0];
-      S = S + mk(Pw[2 * q], Pw[2 * q + 1]);
-    }
-    const int64_t o = (int64_t)i * Nk + j;
-    if (out == 0) {
-      // + Heh^T[kh=i, ke=j] - i hbar (GamE[ke] + GamH[kh]) p[kh, ke]
-      const cplx hT = H[(int64_t)j * Nk + i];
-      const cplx pv = B.p2[o];
-      const double g = kHbar * (gam[j] + gam[Nk + i]);
-      S = S + hT - mk(-g * pv.y, g * pv.x);
-    }
-    const cplx R = div_ih(S, kHbar);
-    const cplx x1 = X1[o];
-    const cplx x3 = x1 + (R * d.dt) * 2.0;
-    Xn[o] = x3;
-    if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
-  }
+  cplx* pk = (cplx*)(ws + L.pk) + ((int64_t)slice * 3 + out) * NN;
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y)
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + wr * 32 + x * 16 + (lane >> 4) + 4 * v;
+        const int j = n0 + wc * 32 + y * 16 + (lane & 15);
+        if (i >= Nk || j >= Nk) continue;
+        const cplx S = mk(accr[x][y][v], acci[x][y][v]);
+        if (ks == 1) fp64_epilogue(d, B, H, gam, out, i, j, S);
+        else pk[(int64_t)i * Nk + j] = S;
+      }
 }
 
 // ------------------------------- K6m: mixed-precision RHS contractions
@@ -1145,7 +1178,7 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
 // sum of the ks partial tiles (fixed order) + epilogue; grid (ceil(NN/256), 3, nwires)
 __global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict__ descs, int ks) {
   const pss_wire_desc& d = descs[blockIdx.z];
-  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  if (!wire_on(d)) return;
   const int Nk = d.Nk, out = blockIdx.y;
   const int64_t NN = (int64_t)Nk * Nk;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1155,7 +1188,10 @@ __global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict
   const cplx* pk = (const cplx*)(ws + L.pk);
   cplx S = mk(0, 0);
   for (int s = 0; s < ks; ++s) S = S + pk[((int64_t)s * 3 + out) * NN + o];
-  mixed_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
+  if (d.precision == PSS_MIXED)
+    mixed_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
+  else
+    fp64_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
 }
 
 // --------------------------- K7: carrier-carrier + phonon collision rates
@@ -1695,16 +1731,13 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
     return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
   hipStream_t s = (hipStream_t)stream;
   const int nd = 2 * max_Nk - 1;
-  const size_t lds_off = 4 * (size_t)max_Nk * sizeof(double);
   const size_t lds_rel = 2 * ((size_t)max_Nk + 1) * sizeof(double2);
   // gfx950 gives a workgroup up to 160 KB of LDS on request; raised once per process
   // for the largest size seen (outside any stream work, so a captured step is unaffected)
   static size_t lds_granted = 65536;
-  const size_t lds_need = std::max(lds_rel, lds_off);
-  if (lds_need > lds_granted) {
+  if (lds_rel > lds_granted) {
     PSS_CHECK(hipFuncSetAttribute((const void*)k_coll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
-    PSS_CHECK(hipFuncSetAttribute((const void*)k_offdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_off));
-    lds_granted = lds_need;
+    lds_granted = lds_rel;
   }
   const int nrb = (max_Nr + 255) / 256;
   hipLaunchKernelGGL(k_prop_solve, dim3(nwires), dim3(256), 0, s, d_descs);
@@ -1712,20 +1745,15 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prep, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
-  const int64_t nq = (int64_t)(2 * max_Nk + 1) * max_Nk;
-  hipLaunchKernelGGL(k_offdiag, dim3((unsigned)((nq + 255) / 256), 2, nwires), dim3(256), lds_off, s, d_descs);
+  hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256), 0, s, d_descs);
   const int t16 = (max_Nk + 15) / 16;
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   {
     const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
     hipLaunchKernelGGL(k_exch, dim3(tn, tm, 3 * nwires), dim3(256), (size_t)max_Nk * sizeof(double), s, d_descs);
   }
-  const int t32 = (max_Nk + 31) / 32;
-  hipLaunchKernelGGL(k_rhs_gemm, dim3(t32, t32, 3 * nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
   {
     const int np = np64(max_Nk), t64 = np / 64;
-    hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
     // split-K: enough workgroups to fill the chip, within every wire's partial buffer
     int ks = g_ksplit;
     if (ks == 0) {
@@ -1734,6 +1762,9 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
       while (ks < ksplit_max(max_Nk) && tiles * ks < 1024) ks *= 2;
     }
     ks = std::min(ks, ksplit_max(max_Nk));
+    hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
     hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
     if (ks > 1) {
       const int64_t nn = (int64_t)max_Nk * max_Nk;

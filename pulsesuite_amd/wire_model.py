@@ -178,8 +178,26 @@ class WireModel:
         return f
 
 
+_MODEL_CACHE = {}
+
+
 def build_model(qw, mb, rr, r0=0.0):
-    """Grids, constants and host tables (PSTD3D/SBEs.py:4756-4984)."""
+    """Grids, constants and host tables (PSTD3D/SBEs.py:4756-4984).
+
+    Models are cached per (material, flags, propagation grid, r0): wires of the same
+    kind (batches, serving admission) share one model and its device tables."""
+    rr = np.asarray(rr, dtype=np.float64)
+    key = (tuple(sorted(vars(qw).items())), tuple(sorted(vars(mb).items())), rr.shape, rr.tobytes(), float(r0))
+    m = _MODEL_CACHE.get(key)
+    if m is None:
+        m = _build_model(qw, mb, rr, r0)
+        if len(_MODEL_CACHE) > 4096:
+            _MODEL_CACHE.clear()
+        _MODEL_CACHE[key] = m
+    return m
+
+
+def _build_model(qw, mb, rr, r0=0.0):
     M = WireModel(qw=qw, mb=MBFlags(**vars(mb)))
     mb = M.mb
     if mb.DCTrans or mb.Recomb or mb.FreePot:
