@@ -1,6 +1,6 @@
 """GPU parity of the SBE wire step against the FP64 oracle.
 
-Bar (north star): relative L2 <= 1e-10 on polarisation, populations and the
+Bar: relative L2 <= 1e-10 on polarisation, populations and the
 propagation-grid outputs, FP64 path.  Inputs are seeded: Hermitian C, D with
 eigenvalues in [0, 0.5], |p| ~ 0.1, a Gaussian-cosine field on rr.
 """
