@@ -4,6 +4,8 @@
 //
 // Fields are complex128, C order (x slowest, z fastest), k-space between
 // half steps exactly as PSTD3D/PSTD3D.py keeps them.
+#include <algorithm>
+
 #include "common.h"
 #include "../../include/pss.h"
 
@@ -73,9 +75,10 @@ struct PassArgs {
   const double* kmul;   // wavenumbers along mul_axis (null = none)
   int Nx, Ny, Nz, axis, mul_axis, dir;  // dir -1 forward, +1 inverse
   double scale;
+  int accum;  // 1: out += result (the last pass of F_k += FFT(dF))
 };
 
-template <int N>
+template <int N, bool ACC>
 __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
   constexpr int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   constexpr int LOG = __builtin_ctz(N);
@@ -160,7 +163,12 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
   for (int e = threadIdx.x; e < TB * N; e += 256) {
     int t, i;
     if (a.axis == 2) { t = e / N; i = e % N; } else { i = e / TB; t = e % TB; }
-    if (t < nvalid) a.out[base0 + (int64_t)t * st + (int64_t)i * sa] = buf[t * N + i] * a.scale;
+    if (t < nvalid) {
+      const int64_t o = base0 + (int64_t)t * st + (int64_t)i * sa;
+      const cplx v = buf[t * N + i] * a.scale;
+      if (ACC) a.out[o] = a.out[o] + v;
+      else a.out[o] = v;
+    }
   }
 }
 
@@ -197,7 +205,7 @@ __device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
 // per element instead of log2(N) staged radix-2 sweeps.
-template <int N1, int N2>
+template <int N1, int N2, bool ACC>
 __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
@@ -294,7 +302,10 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
         const int k = k1 + N1 * k2;
-        a.out[base0 + (int64_t)t * st + (int64_t)k * sa] = u[k2] * a.scale;
+        const int64_t o = base0 + (int64_t)t * st + (int64_t)k * sa;
+        const cplx v = u[k2] * a.scale;
+        if (ACC) a.out[o] = a.out[o] + v;
+        else a.out[o] = v;
       }
     }
   }
@@ -309,11 +320,12 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
-  hipLaunchKernelGGL((k_fft2<N1, N2>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  if (a.accum) hipLaunchKernelGGL((k_fft2<N1, N2, true>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  else hipLaunchKernelGGL((k_fft2<N1, N2, false>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
 }
 
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
-                hipStream_t s) {
+                hipStream_t s, int accum = 0) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
   if (N == 1) {
     if (in != out) return -1;
@@ -329,6 +341,7 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   a.mul_axis = mul_axis;
   a.dir = dir;
   a.scale = dir > 0 ? 1.0 / N : 1.0;
+  a.accum = accum;
   switch (N) {
     case 64: launch_fft2<8, 8>(a, g, s); return 0;
     case 128: launch_fft2<8, 16>(a, g, s); return 0;
@@ -343,7 +356,11 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   else blocks = (int64_t)(axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
   dim3 grid((unsigned)blocks), blk(256);
   switch (N) {
-#define PSS_CASE(n) case n: hipLaunchKernelGGL(k_fft_pass<n>, grid, blk, 0, s, a); break;
+#define PSS_CASE(n)                                                                   \
+  case n:                                                                             \
+    if (accum) hipLaunchKernelGGL((k_fft_pass<n, true>), grid, blk, 0, s, a);        \
+    else hipLaunchKernelGGL((k_fft_pass<n, false>), grid, blk, 0, s, a);             \
+    break;
     PSS_CASE(2) PSS_CASE(4) PSS_CASE(8) PSS_CASE(16) PSS_CASE(32) PSS_CASE(64) PSS_CASE(128) PSS_CASE(256)
     PSS_CASE(512) PSS_CASE(1024) PSS_CASE(2048)
 #undef PSS_CASE
@@ -368,29 +385,51 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
   return 0;
 }
 
+// out += FFT(tmp) along every axis of length > 1; tmp is overwritten (used in place
+// for all passes but the last, which accumulates into out).
+int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s) {
+  int last = -1;
+  for (int axis = 0; axis < 3; ++axis)
+    if ((axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz) > 1) last = axis;
+  if (last < 0) return -1;
+  for (int axis = 0; axis < 3; ++axis) {
+    const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
+    if (N == 1) continue;
+    const bool fin = axis == last;
+    const int rc = launch_pass(g, tmp, fin ? out : tmp, axis, dir, nullptr, -1, s, fin ? 1 : 0);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------ CPML
-// Real-space correction of PSTD3D/cpml.py:150-334 (psi only in PML cells),
-// soft-source add of PSTD3D/tfsf.py:106-136 on component src_comp.
+// Real-space correction of PSTD3D/cpml.py:150-334 (psi only in PML cells) and the
+// soft-source add of PSTD3D/tfsf.py:106-136 on component src_comp, written as a
+// real field dF: the reference adds them to Re F in real space between an IFFT and
+// an FFT of F; by linearity F_k += FFT(dF) is the same update without taking F
+// to real space and back.  d0..d5 hold the real-space derivatives on entry and
+// dF_x, dF_y, dF_z (imaginary part 0) in d0, d1, d2 on exit (each thread reads its
+// cell's six derivatives before writing).
 struct SrcPar {
   double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
 };
 
-__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* Fx, cplx* Fy, cplx* Fz,
-                                              const cplx* __restrict__ d0, const cplx* __restrict__ d1,
-                                              const cplx* __restrict__ d2, const cplx* __restrict__ d3,
-                                              const cplx* __restrict__ d4, const cplx* __restrict__ d5, double* psi,
-                                              int src_comp, SrcPar sp, const double* __restrict__ srcx,
-                                              const double* __restrict__ rx, const double* __restrict__ ry,
-                                              const double* __restrict__ rz) {
+__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* d0, cplx* d1, cplx* d2,
+                                              const cplx* __restrict__ d3, const cplx* __restrict__ d4,
+                                              const cplx* __restrict__ d5, double* psi, int src_comp, SrcPar sp,
+                                              const double* __restrict__ srcx, const double* __restrict__ rx,
+                                              const double* __restrict__ ry, const double* __restrict__ rz) {
   const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (o >= n) return;
-  const int k = (int)(o % g.Nz);
-  const int j = (int)((o / g.Nz) % g.Ny);
-  const int i = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
+  const int k = (int)(o & (g.Nz - 1));
+  const int j = (int)((o >> lz) & (g.Ny - 1));
+  const int i = (int)(o >> (lz + ly));
   const bool inx = i < g.npml_x || i >= g.Nx - g.npml_x;
   const bool iny = j < g.npml_y || j >= g.Ny - g.npml_y;
   const bool inz = k < g.npml_z || k >= g.Nz - g.npml_z;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
   if (inx || iny || inz) {
     // d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
     const double gzy = d0[o].x, gyz = d1[o].x, gxz = d2[o].x, gzx = d3[o].x, gyx = d4[o].x, gxy = d5[o].x;
@@ -410,39 +449,102 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* Fx, c
     const double cx_ = g.iky[j] * gzy + vxy - g.ikz[k] * gyz - vxz;
     const double cy_ = g.ikz[k] * gxz + vyz - g.ikx[i] * gzx - vyx;
     const double cz_ = g.ikx[i] * gyx + vzx - g.iky[j] * gxy - vzy;
-    if (which == 0) {
-      const double sc = g.v2 * g.dt;
-      Fx[o].x += sc * cx_;
-      Fy[o].x += sc * cy_;
-      Fz[o].x += sc * cz_;
-    } else {
-      Fx[o].x -= g.dt * cx_;
-      Fy[o].x -= g.dt * cy_;
-      Fz[o].x -= g.dt * cz_;
-    }
+    const double sc = which == 0 ? g.v2 * g.dt : -g.dt;
+    fx = sc * cx_;
+    fy = sc * cy_;
+    fz = sc * cz_;
   }
   if (src_comp >= 0) {
     const double tau = sp.t - rx[i] / sp.v - sp.Tp;
     const double gyz = exp(-(ry[j] * ry[j] + rz[k] * rz[k]) * sp.w0inv2);
     const double einc = sp.amp * gyz * exp(-(tau * tau) / (sp.tau_G * sp.tau_G)) * cos(sp.omega0 * tau + sp.chirp * tau * tau);
-    cplx* F = src_comp == 0 ? Fx : src_comp == 1 ? Fy : Fz;
-    F[o].x += srcx[i] * einc * sp.dx;
+    const double add = srcx[i] * einc * sp.dx;
+    if (src_comp == 0) fx += add;
+    else if (src_comp == 1) fy += add;
+    else fz += add;
+  }
+  d0[o] = mk(fx, 0.0);
+  d1[o] = mk(fy, 0.0);
+  d2[o] = mk(fz, 0.0);
+}
+
+// Trilinear sample (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085) of the corrected
+// real-space field F + dF at 8 corner cells, with F given in k-space:
+//   F(x_c) = (1/N) sum_k F_k exp(+2 pi i (mx ix/Nx + my iy/Ny + mz iz/Nz)),
+// so sum_c w_c F(x_c) = (1/N) sum_k F_k Phi(k), Phi(k) = sum_c w_c exp(...).
+// Pass 1: per-block partial sums (grid-stride over k); pass 2: one block adds the
+// partials in a fixed order, divides by N and adds sum_c w_c dF(x_c).
+__device__ inline cplx twp(const cplx* tw, int N, int r) {  // exp(+2 pi i r / N), tw = exp(-2 pi i m/N), m < N/2
+  r &= N - 1;
+  return r < N / 2 ? conj(tw[r]) : mk(-tw[r - N / 2].x, tw[r - N / 2].y);
+}
+
+__global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restrict__ Fx, const cplx* __restrict__ Fy,
+                                                 const cplx* __restrict__ Fz, const int32_t* __restrict__ idx,
+                                                 const double* __restrict__ w, double* partial) {
+  __shared__ cplx tx[1024], ty[1024], tz[1024];
+  __shared__ double sh[16];
+  __shared__ int cx[8], cy[8], cz[8];
+  __shared__ double cw[8];
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  for (int m = threadIdx.x; m < Nx / 2; m += 256) tx[m] = mk(g.twx[2 * m], g.twx[2 * m + 1]);
+  for (int m = threadIdx.x; m < Ny / 2; m += 256) ty[m] = mk(g.twy[2 * m], g.twy[2 * m + 1]);
+  for (int m = threadIdx.x; m < Nz / 2; m += 256) tz[m] = mk(g.twz[2 * m], g.twz[2 * m + 1]);
+  if (threadIdx.x < 8) {
+    const int64_t o = idx[threadIdx.x];
+    const bool ok = o >= 0;
+    cx[threadIdx.x] = ok ? (int)(o / ((int64_t)Ny * Nz)) : 0;
+    cy[threadIdx.x] = ok ? (int)((o / Nz) % Ny) : 0;
+    cz[threadIdx.x] = ok ? (int)(o % Nz) : 0;
+    cw[threadIdx.x] = ok ? w[threadIdx.x] : 0.0;
+  }
+  __syncthreads();
+  const int64_t n = (int64_t)Nx * Ny * Nz;
+  cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
+  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);  // axis lengths are powers of two
+  for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
+    const int mz = (int)(o & (Nz - 1));
+    const int my = (int)((o >> lz) & (Ny - 1));
+    const int mx = (int)(o >> (lz + ly));
+    cplx phi = mk(0, 0);
+    for (int c = 0; c < 8; ++c) {
+      if (cw[c] == 0.0) continue;
+      const cplx ex = Nx > 1 ? twp(tx, Nx, mx * cx[c]) : mk(1, 0);
+      const cplx ey = Ny > 1 ? twp(ty, Ny, my * cy[c]) : mk(1, 0);
+      const cplx ez = Nz > 1 ? twp(tz, Nz, mz * cz[c]) : mk(1, 0);
+      phi = phi + cw[c] * (ex * ey * ez);
+    }
+    a0 = a0 + Fx[o] * phi;
+    a1 = a1 + Fy[o] * phi;
+    a2 = a2 + Fz[o] * phi;
+  }
+  const double v[6] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y};
+  for (int q = 0; q < 6; ++q) {
+    const double r = block_sum<256>(v[q], sh);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 6 + q] = r;
   }
 }
 
-// Trilinear sample of three real-space fields at 8 corners (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085).
-__global__ void k_sample(const cplx* Fx, const cplx* Fy, const cplx* Fz, const int32_t* idx, const double* w,
-                         double* out) {
-  if (threadIdx.x != 0) return;
-  cplx s[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
-  for (int c = 0; c < 8; ++c) {
-    const int64_t o = idx[c];
-    if (o < 0) continue;
-    s[0] = s[0] + w[c] * Fx[o];
-    s[1] = s[1] + w[c] * Fy[o];
-    s[2] = s[2] + w[c] * Fz[o];
+__global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
+                                                       const cplx* __restrict__ dFx, const cplx* __restrict__ dFy,
+                                                       const cplx* __restrict__ dFz, const int32_t* __restrict__ idx,
+                                                       const double* __restrict__ w, double* out) {
+  __shared__ double sh[16];
+  const double invn = 1.0 / ((double)g.Nx * g.Ny * g.Nz);
+  for (int q = 0; q < 6; ++q) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[(int64_t)b * 6 + q];
+    acc = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) {
+      double r = acc * invn;
+      if ((q & 1) == 0) {  // dF is real: add the sampled correction to the real parts
+        const cplx* dF = q == 0 ? dFx : q == 2 ? dFy : dFz;
+        for (int c = 0; c < 8; ++c)
+          if (idx[c] >= 0) r += w[c] * dF[idx[c]].x;
+      }
+      out[q] = r;
+    }
   }
-  for (int f = 0; f < 3; ++f) { out[2 * f] = s[f].x; out[2 * f + 1] = s[f].y; }
 }
 
 // Uniform wire line: Exx[f][x] = sample[f] for all x (QuantumWire, PSTD3D/rhoPJ.py:1160-1164).
@@ -537,6 +639,8 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
                              double* sample_out, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  if (g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
+    return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
   cplx* sc = (cplx*)scratch;
   const cplx *gx = (const cplx*)Gx, *gy = (const cplx*)Gy, *gz = (const cplx*)Gz;
   // spectral derivatives  d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
@@ -547,22 +651,24 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     int rc = fft3(g, srcs[q], sc + q * n, +1, ks[ax[q]], ax[q], s);
     if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
   }
-  cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
-  for (int c = 0; c < 3; ++c) {
-    int rc = fft3(g, F[c], F[c], +1, nullptr, -1, s);
-    if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
-  }
   SrcPar sp{};
   if (src_comp >= 0) {
     sp.t = src_par[0]; sp.amp = src_par[1]; sp.omega0 = src_par[2]; sp.tau_G = src_par[3];
     sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
   }
-  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, F[0], F[1], F[2], sc,
-                     sc + n, sc + 2 * n, sc + 3 * n, sc + 4 * n, sc + 5 * n, psi, src_comp, sp, srcx, rx, ry, rz);
-  if (sample_idx)
-    hipLaunchKernelGGL(k_sample, dim3(1), dim3(64), 0, s, F[0], F[1], F[2], sample_idx, sample_w, sample_out);
+  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, sc, sc + n, sc + 2 * n,
+                     sc + 3 * n, sc + 4 * n, sc + 5 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
+  if (sample_idx) {
+    // partial sums live in the (now free) fourth scratch field
+    const int nblk = (int)std::min<int64_t>(4096, (n + 255) / 256);
+    double* part = (double*)(sc + 3 * n);
+    hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), 0, s, *g, F[0], F[1], F[2], sample_idx, sample_w, part);
+    hipLaunchKernelGGL(k_ksample_final, dim3(1), dim3(256), 0, s, *g, part, nblk, sc, sc + n, sc + 2 * n, sample_idx,
+                       sample_w, sample_out);
+  }
   for (int c = 0; c < 3; ++c) {
-    int rc = fft3(g, F[c], F[c], -1, nullptr, -1, s);
+    int rc = fft3_acc(g, sc + c * n, F[c], -1, s);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
   PSS_CHECK(hipGetLastError());

@@ -592,36 +592,38 @@ __global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict_
 // from the potential row in LDS, X~ is gathered row by row of X (contiguous u)
 // into LDS per 16-wide u chunk, and chunks where the parallelogram is empty
 // are skipped.  m = 0: Heh += Y (from p, V_eh); with LF m = 1: Hee -= Y (C,
-// V_ee), m = 2: Hhh -= Y (D, V_hh).  grid (ceil((2Nk-1)/64), ceil(Nk/64), 3 nwires).
+// V_ee), m = 2: Hhh -= Y (D, V_hh).  Tiles of 64 (or 32 when few tiles would fill
+// the chip); grid (ceil((2Nk-1)/T), ceil(Nk/T), 3 nwires).
+template <int NB>  // 16 x 16 blocks per wave and dimension: tile = 32 NB x 32 NB
 __global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ descs) {
+  constexpr int TT = 32 * NB;
   const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || !(d.flags & PSS_EXCITONS)) return;
   if (m > 0 && !(d.flags & PSS_LF)) return;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
-  const int dd0 = blockIdx.x * 64, k10 = blockIdx.y * 64;
+  const int dd0 = blockIdx.x * TT, k10 = blockIdx.y * TT;
   if (dd0 >= nd || k10 >= Nk) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
   const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : B.D2;
   extern __shared__ double vrow[];  // Nk doubles: V(|q|) of this matrix
-  __shared__ double Br[16][65], Bi[16][65];
+  __shared__ double Br[16][TT + 1], Bi[16][TT + 1];
   for (int i = threadIdx.x; i < Nk; i += 256) vrow[i] = ws[L.vc + (int64_t)m * Nk + i];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int wr = wv >> 1, wc = wv & 1;
-  v4d accr[2][2], acci[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) { accr[a][b] = v4d{0, 0, 0, 0}; acci[a][b] = v4d{0, 0, 0, 0}; }
-  // u range with any nonzero X~[u][dd], dd in [dd0, dd0 + 64): u - d in [0, Nk)
-  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + 63, nd - 1) - (Nk - 1);
+  v4d accr[NB][NB], acci[NB][NB];
+  for (int a = 0; a < NB; ++a)
+    for (int b = 0; b < NB; ++b) { accr[a][b] = v4d{0, 0, 0, 0}; acci[a][b] = v4d{0, 0, 0, 0}; }
+  // u range with any nonzero X~[u][dd], dd in [dd0, dd0 + TT): u - d in [0, Nk)
+  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + TT - 1, nd - 1) - (Nk - 1);
   const int ulo = max(0, dlo), uhi = min(Nk - 1, Nk - 1 + dhi);
-  // ... and with T nonzero: u within [0, Nk) (all k1 rows of the tile are in range or masked)
   for (int u0 = (ulo / 16) * 16; u0 <= uhi; u0 += 16) {
     __syncthreads();
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      Br[e >> 6][e & 63] = 0.0;
-      Bi[e >> 6][e & 63] = 0.0;
+    for (int e = threadIdx.x; e < 16 * TT; e += 256) {
+      Br[e / TT][e % TT] = 0.0;
+      Bi[e / TT][e % TT] = 0.0;
     }
     __syncthreads();
     // rows r = u - d of X touched by this chunk: r in [u0 - dhi, u0 + 15 - dlo]
@@ -631,7 +633,7 @@ __global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ 
       const int r = r_lo + (sidx >> 4), u = u0 + (sidx & 15);
       if (u >= Nk) continue;
       const int ddl = u - r + (Nk - 1) - dd0;
-      if (ddl < 0 || ddl >= 64 || dd0 + ddl >= nd) continue;
+      if (ddl < 0 || ddl >= TT || dd0 + ddl >= nd) continue;
       const cplx v = X[(int64_t)r * Nk + u];
       Br[u - u0][ddl] = v.x;
       Bi[u - u0][ddl] = v.y;
@@ -640,19 +642,19 @@ __global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ 
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int uu = u0 + ks * 4 + (lane >> 4);  // A: lane -> T[k1 = row l%16][u = l/16]
-      double ta[2];
-      for (int a = 0; a < 2; ++a) {
-        const int k1 = k10 + wr * 32 + a * 16 + (lane & 15);
+      double ta[NB];
+      for (int a = 0; a < NB; ++a) {
+        const int k1 = k10 + wr * 16 * NB + a * 16 + (lane & 15);
         const int q = uu - k1;
         ta[a] = (q == 0 || uu >= Nk || k1 >= Nk) ? 0.0 : vrow[q < 0 ? -q : q];
       }
-      double br[2], bi[2];
-      for (int b = 0; b < 2; ++b) {  // B: lane -> X~[u = l/16][dd = l%16]
-        br[b] = Br[ks * 4 + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
-        bi[b] = Bi[ks * 4 + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
+      double br[NB], bi[NB];
+      for (int b = 0; b < NB; ++b) {  // B: lane -> X~[u = l/16][dd = l%16]
+        br[b] = Br[ks * 4 + (lane >> 4)][wc * 16 * NB + b * 16 + (lane & 15)];
+        bi[b] = Bi[ks * 4 + (lane >> 4)][wc * 16 * NB + b * 16 + (lane & 15)];
       }
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
+      for (int a = 0; a < NB; ++a)
+        for (int b = 0; b < NB; ++b) {
           accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], br[b], accr[a][b], 0, 0, 0);
           acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], bi[b], acci[a][b], 0, 0, 0);
         }
@@ -661,11 +663,11 @@ __global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ 
   // D element v of lane l: row (l >> 4) + 4 v, column l & 15
   cplx* H = (cplx*)(ws + L.H) + (int64_t)m * Nk * Nk;
   const double sg = m == 0 ? 1.0 : -1.0;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
+  for (int a = 0; a < NB; ++a)
+    for (int b = 0; b < NB; ++b)
       for (int v = 0; v < 4; ++v) {
-        const int k1 = k10 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
-        const int dd = dd0 + wc * 32 + b * 16 + (lane & 15);
+        const int k1 = k10 + wr * 16 * NB + a * 16 + (lane >> 4) + 4 * v;
+        const int dd = dd0 + wc * 16 * NB + b * 16 + (lane & 15);
         if (k1 >= Nk || dd >= nd) continue;
         const int k2 = k1 - (dd - (Nk - 1));
         if (k2 < 0 || k2 >= Nk) continue;
@@ -1750,7 +1752,11 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   {
     const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
-    hipLaunchKernelGGL(k_exch, dim3(tn, tm, 3 * nwires), dim3(256), (size_t)max_Nk * sizeof(double), s, d_descs);
+    if ((int64_t)tn * tm * 3 * nwires >= 512)
+      hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), (size_t)max_Nk * sizeof(double), s, d_descs);
+    else
+      hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires), dim3(256),
+                         (size_t)max_Nk * sizeof(double), s, d_descs);
   }
   {
     const int np = np64(max_Nk), t64 = np / 64;
