@@ -75,10 +75,10 @@ struct PassArgs {
   const double* kmul;   // wavenumbers along mul_axis (null = none)
   int Nx, Ny, Nz, axis, mul_axis, dir;  // dir -1 forward, +1 inverse
   double scale;
-  int accum;  // 1: out += result (the last pass of F_k += FFT(dF))
+  int accum;  // 0: out = result, 1: out += result (F_k += FFT(dF)), 2: real part to ((double*)out)[i]
 };
 
-template <int N, bool ACC>
+template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
   constexpr int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   constexpr int LOG = __builtin_ctz(N);
@@ -166,7 +166,8 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
     if (t < nvalid) {
       const int64_t o = base0 + (int64_t)t * st + (int64_t)i * sa;
       const cplx v = buf[t * N + i] * a.scale;
-      if (ACC) a.out[o] = a.out[o] + v;
+      if (MODE == 1) a.out[o] = a.out[o] + v;
+      else if (MODE == 2) ((double*)a.out)[o] = v.x;
       else a.out[o] = v;
     }
   }
@@ -205,7 +206,7 @@ __device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
 // per element instead of log2(N) staged radix-2 sweeps.
-template <int N1, int N2, bool ACC>
+template <int N1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
@@ -304,7 +305,8 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
         const int k = k1 + N1 * k2;
         const int64_t o = base0 + (int64_t)t * st + (int64_t)k * sa;
         const cplx v = u[k2] * a.scale;
-        if (ACC) a.out[o] = a.out[o] + v;
+        if (MODE == 1) a.out[o] = a.out[o] + v;
+        else if (MODE == 2) ((double*)a.out)[o] = v.x;
         else a.out[o] = v;
       }
     }
@@ -320,8 +322,9 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
-  if (a.accum) hipLaunchKernelGGL((k_fft2<N1, N2, true>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
-  else hipLaunchKernelGGL((k_fft2<N1, N2, false>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  else hipLaunchKernelGGL((k_fft2<N1, N2, 0>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
 }
 
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
@@ -358,8 +361,9 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   switch (N) {
 #define PSS_CASE(n)                                                                   \
   case n:                                                                             \
-    if (accum) hipLaunchKernelGGL((k_fft_pass<n, true>), grid, blk, 0, s, a);        \
-    else hipLaunchKernelGGL((k_fft_pass<n, false>), grid, blk, 0, s, a);             \
+    if (accum == 1) hipLaunchKernelGGL((k_fft_pass<n, 1>), grid, blk, 0, s, a);       \
+    else if (accum == 2) hipLaunchKernelGGL((k_fft_pass<n, 2>), grid, blk, 0, s, a);  \
+    else hipLaunchKernelGGL((k_fft_pass<n, 0>), grid, blk, 0, s, a);                  \
     break;
     PSS_CASE(2) PSS_CASE(4) PSS_CASE(8) PSS_CASE(16) PSS_CASE(32) PSS_CASE(64) PSS_CASE(128) PSS_CASE(256)
     PSS_CASE(512) PSS_CASE(1024) PSS_CASE(2048)
@@ -402,21 +406,62 @@ int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s) {
   return 0;
 }
 
+// Real parts of the two spectral derivatives of one field G along axes a1 and a2
+// (the axes other than c):  R_a = Re IFFT(i k_a G).  The plain inverse pass along c is
+// shared: T = IFFT_c(G); R_a1 = Re IFFT_a1(i k_a1 IFFT_a2(T)); R_a2 likewise with the
+// roles swapped (the factor i k_a is applied on the load of the pass along a, where
+// it commutes with the passes along the other axes).  5 passes instead of 6, the last
+// of each writing only the real part.  T, U: complex scratch.  Along an axis of
+// length 1, k = 0 and the derivative is zero.
+__global__ void k_zero(double* x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+int deriv_pair(const pss_grid* g, const cplx* G, int c, int a1, int a2, double* R1, double* R2, cplx* T, cplx* U,
+               hipStream_t s) {
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  const int len[3] = {g->Nx, g->Ny, g->Nz};
+  const double* ks[3] = {g->kx, g->ky, g->kz};
+  const cplx* t = G;
+  if (len[c] > 1) {
+    int rc = launch_pass(g, G, T, c, +1, nullptr, -1, s);
+    if (rc) return rc;
+    t = T;
+  }
+  for (int h = 0; h < 2; ++h) {
+    const int a = h == 0 ? a1 : a2, b = h == 0 ? a2 : a1;
+    double* R = h == 0 ? R1 : R2;
+    if (len[a] == 1) {
+      hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R, n);
+      continue;
+    }
+    const cplx* u = t;
+    if (len[b] > 1) {
+      int rc = launch_pass(g, t, U, b, +1, nullptr, -1, s);
+      if (rc) return rc;
+      u = U;
+    }
+    int rc = launch_pass(g, u, (cplx*)R, a, +1, ks[a], a, s, 2);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------ CPML
 // Real-space correction of PSTD3D/cpml.py:150-334 (psi only in PML cells) and the
 // soft-source add of PSTD3D/tfsf.py:106-136 on component src_comp, written as a
 // real field dF: the reference adds them to Re F in real space between an IFFT and
 // an FFT of F; by linearity F_k += FFT(dF) is the same update without taking F
-// to real space and back.  d0..d5 hold the real-space derivatives on entry and
-// dF_x, dF_y, dF_z (imaginary part 0) in d0, d1, d2 on exit (each thread reads its
-// cell's six derivatives before writing).
+// to real space and back.  R holds the real parts of the six derivatives; dF_x,
+// dF_y, dF_z (imaginary part 0) are written for every cell.
 struct SrcPar {
   double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
 };
 
-__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* d0, cplx* d1, cplx* d2,
-                                              const cplx* __restrict__ d3, const cplx* __restrict__ d4,
-                                              const cplx* __restrict__ d5, double* psi, int src_comp, SrcPar sp,
+__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const double* __restrict__ R,
+                                              cplx* __restrict__ dFx, cplx* __restrict__ dFy, cplx* __restrict__ dFz,
+                                              double* psi, int src_comp, SrcPar sp,
                                               const double* __restrict__ srcx, const double* __restrict__ rx,
                                               const double* __restrict__ ry, const double* __restrict__ rz) {
   const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
@@ -431,8 +476,9 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* d0, c
   const bool inz = k < g.npml_z || k >= g.Nz - g.npml_z;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (inx || iny || inz) {
-    // d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
-    const double gzy = d0[o].x, gyz = d1[o].x, gxz = d2[o].x, gzx = d3[o].x, gyx = d4[o].x, gxy = d5[o].x;
+    // R: [6][n] real parts of d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
+    const double gzy = R[o], gyz = R[n + o], gxz = R[2 * n + o], gzx = R[3 * n + o], gyx = R[4 * n + o],
+                 gxy = R[5 * n + o];
     double* pxy = psi;
     double* pxz = psi + n;
     double* pyx = psi + 2 * n;
@@ -463,9 +509,9 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, cplx* d0, c
     else if (src_comp == 1) fy += add;
     else fz += add;
   }
-  d0[o] = mk(fx, 0.0);
-  d1[o] = mk(fy, 0.0);
-  d2[o] = mk(fz, 0.0);
+  dFx[o] = mk(fx, 0.0);
+  dFy[o] = mk(fy, 0.0);
+  dFz[o] = mk(fz, 0.0);
 }
 
 // Trilinear sample (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085) of the corrected
@@ -482,11 +528,14 @@ __device__ inline cplx twp(const cplx* tw, int N, int r) {  // exp(+2 pi i r / N
 __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restrict__ Fx, const cplx* __restrict__ Fy,
                                                  const cplx* __restrict__ Fz, const int32_t* __restrict__ idx,
                                                  const double* __restrict__ w, double* partial) {
-  __shared__ cplx tx[1024], ty[1024], tz[1024];
-  __shared__ double sh[16];
+  extern __shared__ cplx tws[];  // Nx/2 + Ny/2 + Nz/2 twiddles
+  __shared__ double red[6][4];
   __shared__ int cx[8], cy[8], cz[8];
   __shared__ double cw[8];
   const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  cplx* tx = tws;
+  cplx* ty = tx + Nx / 2;
+  cplx* tz = ty + Ny / 2;
   for (int m = threadIdx.x; m < Nx / 2; m += 256) tx[m] = mk(g.twx[2 * m], g.twx[2 * m + 1]);
   for (int m = threadIdx.x; m < Ny / 2; m += 256) ty[m] = mk(g.twy[2 * m], g.twy[2 * m + 1]);
   for (int m = threadIdx.x; m < Nz / 2; m += 256) tz[m] = mk(g.twz[2 * m], g.twz[2 * m + 1]);
@@ -500,9 +549,10 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
   }
   __syncthreads();
   const int64_t n = (int64_t)Nx * Ny * Nz;
-  cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
   const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);  // axis lengths are powers of two
+  cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
   for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
+    const cplx f0 = Fx[o], f1 = Fy[o], f2 = Fz[o];
     const int mz = (int)(o & (Nz - 1));
     const int my = (int)((o >> lz) & (Ny - 1));
     const int mx = (int)(o >> (lz + ly));
@@ -514,15 +564,20 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
       const cplx ez = Nz > 1 ? twp(tz, Nz, mz * cz[c]) : mk(1, 0);
       phi = phi + cw[c] * (ex * ey * ez);
     }
-    a0 = a0 + Fx[o] * phi;
-    a1 = a1 + Fy[o] * phi;
-    a2 = a2 + Fz[o] * phi;
+    a0 = a0 + f0 * phi;
+    a1 = a1 + f1 * phi;
+    a2 = a2 + f2 * phi;
   }
-  const double v[6] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y};
+  double v[6] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y};
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int q = 0; q < 6; ++q) {
-    const double r = block_sum<256>(v[q], sh);
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 6 + q] = r;
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0) red[q][wid] = v[q];
   }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    partial[(int64_t)blockIdx.x * 6 + threadIdx.x] =
+        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
 }
 
 __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
@@ -641,34 +696,36 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   if (g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
     return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
+  // scratch: 6 complex fields.  Slots 0-2 hold the six real derivative arrays,
+  // slots 3-4 the complex temporaries of the derivative passes, then slots 3-5 dF.
   cplx* sc = (cplx*)scratch;
+  double* R = (double*)scratch;
   const cplx *gx = (const cplx*)Gx, *gy = (const cplx*)Gy, *gz = (const cplx*)Gz;
-  // spectral derivatives  d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
-  const cplx* srcs[6] = {gz, gy, gx, gz, gy, gx};
-  const int ax[6] = {1, 2, 2, 0, 0, 1};
-  const double* ks[3] = {g->kx, g->ky, g->kz};
-  for (int q = 0; q < 6; ++q) {
-    int rc = fft3(g, srcs[q], sc + q * n, +1, ks[ax[q]], ax[q], s);
-    if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
-  }
+  // R: 0 = dGz/dy, 1 = dGy/dz, 2 = dGx/dz, 3 = dGz/dx, 4 = dGy/dx, 5 = dGx/dy
+  int rc = deriv_pair(g, gz, 2, 1, 0, R, R + 3 * n, sc + 3 * n, sc + 4 * n, s);
+  if (!rc) rc = deriv_pair(g, gy, 1, 2, 0, R + n, R + 4 * n, sc + 3 * n, sc + 4 * n, s);
+  if (!rc) rc = deriv_pair(g, gx, 0, 2, 1, R + 2 * n, R + 5 * n, sc + 3 * n, sc + 4 * n, s);
+  if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
   SrcPar sp{};
   if (src_comp >= 0) {
     sp.t = src_par[0]; sp.amp = src_par[1]; sp.omega0 = src_par[2]; sp.tau_G = src_par[3];
     sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
   }
-  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, sc, sc + n, sc + 2 * n,
-                     sc + 3 * n, sc + 4 * n, sc + 5 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  cplx* dF = sc + 3 * n;
+  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R, dF, dF + n,
+                     dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
   if (sample_idx) {
-    // partial sums live in the (now free) fourth scratch field
-    const int nblk = (int)std::min<int64_t>(4096, (n + 255) / 256);
-    double* part = (double*)(sc + 3 * n);
-    hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), 0, s, *g, F[0], F[1], F[2], sample_idx, sample_w, part);
-    hipLaunchKernelGGL(k_ksample_final, dim3(1), dim3(256), 0, s, *g, part, nblk, sc, sc + n, sc + 2 * n, sample_idx,
-                       sample_w, sample_out);
+    // partial sums live in the (now free) derivative slots
+    const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    double* part = R;
+    const size_t lds = (size_t)(g->Nx / 2 + g->Ny / 2 + g->Nz / 2 + 1) * sizeof(cplx);
+    hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), lds, s, *g, F[0], F[1], F[2], sample_idx, sample_w, part);
+    hipLaunchKernelGGL(k_ksample_final, dim3(1), dim3(256), 0, s, *g, part, nblk, dF, dF + n, dF + 2 * n,
+                       sample_idx, sample_w, sample_out);
   }
   for (int c = 0; c < 3; ++c) {
-    int rc = fft3_acc(g, sc + c * n, F[c], -1, s);
+    rc = fft3_acc(g, dF + c * n, F[c], -1, s);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
   PSS_CHECK(hipGetLastError());
