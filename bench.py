@@ -77,7 +77,7 @@ def setup(grid, nk, npml, precision, feedback=False):
     mb = MBFlags(Screened=False, ignorewire=not feedback)  # bare Coulomb for this config
     qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, precision=precision)
     prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type="cpml",
-                            calc_j=qa, npml=npml, check_every=0)
+                            calc_j=qa, npml=npml, check_every=0, precision=precision)
     return prop, qa
 
 
@@ -273,7 +273,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128/f64" if main_prec == "fp64" else
-                 "mixed (SBE products fp16 hi/lo MFMA, fp32 accumulate; fields, populations, time step fp64)",
+                 "mixed (SBE products fp16 hi/lo MFMA, fp32 accumulate; CPML derivative transforms fp32; fields, "
+                 "populations, time step fp64)",
         "data": "synthetic (seeded soft-source pulse, reference initial wire state)",
         "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
                                f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,

@@ -116,7 +116,9 @@ int pss_vint(const double* y, int32_t Ny, const double* q, int32_t nq, double al
 
 /* 3-D grid (C order: x slowest, z fastest), complex128 fields. */
 typedef struct pss_grid {
-  int32_t Nx, Ny, Nz, npml_x, npml_y, npml_z, pad0, pad1;
+  int32_t Nx, Ny, Nz, npml_x, npml_y, npml_z;
+  int32_t precision; /* PSS_FP64, or PSS_MIXED: the CPML's spectral derivatives in FP32 */
+  int32_t pad1;
   double dt, v2, mu0;
   const double *kx, *ky, *kz;            /* [N] FFT-order wavenumbers */
   const double *bx, *cx, *ikx;           /* [Nx] CPML b, c, 1/kappa - 1 */

@@ -93,7 +93,8 @@ class PSTD3DPropagator:
     thickness override), ``check_every`` (NaN guard period), ``device``."""
 
     def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="cpml", calc_j=None,
-                 output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None):
+                 output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None,
+                 precision="fp64"):
         if boundary_type == "mask":
             raise NotImplementedError("boundary_type='mask' (absorber) is not built on the device; use 'cpml'")
         if boundary_type not in ("cpml", "none"):
@@ -112,6 +113,12 @@ class PSTD3DPropagator:
         if boundary_type == "none":
             self.cpml.ctx.g.npml_x = self.cpml.ctx.g.npml_y = self.cpml.ctx.g.npml_z = 0
         self.ctx = self.cpml.ctx
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"unknown precision {precision!r}")
+        # mixed: the CPML's spectral derivatives (real parts, PML cells only) in FP32;
+        # the fields, their transforms and the curl updates stay FP64
+        self.precision = precision
+        self.ctx.g.precision = _lib.MIXED if precision == "mixed" else _lib.FP64
         self.v = _c0 / np.sqrt(GetEpsr(space))
         self.v2 = _c0**2 / GetEpsr(space)
         f = self.ctx.field

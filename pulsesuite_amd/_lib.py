@@ -48,7 +48,7 @@ class WireDesc(C.Structure):
 
 class Grid(C.Structure):
     _fields_ = (
-        [(n, _i) for n in ("Nx", "Ny", "Nz", "npml_x", "npml_y", "npml_z", "pad0", "pad1")]
+        [(n, _i) for n in ("Nx", "Ny", "Nz", "npml_x", "npml_y", "npml_z", "precision", "pad1")]
         + [("dt", _d), ("v2", _d), ("mu0", _d)]
         + [(n, _vp) for n in ("kx", "ky", "kz", "bx", "cx", "ikx", "by", "cy", "iky", "bz", "cz", "ikz",
                               "twx", "twy", "twz")]

@@ -68,9 +68,25 @@ __global__ void __launch_bounds__(256) k_update_b(pss_grid g, const cplx* __rest
 // pre-multiply by i*k_m along axis `mul_axis` (spectral derivative), scale on
 // store (1/N per pass for the inverse).  In place is allowed (each block owns
 // its lines).
+// complex of either precision for the templated passes
+template <class T>
+struct cpx {
+  T x, y;
+};
+template <class T> __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T> __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T> __device__ __forceinline__ cpx<T> operator*(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class T> __device__ __forceinline__ cpx<T> operator*(cpx<T> a, T s) { return {s * a.x, s * a.y}; }
+struct cflt {
+  float x, y;
+};
+
 struct PassArgs {
   const cplx* in;
   cplx* out;
+  int in32, out32;      // element types of in / out: 0 complex128 (real double), 1 complex64 (real float)
   const double* tw;     // exp(-2 pi i m/N), m < N/2, interleaved
   const double* kmul;   // wavenumbers along mul_axis (null = none)
   int Nx, Ny, Nz, axis, mul_axis, dir;  // dir -1 forward, +1 inverse
@@ -175,14 +191,14 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassArgs a) {
 
 // In-register radix-2 DFT of R points (natural order in and out); twiddles
 // W_{2h}^j = tw[j * (N / 2h)] from the length-N table in LDS.
-template <int R, int N>
-__device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
+template <int R, int N, class T>
+__device__ __forceinline__ void dft_reg(cpx<T>* v, const cpx<T>* tw) {
   constexpr int LOG = __builtin_ctz(R);
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int r = (int)(__builtin_bitreverse32((unsigned)i) >> (32 - LOG));
     if (r > i) {
-      const cplx tmp = v[i];
+      const cpx<T> tmp = v[i];
       v[i] = v[r];
       v[r] = tmp;
     }
@@ -193,8 +209,8 @@ __device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
     for (int g0 = 0; g0 < R; g0 += 2 * h) {
 #pragma unroll
       for (int j = 0; j < h; ++j) {
-        const cplx w = tw[j * (N / (2 * h))];
-        const cplx x0 = v[g0 + j], x1 = v[g0 + j + h] * w;
+        const cpx<T> w = tw[j * (N / (2 * h))];
+        const cpx<T> x0 = v[g0 + j], x1 = v[g0 + j + h] * w;
         v[g0 + j] = x0 + x1;
         v[g0 + j + h] = x0 - x1;
       }
@@ -202,18 +218,42 @@ __device__ __forceinline__ void dft_reg(cplx* v, const cplx* tw) {
   }
 }
 
+template <class T>
+__device__ __forceinline__ cpx<T> load_in(const PassArgs& a, int64_t o) {
+  if (a.in32) {
+    const cflt v = ((const cflt*)a.in)[o];
+    return {(T)v.x, (T)v.y};
+  }
+  const cplx v = a.in[o];
+  return {(T)v.x, (T)v.y};
+}
+
+// MODE 0: out = v, MODE 1: out += v (complex128), MODE 2: out = Re v (real array)
+template <int MODE, class T>
+__device__ __forceinline__ void store_out(const PassArgs& a, int64_t o, cpx<T> v) {
+  if (MODE == 1) {
+    a.out[o] = a.out[o] + mk((double)v.x, (double)v.y);
+  } else if (MODE == 2) {
+    if (a.out32) ((float*)a.out)[o] = (float)v.x;
+    else ((double*)a.out)[o] = (double)v.x;
+  } else {
+    if (a.out32) ((cflt*)a.out)[o] = cflt{(float)v.x, (float)v.y};
+    else a.out[o] = mk((double)v.x, (double)v.y);
+  }
+}
+
 // Two-level transform N = N1 x N2 along `axis`: each line is read straight from
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
 // per element instead of log2(N) staged radix-2 sweeps.
-template <int N1, int N2, int MODE>
+template <int N1, int N2, int MODE, class T>
 __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
   constexpr int TB0 = 256 / G, TBL = 2048 / N;
   constexpr int TB = TB0 < TBL ? TB0 : TBL;
-  __shared__ cplx Y[TB * N1 * (N2 + 1)];
-  __shared__ cplx tws[N / 2];
+  __shared__ cpx<T> Y[TB * N1 * (N2 + 1)];
+  __shared__ cpx<T> tws[N / 2];
   const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
   int64_t base0, sa, st;
   int nvalid;
@@ -248,7 +288,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   }
   for (int m = threadIdx.x; m < N / 2; m += TB * G) {
     const double wr = a.tw[2 * m], wi = a.tw[2 * m + 1];
-    tws[m] = mk(wr, a.dir > 0 ? -wi : wi);
+    tws[m] = cpx<T>{(T)wr, (T)(a.dir > 0 ? -wi : wi)};
   }
   __syncthreads();
   const int tid = threadIdx.x;
@@ -256,7 +296,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   {
     int t, n2;
     if (a.axis == 2) { n2 = tid % G; t = tid / G; } else { t = tid % TB; n2 = tid / TB; }
-    cplx v[N1];
+    cpx<T> v[N1];
     double kf = 0.0;
     bool kconst = false;
     if (a.kmul && a.mul_axis != a.axis) {
@@ -271,22 +311,22 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
     }
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-      cplx x = mk(0, 0);
+      cpx<T> x = {(T)0, (T)0};
       if (t < nvalid) {
         const int idx = n1 * N2 + n2;
-        x = a.in[base0 + (int64_t)t * st + (int64_t)idx * sa];
+        x = load_in<T>(a, base0 + (int64_t)t * st + (int64_t)idx * sa);
         if (a.kmul) {
-          const double k = kconst ? kf : a.kmul[idx];
-          x = mk(-k * x.y, k * x.x);  // i k x
+          const T k = (T)(kconst ? kf : a.kmul[idx]);
+          x = cpx<T>{-k * x.y, k * x.x};  // i k x
         }
       }
       v[n1] = x;
     }
-    dft_reg<N1, N>(v, tws);
+    dft_reg<N1, N, T>(v, tws);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
       const int m = (n2 * k1) % N;
-      const cplx w = m < N / 2 ? tws[m] : mk(-tws[m - N / 2].x, -tws[m - N / 2].y);
+      const cpx<T> w = m < N / 2 ? tws[m] : cpx<T>{-tws[m - N / 2].x, -tws[m - N / 2].y};
       Y[(t * N1 + k1) * (N2 + 1) + n2] = v[k1] * w;
     }
   }
@@ -296,18 +336,15 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
     int t, k1;
     if (a.axis == 2) { k1 = tid % G; t = tid / G; } else { t = tid % TB; k1 = tid / TB; }
     if (k1 < N1 && t < nvalid) {
-      cplx u[N2];
+      cpx<T> u[N2];
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) u[n2] = Y[(t * N1 + k1) * (N2 + 1) + n2];
-      dft_reg<N2, N>(u, tws);
+      dft_reg<N2, N, T>(u, tws);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
         const int k = k1 + N1 * k2;
         const int64_t o = base0 + (int64_t)t * st + (int64_t)k * sa;
-        const cplx v = u[k2] * a.scale;
-        if (MODE == 1) a.out[o] = a.out[o] + v;
-        else if (MODE == 2) ((double*)a.out)[o] = v.x;
-        else a.out[o] = v;
+        store_out<MODE, T>(a, o, u[k2] * (T)a.scale);
       }
     }
   }
@@ -322,13 +359,17 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
-  if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
-  else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
-  else hipLaunchKernelGGL((k_fft2<N1, N2, 0>), dim3((unsigned)blocks), dim3(TB * G), 0, s, a);
+  const dim3 gr((unsigned)blocks), bl(TB * G);
+  if (a.in32 || a.out32) {  // FP32 arithmetic whenever either side is single precision
+    if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float>), gr, bl, 0, s, a);
+    else hipLaunchKernelGGL((k_fft2<N1, N2, 0, float>), gr, bl, 0, s, a);
+  } else if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, double>), gr, bl, 0, s, a);
+  else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, double>), gr, bl, 0, s, a);
+  else hipLaunchKernelGGL((k_fft2<N1, N2, 0, double>), gr, bl, 0, s, a);
 }
 
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
-                hipStream_t s, int accum = 0) {
+                hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
   if (N == 1) {
     if (in != out) return -1;
@@ -345,6 +386,8 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   a.dir = dir;
   a.scale = dir > 0 ? 1.0 / N : 1.0;
   a.accum = accum;
+  a.in32 = in32;
+  a.out32 = out32;
   switch (N) {
     case 64: launch_fft2<8, 8>(a, g, s); return 0;
     case 128: launch_fft2<8, 16>(a, g, s); return 0;
@@ -353,6 +396,7 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
     case 1024: launch_fft2<32, 32>(a, g, s); return 0;
     default: break;
   }
+  if (in32 || out32) return -3;  // single-precision passes exist for N = 64 ... 1024 only
   const int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   int64_t blocks;
   if (axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
@@ -413,36 +457,43 @@ int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s) {
 // it commutes with the passes along the other axes).  5 passes instead of 6, the last
 // of each writing only the real part.  T, U: complex scratch.  Along an axis of
 // length 1, k = 0 and the derivative is zero.
-__global__ void k_zero(double* x, int64_t n) {
+template <class T>
+__global__ void k_zero(T* x, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) x[i] = 0.0;
+  if (i < n) x[i] = (T)0;
 }
 
-int deriv_pair(const pss_grid* g, const cplx* G, int c, int a1, int a2, double* R1, double* R2, cplx* T, cplx* U,
-               hipStream_t s) {
+int deriv_pair(const pss_grid* g, const cplx* G, int c, int a1, int a2, void* R1, void* R2, cplx* T, cplx* U,
+               int fp32, hipStream_t s) {
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   const int len[3] = {g->Nx, g->Ny, g->Nz};
   const double* ks[3] = {g->kx, g->ky, g->kz};
   const cplx* t = G;
+  int t32 = 0;  // element type of t
   if (len[c] > 1) {
-    int rc = launch_pass(g, G, T, c, +1, nullptr, -1, s);
+    int rc = launch_pass(g, G, T, c, +1, nullptr, -1, s, 0, 0, fp32);
     if (rc) return rc;
     t = T;
+    t32 = fp32;
   }
   for (int h = 0; h < 2; ++h) {
     const int a = h == 0 ? a1 : a2, b = h == 0 ? a2 : a1;
-    double* R = h == 0 ? R1 : R2;
+    void* R = h == 0 ? R1 : R2;
     if (len[a] == 1) {
-      hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R, n);
+      // k = 0 along a length-1 axis: the derivative is zero
+      if (fp32) hipLaunchKernelGGL((k_zero<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (float*)R, n);
+      else hipLaunchKernelGGL((k_zero<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (double*)R, n);
       continue;
     }
     const cplx* u = t;
+    int u32 = t32;
     if (len[b] > 1) {
-      int rc = launch_pass(g, t, U, b, +1, nullptr, -1, s);
+      int rc = launch_pass(g, t, U, b, +1, nullptr, -1, s, 0, t32, fp32);
       if (rc) return rc;
       u = U;
+      u32 = fp32;
     }
-    int rc = launch_pass(g, u, (cplx*)R, a, +1, ks[a], a, s, 2);
+    int rc = launch_pass(g, u, (cplx*)R, a, +1, ks[a], a, s, 2, u32, fp32);
     if (rc) return rc;
   }
   return 0;
@@ -459,7 +510,8 @@ struct SrcPar {
   double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
 };
 
-__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const double* __restrict__ R,
+template <class TR>
+__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* __restrict__ R,
                                               cplx* __restrict__ dFx, cplx* __restrict__ dFy, cplx* __restrict__ dFz,
                                               double* psi, int src_comp, SrcPar sp,
                                               const double* __restrict__ srcx, const double* __restrict__ rx,
@@ -477,8 +529,8 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const doubl
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (inx || iny || inz) {
     // R: [6][n] real parts of d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
-    const double gzy = R[o], gyz = R[n + o], gxz = R[2 * n + o], gzx = R[3 * n + o], gyx = R[4 * n + o],
-                 gxy = R[5 * n + o];
+    const double gzy = (double)R[o], gyz = (double)R[n + o], gxz = (double)R[2 * n + o],
+                 gzx = (double)R[3 * n + o], gyx = (double)R[4 * n + o], gxy = (double)R[5 * n + o];
     double* pxy = psi;
     double* pxz = psi + n;
     double* pyx = psi + 2 * n;
@@ -699,12 +751,18 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   // scratch: 6 complex fields.  Slots 0-2 hold the six real derivative arrays,
   // slots 3-4 the complex temporaries of the derivative passes, then slots 3-5 dF.
   cplx* sc = (cplx*)scratch;
+  // mixed precision: derivative passes in FP32 when every axis uses the two-level kernels
+  bool fp32 = g->precision == PSS_MIXED;
+  for (int N : {g->Nx, g->Ny, g->Nz})
+    if (N > 1 && (N < 64 || N > 1024)) fp32 = false;
   double* R = (double*)scratch;
+  float* Rf = (float*)scratch;
   const cplx *gx = (const cplx*)Gx, *gy = (const cplx*)Gy, *gz = (const cplx*)Gz;
   // R: 0 = dGz/dy, 1 = dGy/dz, 2 = dGx/dz, 3 = dGz/dx, 4 = dGy/dx, 5 = dGx/dy
-  int rc = deriv_pair(g, gz, 2, 1, 0, R, R + 3 * n, sc + 3 * n, sc + 4 * n, s);
-  if (!rc) rc = deriv_pair(g, gy, 1, 2, 0, R + n, R + 4 * n, sc + 3 * n, sc + 4 * n, s);
-  if (!rc) rc = deriv_pair(g, gx, 0, 2, 1, R + 2 * n, R + 5 * n, sc + 3 * n, sc + 4 * n, s);
+  auto rq = [&](int q) -> void* { return fp32 ? (void*)(Rf + q * n) : (void*)(R + q * n); };
+  int rc = deriv_pair(g, gz, 2, 1, 0, rq(0), rq(3), sc + 3 * n, sc + 4 * n, fp32, s);
+  if (!rc) rc = deriv_pair(g, gy, 1, 2, 0, rq(1), rq(4), sc + 3 * n, sc + 4 * n, fp32, s);
+  if (!rc) rc = deriv_pair(g, gx, 0, 2, 1, rq(2), rq(5), sc + 3 * n, sc + 4 * n, fp32, s);
   if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
   SrcPar sp{};
   if (src_comp >= 0) {
@@ -712,8 +770,12 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
   }
   cplx* dF = sc + 3 * n;
-  hipLaunchKernelGGL(k_cpml, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R, dF, dF + n,
-                     dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  if (fp32)
+    hipLaunchKernelGGL((k_cpml<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, Rf, dF,
+                       dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  else
+    hipLaunchKernelGGL((k_cpml<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R, dF,
+                       dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
   if (sample_idx) {
     // partial sums live in the (now free) derivative slots

@@ -190,3 +190,48 @@ def test_cpml_128_matches_oracle():
     cp.ApplyCPML_B(*dB, *dE, None, None, None, dt)
     for a, b in zip(dE + dB, E + B):
         assert rel(a, b) < 1e-13
+
+
+def test_cpml_mixed_precision_derivatives():
+    """Mixed precision: the six spectral derivatives of a CPML half step in FP32 (real
+    parts, used only in PML cells), fields FP64.  Bar 1e-6 against the FP64 oracle on
+    one E and one B half step at 128^3 with 12-cell layers."""
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd.cpml import CPML
+
+    N, dx, epsr = 128, 20e-9, 9.1
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    rng = np.random.default_rng(6)
+    E = [rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)) for _ in range(3)]
+    B = [1e-8 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))) for _ in range(3)]
+    dE, dB = [cu(a) for a in E], [cu(a) for a in B]
+    sp = OP.Space(3, N, N, N, dx, dx, dx, epsr)
+    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
+    kx, ky, kz = OP.GetKxArray(sp), OP.GetKyArray(sp), OP.GetKzArray(sp)
+    v2 = 2.99792458e8**2 / epsr
+    ocp.ApplyCPML_E(*E, *B, kx, ky, kz, v2, dt)
+    ocp.ApplyCPML_B(*B, *E, kx, ky, kz, dt)
+    cp = CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
+    cp.ctx.g.precision = _lib.MIXED
+    cp.ApplyCPML_E(*dE, *dB, None, None, None, v2, dt)
+    cp.ApplyCPML_B(*dB, *dE, None, None, None, dt)
+    errs = [float(rel(a, b)) for a, b in zip(dE + dB, E + B)]
+    print("mixed CPML rel errors", ["%.1e" % e for e in errs], flush=True)
+    assert max(errs) < 1e-6
+    assert max(errs) > 0.0  # the FP32 path ran
+
+
+def test_propagator_golden_mixed(gold):
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+
+    Nx, Ny, Nz, dx, dy, dz, epsr, dt = gold["cpml_grid"]
+    sp = ss(3, int(Nx), int(Ny), int(Nz), dx, dy, dz, epsr)
+    lam, amp, tw, tp, ch, _ = gold["tfsf_pulse"]
+    prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), ps(lam, amp, tw, tp, ch), source_type="soft",
+                            boundary_type="cpml", precision="mixed")
+    prop.run(int(gold["run_nsteps"][0]))
+    for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+        assert rel(getattr(prop, n), gold["run_" + n]) < 1e-6, n
