@@ -361,7 +361,8 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
   const dim3 gr((unsigned)blocks), bl(TB * G);
   if (a.in32 || a.out32) {  // FP32 arithmetic whenever either side is single precision
-    if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float>), gr, bl, 0, s, a);
+    if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, float>), gr, bl, 0, s, a);
+    else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float>), gr, bl, 0, s, a);
     else hipLaunchKernelGGL((k_fft2<N1, N2, 0, float>), gr, bl, 0, s, a);
   } else if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, double>), gr, bl, 0, s, a);
   else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, double>), gr, bl, 0, s, a);
@@ -434,8 +435,9 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
 }
 
 // out += FFT(tmp) along every axis of length > 1; tmp is overwritten (used in place
-// for all passes but the last, which accumulates into out).
-int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s) {
+// for all passes but the last, which accumulates into out).  tmp32: tmp holds
+// complex64 and the passes compute in FP32 (out stays complex128).
+int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s, int tmp32 = 0) {
   int last = -1;
   for (int axis = 0; axis < 3; ++axis)
     if ((axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz) > 1) last = axis;
@@ -444,7 +446,8 @@ int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s) {
     const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
     if (N == 1) continue;
     const bool fin = axis == last;
-    const int rc = launch_pass(g, tmp, fin ? out : tmp, axis, dir, nullptr, -1, s, fin ? 1 : 0);
+    const int rc = launch_pass(g, tmp, fin ? out : tmp, axis, dir, nullptr, -1, s, fin ? 1 : 0, tmp32,
+                               fin ? 0 : tmp32);
     if (rc) return rc;
   }
   return 0;
@@ -510,9 +513,9 @@ struct SrcPar {
   double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
 };
 
-template <class TR>
+template <class TR, class TF>
 __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* __restrict__ R,
-                                              cplx* __restrict__ dFx, cplx* __restrict__ dFy, cplx* __restrict__ dFz,
+                                              TF* __restrict__ dFx, TF* __restrict__ dFy, TF* __restrict__ dFz,
                                               double* psi, int src_comp, SrcPar sp,
                                               const double* __restrict__ srcx, const double* __restrict__ rx,
                                               const double* __restrict__ ry, const double* __restrict__ rz) {
@@ -561,9 +564,9 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
     else if (src_comp == 1) fy += add;
     else fz += add;
   }
-  dFx[o] = mk(fx, 0.0);
-  dFy[o] = mk(fy, 0.0);
-  dFz[o] = mk(fz, 0.0);
+  dFx[o] = TF{(decltype(TF::x))fx, 0};
+  dFy[o] = TF{(decltype(TF::x))fy, 0};
+  dFz[o] = TF{(decltype(TF::x))fz, 0};
 }
 
 // Trilinear sample (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085) of the corrected
@@ -632,9 +635,10 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
         ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
 }
 
+template <class TF>
 __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
-                                                       const cplx* __restrict__ dFx, const cplx* __restrict__ dFy,
-                                                       const cplx* __restrict__ dFz, const int32_t* __restrict__ idx,
+                                                       const TF* __restrict__ dFx, const TF* __restrict__ dFy,
+                                                       const TF* __restrict__ dFz, const int32_t* __restrict__ idx,
                                                        const double* __restrict__ w, double* out) {
   __shared__ double sh[16];
   const double invn = 1.0 / ((double)g.Nx * g.Ny * g.Nz);
@@ -645,9 +649,9 @@ __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double*
     if (threadIdx.x == 0) {
       double r = acc * invn;
       if ((q & 1) == 0) {  // dF is real: add the sampled correction to the real parts
-        const cplx* dF = q == 0 ? dFx : q == 2 ? dFy : dFz;
+        const TF* dF = q == 0 ? dFx : q == 2 ? dFy : dFz;
         for (int c = 0; c < 8; ++c)
-          if (idx[c] >= 0) r += w[c] * dF[idx[c]].x;
+          if (idx[c] >= 0) r += w[c] * (double)dF[idx[c]].x;
       }
       out[q] = r;
     }
@@ -769,13 +773,14 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     sp.t = src_par[0]; sp.amp = src_par[1]; sp.omega0 = src_par[2]; sp.tau_G = src_par[3];
     sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
   }
-  cplx* dF = sc + 3 * n;
+  cplx* dF = sc + 3 * n;       // complex128 dF (FP64), or
+  cflt* dFf = (cflt*)dF;       // complex64 dF (mixed: transformed in FP32, added to the FP64 fields)
   if (fp32)
-    hipLaunchKernelGGL((k_cpml<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, Rf, dF,
-                       dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+    hipLaunchKernelGGL((k_cpml<float, cflt>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, Rf,
+                       dFf, dFf + n, dFf + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   else
-    hipLaunchKernelGGL((k_cpml<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R, dF,
-                       dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+    hipLaunchKernelGGL((k_cpml<double, cplx>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R,
+                       dF, dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
   if (sample_idx) {
     // partial sums live in the (now free) derivative slots
@@ -783,11 +788,15 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     double* part = R;
     const size_t lds = (size_t)(g->Nx / 2 + g->Ny / 2 + g->Nz / 2 + 1) * sizeof(cplx);
     hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), lds, s, *g, F[0], F[1], F[2], sample_idx, sample_w, part);
-    hipLaunchKernelGGL(k_ksample_final, dim3(1), dim3(256), 0, s, *g, part, nblk, dF, dF + n, dF + 2 * n,
-                       sample_idx, sample_w, sample_out);
+    if (fp32)
+      hipLaunchKernelGGL((k_ksample_final<cflt>), dim3(1), dim3(256), 0, s, *g, part, nblk, dFf, dFf + n,
+                         dFf + 2 * n, sample_idx, sample_w, sample_out);
+    else
+      hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, part, nblk, dF, dF + n,
+                         dF + 2 * n, sample_idx, sample_w, sample_out);
   }
   for (int c = 0; c < 3; ++c) {
-    rc = fft3_acc(g, dF + c * n, F[c], -1, s);
+    rc = fp32 ? fft3_acc(g, (cplx*)(dFf + c * n), F[c], -1, s, 1) : fft3_acc(g, dF + c * n, F[c], -1, s);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
   PSS_CHECK(hipGetLastError());
