@@ -30,6 +30,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the serving and batched runs spread small launches over up to 8 streams; give the
+# process as many hardware queues (HIP reads this at its first GPU call)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 METRIC = "k-point·steps/s mixed vs FP64 GPU at ≤1e-6 rel err; p50/p99 solve latency"
 HBM_PEAK = 8.0e12
@@ -53,8 +56,8 @@ def parse():
     ap.add_argument("--batched-eager", action="store_true", help="batched run without autotune or graph capture")
     ap.add_argument("--workloads", default="coupled,batched,serving",
                     help="comma list of coupled, batched, serving (the JSON line needs coupled)")
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving run (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
     return ap.parse_args()
 
 

@@ -94,7 +94,7 @@ class PSTD3DPropagator:
 
     def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="cpml", calc_j=None,
                  output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None,
-                 precision="fp64"):
+                 precision="fp64", device_time=False):
         if boundary_type == "mask":
             raise NotImplementedError("boundary_type='mask' (absorber) is not built on the device; use 'cpml'")
         if boundary_type not in ("cpml", "none"):
@@ -119,6 +119,12 @@ class PSTD3DPropagator:
         # the fields, their transforms and the curl updates stay FP64
         self.precision = precision
         self.ctx.g.precision = _lib.MIXED if precision == "mixed" else _lib.FP64
+        # device_time: the source reads the time from a device scalar advanced by a kernel
+        # after each half step, so a half step can be captured once and replayed
+        self._tdev = None
+        if device_time:
+            self._tdev = torch.full((1,), float(time.t), dtype=torch.float64, device=self.device)
+            self.ctx.g.tdev = self._tdev.data_ptr()
         self.v = _c0 / np.sqrt(GetEpsr(space))
         self.v2 = _c0**2 / GetEpsr(space)
         f = self.ctx.field
@@ -142,7 +148,8 @@ class PSTD3DPropagator:
             return None
         p = self.pulse
         w0inv2 = 0.0 if not np.isfinite(p.w0) else 1.0 / p.w0**2
-        par = (_lib.C.c_double * 9)(t, amp, p.CalcOmega0(), p.CalcTau(), p.chirp, self.v, p.Tp, w0inv2,
+        t_arg = 0.0 if self._tdev is not None else t  # device time: offset 0 from *tdev
+        par = (_lib.C.c_double * 9)(t_arg, amp, p.CalcOmega0(), p.CalcTau(), p.chirp, self.v, p.Tp, w0inv2,
                                     GetDx(self.space))
         return (comp, par, self._tfsf, self._rx, self._ry, self._rz)
 
@@ -171,6 +178,7 @@ class PSTD3DPropagator:
         sample = self.calc_j.sample_args() if sep else None
         self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
                               src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
+        self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
 
     def step_b_half(self, stream=None):
@@ -183,11 +191,16 @@ class PSTD3DPropagator:
                                            P(self.Bz), st), "pss_update_b3d")
         self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, dt=tm.dt,
                               src=self._src(2, self.pulse.Amp / self.v, tm.t), stream=stream)
+        self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
         tm.n += 1
         self._nstep += 1
         if self.check_every and self._nstep % self.check_every == 0:
             self.check_finite()
+
+    def _advance(self, dt2, st):
+        if self._tdev is not None:
+            _lib.check(_lib.lib.pss_add_scalar(_lib.ptr(self._tdev), dt2, st), "pss_add_scalar")
 
     def check_finite(self):
         for F in (self.Ex, self.Ey, self.Ez):

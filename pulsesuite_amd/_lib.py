@@ -51,7 +51,7 @@ class Grid(C.Structure):
         [(n, _i) for n in ("Nx", "Ny", "Nz", "npml_x", "npml_y", "npml_z", "precision", "pad1")]
         + [("dt", _d), ("v2", _d), ("mu0", _d)]
         + [(n, _vp) for n in ("kx", "ky", "kz", "bx", "cx", "ikx", "by", "cy", "iky", "bz", "cz", "ikz",
-                              "twx", "twy", "twz")]
+                              "twx", "twy", "twz", "tdev")]
     )
 
 
@@ -75,6 +75,7 @@ def _load():
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
         "pss_drive_uniform": (C.c_int, [_vp, _vp, _i, _vp]),
+        "pss_add_scalar": (C.c_int, [_vp, _d, _vp]),
         "pss_rk4_stage": (C.c_int, [_i, C.c_int64, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_last_error": (C.c_char_p, []),
         "pss_version": (C.c_char_p, []),
@@ -89,7 +90,7 @@ def _load():
 lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform",
+           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 
 

@@ -556,7 +556,8 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
     fz = sc * cz_;
   }
   if (src_comp >= 0) {
-    const double tau = sp.t - rx[i] / sp.v - sp.Tp;
+    const double t = g.tdev ? *g.tdev + sp.t : sp.t;
+    const double tau = t - rx[i] / sp.v - sp.Tp;
     const double gyz = exp(-(ry[j] * ry[j] + rz[k] * rz[k]) * sp.w0inv2);
     const double einc = sp.amp * gyz * exp(-(tau * tau) / (sp.tau_G * sp.tau_G)) * cos(sp.omega0 * tau + sp.chirp * tau * tau);
     const double add = srcx[i] * einc * sp.dx;
@@ -799,6 +800,17 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     rc = fp32 ? fft3_acc(g, (cplx*)(dFf + c * n), F[c], -1, s, 1) : fft3_acc(g, dF + c * n, F[c], -1, s);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+__global__ void k_add_scalar(double* x, double v) {
+  if (threadIdx.x == 0) x[0] += v;
+}
+
+extern "C" int pss_add_scalar(double* x, double v, void* stream) {
+  if (!x) return pss_set_msg(2, "pss_add_scalar: null pointer");
+  hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(64), 0, (hipStream_t)stream, x, v);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

@@ -10,6 +10,9 @@ device and advances all of them together, one tick per step:
   2. one batched SBE step over every in-flight wire (wires of different Nk share the
      launches: sorted by Nk, cut into work-balanced buckets, as in batch.WireBatch);
   3. per solve: current line, its FFT, B half step.
+Steps 1 and 3 of a solve are captured as two HIP graphs at its first tick (the
+source time lives in a device counter), so a tick costs two graph launches per
+solve plus the batched SBE launches.
 
 Requests that arrive while others run are admitted at the next tick when a slot is
 free and their device memory fits in the HBM budget (first fit in arrival order);
@@ -89,24 +92,63 @@ class Solve:
         self.qa = QWArray(QWParams(L=L_of_nk(req.Nk)), mbf, Nw=1, Emax0=0.0, precision=precision)
         self.prop = PSTD3DPropagator(sp, ts(0.0, 1.0, DT, 0), ps(800e-9, req.E0, 100e-15, 50e-15, 0.0),
                                      source_type="soft", boundary_type="cpml", calc_j=self.qa, npml=NPML,
-                                     check_every=0, device=device)
+                                     check_every=0, device=device, precision=precision, device_time=True)
         if self.qa.model.Nk != req.Nk:
             raise ValueError(f"request {req.rid}: Nk={req.Nk} not reachable (gives {self.qa.model.Nk})")
         self.done = 0
         self.nbytes = solve_bytes(req.Nk)
         self.admitted = None
         self.event = None
+        self.g_e = self.g_b = None
 
     def desc(self):
         return self.qa.wires.descs[0]
 
-    def half_e(self):
+    def _launch_e(self):
         self.prop.step_e_half()
         self.qa.drive(self.prop.space, self.prop.time)
 
-    def half_b(self):
+    def _launch_b(self):
         self.qa.deposit(self.prop.space, self.prop.time)
         self.prop.step_b_half()
+
+    def capture(self):
+        """Record the two half steps (E side with the wire drive line, B side with the
+        current line) as HIP graphs; the source time is a device counter, so replays
+        advance it.  Host bookkeeping done by the launches is rolled back."""
+        import torch
+
+        tm, prop = self.prop.time, self.prop
+        saved = (tm.t, tm.n, prop._nstep)
+        s = torch.cuda.Stream(device=prop.device)
+        s.wait_stream(torch.cuda.current_stream(prop.device))
+        self.g_e, self.g_b = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(self.g_e, stream=s):
+                self._launch_e()
+            with torch.cuda.graph(self.g_b, stream=s):
+                self._launch_b()
+        torch.cuda.current_stream(prop.device).wait_stream(s)
+        tm.t, tm.n, prop._nstep = saved
+
+    def half_e(self, graphs=False):
+        if graphs:
+            if self.g_e is None:
+                self.capture()
+            self.g_e.replay()
+            self.prop.time.t += self.prop.time.dt / 2.0
+        else:
+            self._launch_e()
+
+    def half_b(self, graphs=False):
+        if graphs:
+            self.g_b.replay()
+            tm = self.prop.time
+            tm.t += tm.dt / 2.0
+            tm.n += 1
+            self.prop._nstep += 1
+        else:
+            self._launch_b()
         self.done += 1
 
     def result(self):
@@ -120,7 +162,7 @@ class Server:
     """Admission by HBM budget and slot count, one batched tick per step."""
 
     def __init__(self, hbm_budget=64 << 30, max_slots=64, precision="mixed", mb_flags=None, buckets=4,
-                 device=None, feedback=False, keep_results=False):
+                 device=None, feedback=False, keep_results=False, graphs=True, streams=8):
         import torch
 
         self.budget = int(hbm_budget)
@@ -131,6 +173,13 @@ class Server:
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.feedback = feedback
         self.keep = keep_results
+        self.graphs = graphs
+        # the per-solve half steps are small (1-D lines): spread them over a few streams
+        # (default priority) so they run side by side; joined before and after the SBE step
+        self.nstreams = max(1, streams)
+        self._pool = [torch.cuda.Stream(device=self.device) for _ in range(self.nstreams)] if streams > 1 else []
+        self._ev_fork = torch.cuda.Event()
+        self._ev_join = [torch.cuda.Event() for _ in range(self.nstreams)]
         self.used = 0
         self.active = []
         self._dirty = True
@@ -159,25 +208,59 @@ class Server:
         self._dirty = False
 
     def _sbe_step(self):
+        """One batched SBE step of every in-flight wire: one launch sequence per bucket,
+        buckets spread over the stream pool."""
+        import torch
+
         sz = C.sizeof(_lib.WireDesc)
         base = self._d_descs.data_ptr()
-        st = _lib.stream_ptr()
-        for lo, hi in self._ranges:
+        main = torch.cuda.current_stream(self.device)
+        nb = len(self._ranges)
+        ns = min(len(self._pool), nb) if self._pool else 0
+        if ns > 1:
+            self._ev_fork.record(main)
+            for j in range(ns):
+                self._pool[j].wait_event(self._ev_fork)
+        for b, (lo, hi) in enumerate(self._ranges):
             sub = self.active[lo:hi]
             m = [s.qa.model for s in sub]
+            st = _lib.stream_ptr(self._pool[b % ns] if ns > 1 else main)
             _lib.check(_lib.lib.pss_sbe_step(C.c_void_p(base + lo * sz), hi - lo, max(x.Nk for x in m),
                                              max(x.Nr for x in m), NX, st), "pss_sbe_step")
+        if ns > 1:
+            for j in range(ns):
+                self._ev_join[j].record(self._pool[j])
+                main.wait_event(self._ev_join[j])
 
     def tick(self):
         if self._dirty:
             self._rebuild()
         self.max_active = max(self.max_active, len(self.active))
-        for s in self.active:
-            s.half_e()
+        self._each(lambda s: s.half_e(self.graphs))
         self._sbe_step()
-        for s in self.active:
-            s.half_b()
+        self._each(lambda s: s.half_b(self.graphs))
         self.ticks += 1
+
+    def _each(self, fn):
+        """fn(solve) for every in-flight solve, forked over the stream pool and joined
+        back into the current stream."""
+        import torch
+
+        if not self._pool:
+            for s in self.active:
+                fn(s)
+            return
+        main = torch.cuda.current_stream(self.device)
+        self._ev_fork.record(main)
+        ns = min(self.nstreams, len(self.active))
+        for j in range(ns):
+            self._pool[j].wait_event(self._ev_fork)
+        for i, s in enumerate(self.active):
+            with torch.cuda.stream(self._pool[i % ns]):
+                fn(s)
+        for j in range(ns):
+            self._ev_join[j].record(self._pool[j])
+            main.wait_event(self._ev_join[j])
 
     # ---------------------------------------------------------------- loop
     def admit(self, req):
