@@ -97,6 +97,14 @@ def time_steps(prop, steps, warmup):
     return time.perf_counter() - t0
 
 
+# HBM bytes per k_fft2<8,16,0,double> launch (one pass over a 128^3 complex128 field),
+# from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of `bench.py --workloads
+# traffic`, each counter scaled by what it reads for a 512 MiB device copy
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 16471 KiB x 2 + 32768 KiB per pass;
+# profiles/fft_pass_128cube_traffic.md.
+FFT_PASS_TRAFFIC_128 = (16471.0 * 2.0 + 32768.0) * 1024
+
+
 def fft_pass_timing(prop, reps=50):
     """Average duration of one FP64 3-D FFT (3 two-level register/LDS passes) on this grid,
     HIP events on the launch stream; algorithmic bytes = 3 passes x (read + write)."""
@@ -211,6 +219,28 @@ def precision_errors(p64, q64, pmx, qmx):
     return out
 
 
+def traffic_kernels(grid, reps=3):
+    """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs: `reps` plain device
+    copies of 512 MiB (the counters' calibration: known bytes read and written), then
+    `reps` FP64 3-D FFTs of one grid^3 field (3 k_fft2 passes each)."""
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd._grid import GridContext
+    from pulsesuite_amd.typespace import ss
+
+    src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")  # 512 MiB
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    ctx = GridContext(ss(3, grid, grid, grid, 20e-9, 20e-9, 20e-9, 9.1), 1e-17, with_cpml=False, device="cuda")
+    F = torch.ones((grid, grid, grid), dtype=torch.complex128, device="cuda")
+    for _ in range(reps):
+        _lib.check(_lib.lib.pss_fft3d(ctx.gp, _lib.ptr(F), -1, _lib.stream_ptr()), "fft")
+    torch.cuda.synchronize()
+    print(json.dumps({"traffic_kernels": {"copy_bytes": src.numel() * 8, "fft_grid": grid, "reps": reps}}))
+
+
 def cpu_baseline(grid, nk, npml, steps, feedback=False):
     """The FP64 NumPy oracle on the same workload for `steps` steps (bounded sample)."""
     from oracle import pstd as OP
@@ -243,6 +273,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     wl = set(a.workloads.split(","))
+    if "traffic" in wl:  # kernels for the HBM-counter runs: a plain copy of known size, then 3-D FFTs
+        traffic_kernels(a.grid)
+        return
     if "coupled" not in wl:  # partial runs (profiling): no headline value
         line = {"metric": METRIC, "value": None, "note": "coupled workload skipped"}
         if "batched" in wl and a.batched_wires > 0:
@@ -290,11 +323,15 @@ def main():
         line["mixed_rel_err_vs_fp64"] = errs
         line["mixed_within_1e-6"] = all(v <= 1e-6 for v in errs.values())
     if not a.no_kernel_timing:
+        # one pass launch: algorithmic bytes = read + write of every complex128 cell
         ms, b = fft_pass_timing(r["prop"])
-        ach = b / (ms * 1e-3)
-        line["roofline"] = {"bound": "hbm", "kernel": "k_fft2<8,16> x3 (one 3-D FFT)", "achieved": ach / 1e9,
-                            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
-                            "avg_ms": ms}
+        npass = sum(1 for N in r["prop"].ctx.shape if N > 1)
+        pass_ms, pass_bytes = ms / npass, b / npass
+        ach = pass_bytes / (pass_ms * 1e-3)
+        line["roofline"] = {"bound": "hbm", "kernel": f"k_fft2 pass of the {a.grid}^3 complex128 3-D FFT",
+                            "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+                            "traffic": FFT_PASS_TRAFFIC_128 if a.grid == 128 else None,
+                            "algorithmic_bytes": pass_bytes, "avg_us": 1e3 * pass_ms}
     if "batched" in wl and a.batched_wires > 0:
         line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max, eager=a.batched_eager)
     if "serving" in wl and a.serve_requests > 0:
