@@ -125,6 +125,7 @@ typedef struct pss_grid {
   const double *by, *cy, *iky;
   const double *bz, *cz, *ikz;
   const double *twx, *twy, *twz;         /* [N] FFT twiddles exp(-2 pi i m/N), interleaved, N/2 used */
+  const double* tdev;                    /* optional device time: the source time is *tdev + src_par[0] */
 } pss_grid;
 
 /*
@@ -186,6 +187,9 @@ int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, 
  */
 int pss_rk4_stage(int32_t stage, int64_t n, double h, double* y, const double* dydt, const double* dyt,
                   double* dym, double* ytmp, void* stream);
+
+/* x[0] += v on the device (advances a device-side time counter inside captured steps). */
+int pss_add_scalar(double* x, double v, void* stream);
 
 /* Message of the last failing call (host string, valid until the next call). */
 const char* pss_last_error(void);

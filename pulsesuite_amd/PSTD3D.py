@@ -94,7 +94,7 @@ class PSTD3DPropagator:
 
     def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="cpml", calc_j=None,
                  output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None,
-                 precision="fp64", device_time=False):
+                 precision="fp64", device_time=False, overlap=None):
         if boundary_type == "mask":
             raise NotImplementedError("boundary_type='mask' (absorber) is not built on the device; use 'cpml'")
         if boundary_type not in ("cpml", "none"):
@@ -142,6 +142,14 @@ class PSTD3DPropagator:
         if calc_j is not None and hasattr(calc_j, "bind"):
             calc_j.bind(self)
         self._nstep = 0
+        # overlap: the calc_j hook (reads E, writes J) runs on a side stream beside the
+        # B half step (reads E, writes B); both join before the next E update, which
+        # reads J and B.  Same launches and operands as the serial order, so identical
+        # results.  Off with device time (the B half advances the device clock).
+        if overlap is None:
+            overlap = calc_j is not None and getattr(calc_j, "separable", False)
+        self.overlap = bool(overlap) and calc_j is not None and self._tdev is None
+        self._side = None
 
     def _src(self, comp, amp, t):
         if self._tfsf is None:
@@ -156,9 +164,23 @@ class PSTD3DPropagator:
     def step(self, stream=None):
         """One leapfrog step (PSTD3D/PSTD3D.py:523-627)."""
         self.step_e_half(stream)
-        if self.calc_j is not None:
+        if self.calc_j is None:
+            self.step_b_half(stream)
+        elif not self.overlap:
             self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
-        self.step_b_half(stream)
+            self.step_b_half(stream)
+        else:
+            cur = stream if stream is not None else torch.cuda.current_stream(self.device)
+            if self._side is None:
+                self._side = torch.cuda.Stream(device=self.device)  # default priority
+                self._ev_e, self._ev_j = torch.cuda.Event(), torch.cuda.Event()
+            self._ev_e.record(cur)
+            self._side.wait_event(self._ev_e)
+            with torch.cuda.stream(self._side):
+                self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+            self._ev_j.record(self._side)
+            self.step_b_half(stream)
+            cur.wait_event(self._ev_j)
 
     def step_e_half(self, stream=None):
         """E update + CPML-E half step (with the soft source on Ey and the wire's point

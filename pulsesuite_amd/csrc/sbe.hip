@@ -33,7 +33,7 @@ __device__ inline double ddiv(double n, double d) {
 // ---------------------------------------------------------------- workspace
 struct WS {
   int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, sp, pk,
-      total_fp64, total;
+      xp, total_fp64, total;
 };
 
 __host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
@@ -68,6 +68,7 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);  // split-K partial tiles
   w.total_fp64 = o;
   w.sp = o;   o += al(16 * np * np);  // mixed precision only: FP16 operand planes
+  w.xp = o;   o += al(8 * np * np);   // mixed precision only: exchange planes (X~ 24 NP^2, T 6 NP^2 halves)
   w.total = o;
   return w;
 }
@@ -293,6 +294,7 @@ __global__ void __launch_bounds__(256) k_prep(const pss_wire_desc* __restrict__ 
   double n1 = s / 2.0 / d.L;
   n1 = fmin(n1, d.density_max);
   const bool scr = d.flags & PSS_SCREENED;
+  double vmax[3] = {0.0, 0.0, 0.0};
   for (int q = threadIdx.x; q < Nk; q += 256) {
     double e1 = 1.0;
     if (scr) {
@@ -300,10 +302,16 @@ __global__ void __launch_bounds__(256) k_prep(const pss_wire_desc* __restrict__ 
                - d.chih[q] * 2 * log(fabs((d.qh[q] - kPi * n1) / (d.qh[q] + n1)));
     }
     ws[L.e1d + q] = e1;
-    for (int n = 0; n < 3; ++n) ws[L.vc + n * Nk + q] = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
+    for (int n = 0; n < 3; ++n) {
+      const double v = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
+      ws[L.vc + n * Nk + q] = v;
+      if (q > 0) vmax[n] = fmax(vmax[n], fabs(v));
+    }
   }
+  for (int n = 0; n < 3; ++n) vmax[n] = block_max<256>(vmax[n], sh);
   if (threadIdx.x == 0) ws[L.red + 0] = n1;
   if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
+  if (threadIdx.x < 3) ws[L.red + 16 + threadIdx.x] = vmax[threadIdx.x];  // max V(|q|), q != 0
 }
 
 // ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H
@@ -599,7 +607,7 @@ __global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ 
   constexpr int TT = 32 * NB;
   const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
   const pss_wire_desc& d = descs[wire];
-  if (!wire_on(d) || !(d.flags & PSS_EXCITONS)) return;
+  if (!wire_on(d) || !(d.flags & PSS_EXCITONS) || d.precision == PSS_MIXED) return;
   if (m > 0 && !(d.flags & PSS_LF)) return;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
   const int dd0 = blockIdx.x * TT, k10 = blockIdx.y * TT;
@@ -868,11 +876,11 @@ __device__ inline void atomic_max_pos(double* addr, double v) {
   atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void __launch_bounds__(256) k_maxabs(const pss_wire_desc* __restrict__ descs) {
+__global__ void __launch_bounds__(256) k_maxabs(const pss_wire_desc* __restrict__ descs, int m0) {
   const pss_wire_desc& d = descs[blockIdx.z];
   if (!wire_on(d) || d.precision != PSS_MIXED) return;
   const int Nk = d.Nk;
-  const int m = blockIdx.y;
+  const int m = m0 + (int)blockIdx.y;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
@@ -1035,13 +1043,20 @@ __device__ inline void mixed_epilogue(const pss_wire_desc& d, const Bufs& B, con
 }
 
 // ---- contraction: one 64 x 64 output tile per workgroup, 4 waves (32 x 32 each,
-// 2 x 2 blocks of 16 x 16), A (64 x 16) and B (16 x 64) FP16 planes of one k chunk
-// staged in LDS and shared by the waves.  12 v_mfma_f32_16x16x16f16 per block and
-// chunk: re += ar br - ai bi, im += ar bi + ai br, each product hi*hi + hi*lo + lo*hi.
-// FP32 partials are folded into FP64 (times 1/(sa sb)) every 8 chunks and at term
-// ends.  The (term, chunk) list is cut into `ks` contiguous slices (blockIdx.z);
-// with ks == 1 the epilogue runs here, else FP64 partial tiles go to the pk buffer.
-constexpr int kLdk = 20;  // padded k stride of a staged row (halves): conflict-free fragment reads
+// 2 x 2 blocks of 16 x 16).  A stage is 32 k values: the FP16 planes of A (64
+// rows) and B (64 rows) of the next stage are loaded into registers while the
+// current one computes, then staged in LDS and shared by the waves.  12
+// v_mfma_f32_16x16x32_f16 per block and stage: re += ar br - ai bi, im += ar bi +
+// ai br, each product hi*hi + hi*lo + lo*hi.  FP32 partials are folded into FP64
+// (times 1/(sa sb)) every 4 stages (128 k values) and at term ends.  The (term,
+// stage) list is cut into `ks` contiguous slices (blockIdx.z); with ks == 1 the
+// epilogue runs here, else FP64 partial tiles go to the pk buffer.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+constexpr int kKs = 32;        // k values per stage
+constexpr int kLdk = kKs + 8;  // padded stride of a staged row (halves), a 16-byte multiple
+constexpr int kFold = 4;       // stages per FP32 partial
+
+__device__ inline f4 mfma32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
 __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict__ descs, int ks) {
   const int zz = blockIdx.z;
@@ -1062,27 +1077,28 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
 
   double accr[2][2][4], acci[2][2][4];
   f4 fr[2][2], fi[2][2];
+  #pragma unroll
   for (int a = 0; a < 2; ++a)
+    #pragma unroll
     for (int b = 0; b < 2; ++b) {
       fr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
       fi[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      #pragma unroll
       for (int v = 0; v < 4; ++v) { accr[a][b][v] = 0.0; acci[a][b][v] = 0.0; }
     }
   const int nterms = out == 0 ? ((d.flags & PSS_OFFDIAGDPH) ? 6 : 4) : 4;
-  const int nkc = NP / 16;
+  const int nkc = NP / kKs;
   const int nitems = nterms * nkc;
   const int it0 = (int)((int64_t)nitems * slice / ks), it1 = (int)((int64_t)nitems * (slice + 1) / ks);
-  // loader role: thread -> (operand, plane, row) with two 16-byte loads of 8 halves
-  const int lop = threadIdx.x >> 7;         // 0 = A, 1 = B
-  const int lq = (threadIdx.x >> 5) & 3;    // plane
-  const int lr = threadIdx.x & 31;          // row pair base: rows lr and lr + 32
   int cur_t = -1, nacc = 0;
   double cur_f = 0.0;
   bool acj = false, bcj = false;
-  const _Float16 *pa = nullptr, *pb = nullptr;
   auto fold = [&]() {
+    #pragma unroll
     for (int a = 0; a < 2; ++a)
+      #pragma unroll
       for (int b = 0; b < 2; ++b)
+        #pragma unroll
         for (int v = 0; v < 4; ++v) {
           accr[a][b][v] += (double)fr[a][b][v] * cur_f;
           acci[a][b][v] += (double)fi[a][b][v] * cur_f;
@@ -1091,8 +1107,35 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
         }
     nacc = 0;
   };
+  // loader: a stage is 2 operands x 4 planes x 64 rows x 4 sixteen-byte quarters;
+  // thread t loads quarter t & 3 of row t >> 2 of every plane of A and of B, so 4
+  // consecutive threads read one row's 64 contiguous bytes
+  const int lrow = threadIdx.x >> 2, lqt = threadIdx.x & 3;
+  const int64_t PL = (int64_t)NP * NP;
+  const _Float16* spl = (const _Float16*)(ws + L.sp);
+  // named registers, not an array: the prefetched segments must stay in VGPRs
+  uint4 pa0, pa1, pa2, pa3, pb0, pb1, pb2, pb3;
+  auto prefetch = [&](int it) {
+    const int t = it / nkc, k0 = (it - t * nkc) * kKs;
+    const Term T = kTerms[out][t];
+    int ma, oa, mb, ob;
+    bool c;
+    a_side(T.a, T.aop, ma, oa, c);
+    b_side(T.b, T.bop, mb, ob, c);
+    const _Float16* ga = spl + (ma * 2 + oa) * 4 * PL + (int64_t)(m0 + lrow) * NP + k0 + 8 * lqt;
+    const _Float16* gb = spl + (mb * 2 + ob) * 4 * PL + (int64_t)(n0 + lrow) * NP + k0 + 8 * lqt;
+    pa0 = *(const uint4*)ga;
+    pa1 = *(const uint4*)(ga + PL);
+    pa2 = *(const uint4*)(ga + 2 * PL);
+    pa3 = *(const uint4*)(ga + 3 * PL);
+    pb0 = *(const uint4*)gb;
+    pb1 = *(const uint4*)(gb + PL);
+    pb2 = *(const uint4*)(gb + 2 * PL);
+    pb3 = *(const uint4*)(gb + 3 * PL);
+  };
+  if (it0 < it1) prefetch(it0);
   for (int it = it0; it < it1; ++it) {
-    const int t = it / nkc, k0 = (it % nkc) * 16;
+    const int t = it / nkc;
     if (t != cur_t) {
       if (cur_t >= 0) fold();
       const Term T = kTerms[out][t];
@@ -1102,61 +1145,60 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
       const double sa = pow2_scale(mx[ma]), sb = pow2_scale(mx[mb]);
       cur_f = (T.neg ? -1.0 : 1.0) / (sa * sb);
       cur_t = t;
-      pa = (const _Float16*)(ws + L.sp) + (int64_t)((ma * 2 + oa) * 4) * NP * NP;
-      pb = (const _Float16*)(ws + L.sp) + (int64_t)((mb * 2 + ob) * 4) * NP * NP;
     }
-    {  // stage A rows m0.., B rows (= output columns) n0.., 16 k values, 4 planes
-      const _Float16* src = (lop == 0 ? pa : pb) + (int64_t)lq * NP * NP;
-      const int r0 = (lop == 0 ? m0 : n0);
-      for (int h = 0; h < 2; ++h) {
-        const int r = lr + 32 * h;
-        const _Float16* g = src + (int64_t)(r0 + r) * NP + k0;
-        const uint4 v0 = *(const uint4*)g;
-        const uint4 v1 = *(const uint4*)(g + 8);
-        _Float16* dst = lop == 0 ? &As[lq][r][0] : &Bs[lq][r][0];
-        *(uint2*)(dst) = make_uint2(v0.x, v0.y);
-        *(uint2*)(dst + 4) = make_uint2(v0.z, v0.w);
-        *(uint2*)(dst + 8) = make_uint2(v1.x, v1.y);
-        *(uint2*)(dst + 12) = make_uint2(v1.z, v1.w);
-      }
-    }
+    *(uint4*)&As[0][lrow][8 * lqt] = pa0;
+    *(uint4*)&As[1][lrow][8 * lqt] = pa1;
+    *(uint4*)&As[2][lrow][8 * lqt] = pa2;
+    *(uint4*)&As[3][lrow][8 * lqt] = pa3;
+    *(uint4*)&Bs[0][lrow][8 * lqt] = pb0;
+    *(uint4*)&Bs[1][lrow][8 * lqt] = pb1;
+    *(uint4*)&Bs[2][lrow][8 * lqt] = pb2;
+    *(uint4*)&Bs[3][lrow][8 * lqt] = pb3;
     __syncthreads();
+    if (it + 1 < it1) prefetch(it + 1);
     {
-      const int li = lane & 15, kq = (lane >> 4) * 4;
-      h4 a[2][4], bq[2][4];
+      // fragments: lane l holds row l % 16, k values 8 (l / 16) .. + 7 of the stage
+      const int li = lane & 15, kq = (lane >> 4) * 8;
+      h8 a[2][4], bq[2][4];
+      #pragma unroll
       for (int x = 0; x < 2; ++x)
+        #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a[x][q] = *(const h4*)&As[q][wr * 32 + x * 16 + li][kq];
-          bq[x][q] = *(const h4*)&Bs[q][wc * 32 + x * 16 + li][kq];
+          a[x][q] = *(const h8*)&As[q][wr * 32 + x * 16 + li][kq];
+          bq[x][q] = *(const h8*)&Bs[q][wc * 32 + x * 16 + li][kq];
         }
       if (acj)
+        #pragma unroll
         for (int x = 0; x < 2; ++x) { a[x][2] = -a[x][2]; a[x][3] = -a[x][3]; }
       if (bcj)
+        #pragma unroll
         for (int x = 0; x < 2; ++x) { bq[x][2] = -bq[x][2]; bq[x][3] = -bq[x][3]; }
+      #pragma unroll
       for (int x = 0; x < 2; ++x) {
-        const h4 nih = -a[x][2], nil = -a[x][3];
+        const h8 nih = -a[x][2], nil = -a[x][3];
+        #pragma unroll
         for (int y = 0; y < 2; ++y) {
           f4 r = fr[x][y], im = fi[x][y];
           // re: ar br - ai bi
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][0], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][1], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][1], bq[y][0], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nih, bq[y][2], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nih, bq[y][3], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nil, bq[y][2], r, 0, 0, 0);
+          r = mfma32(a[x][0], bq[y][0], r);
+          r = mfma32(a[x][0], bq[y][1], r);
+          r = mfma32(a[x][1], bq[y][0], r);
+          r = mfma32(nih, bq[y][2], r);
+          r = mfma32(nih, bq[y][3], r);
+          r = mfma32(nil, bq[y][2], r);
           // im: ar bi + ai br
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][2], im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][0], bq[y][3], im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][1], bq[y][2], im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][2], bq[y][0], im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][2], bq[y][1], im, 0, 0, 0);
-          im = __builtin_amdgcn_mfma_f32_16x16x16f16(a[x][3], bq[y][0], im, 0, 0, 0);
+          im = mfma32(a[x][0], bq[y][2], im);
+          im = mfma32(a[x][0], bq[y][3], im);
+          im = mfma32(a[x][1], bq[y][2], im);
+          im = mfma32(a[x][2], bq[y][0], im);
+          im = mfma32(a[x][2], bq[y][1], im);
+          im = mfma32(a[x][3], bq[y][0], im);
           fr[x][y] = r;
           fi[x][y] = im;
         }
       }
     }
-    if (++nacc == 8) fold();
+    if (++nacc == kFold) fold();
     __syncthreads();
   }
   if (cur_t >= 0) fold();
@@ -1165,8 +1207,11 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
   const cplx* H = (const cplx*)(ws + L.H);
   const double* gam = ws + L.gam;
   cplx* pk = (cplx*)(ws + L.pk) + ((int64_t)slice * 3 + out) * Nk * Nk;
+  #pragma unroll
   for (int x = 0; x < 2; ++x)
+    #pragma unroll
     for (int y = 0; y < 2; ++y)
+      #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = m0 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
         const int j = n0 + wc * 32 + y * 16 + (lane & 15);
@@ -1174,6 +1219,218 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
         const cplx S = mk(accr[x][y][v], acci[x][y][v]);
         if (ks == 1) mixed_epilogue(d, B, H, gam, out, i, j, S);
         else pk[(int64_t)i * Nk + j] = S;
+      }
+}
+
+// ---- mixed-precision exchange operands.  For each exchange matrix m (p with V_eh;
+// with LF also C with V_ee and D with V_hh) k_xsplit writes the FP16 hi/lo planes
+//   X~ (re hi, re lo, im hi, im lo), [dd][u] = X[u - d][u] * sX, d = dd - (Nk - 1),
+//   T (hi, lo),                       [k1][u] = V(|u - k1|) * sT  (0 at u == k1),
+// zero outside the matrix (dd < 2 NP, u and k1 < NP), sX and sT powers of two from
+// max |X| (red slot 8 + m) and max V (red slot 16 + m).  X~ tiles of 32 dd x 64 u:
+// the parallelogram's 95 source rows are read row by row into LDS, split, and
+// written out along u.  Grid (2 NP / 32, NP / 64 + 1, 3 nwires); the last row of
+// blocks writes T, 32 rows k1 per block.
+__device__ inline _Float16* xt_plane(double* ws, const WS& L, int NP, int m, int q) {
+  return (_Float16*)(ws + L.xp) + (int64_t)(m * 4 + q) * 2 * NP * NP;
+}
+__device__ inline _Float16* tp_plane(double* ws, const WS& L, int NP, int m, int q) {
+  return (_Float16*)(ws + L.xp) + (int64_t)24 * NP * NP + (int64_t)(m * 2 + q) * NP * NP;
+}
+
+__device__ inline bool exch_active(const pss_wire_desc& d, int m) {
+  return wire_on(d) && (d.flags & PSS_EXCITONS) && d.precision == PSS_MIXED && (m == 0 || (d.flags & PSS_LF));
+}
+
+__global__ void __launch_bounds__(256) k_xsplit(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!exch_active(d, m)) return;
+  const int Nk = d.Nk, NP = np64(Nk);
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  if ((int)blockIdx.y == (int)gridDim.y - 1) {  // T planes
+    const int r0 = blockIdx.x * 32;
+    if (r0 >= NP) return;
+    const double sT = pow2_scale(ws[L.red + 16 + m]);
+    const double* vrow = ws + L.vc + (int64_t)m * Nk;
+    _Float16* th = tp_plane(ws, L, NP, m, 0);
+    _Float16* tl = tp_plane(ws, L, NP, m, 1);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int rr = wv; rr < 32; rr += 4) {
+      const int k1 = r0 + rr;
+      for (int u4 = lane * 4; u4 < NP; u4 += 256) {
+        h4 hi, lo;
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int u = u4 + j, q = u - k1;
+          const double t = (q == 0 || u >= Nk || k1 >= Nk) ? 0.0 : vrow[q < 0 ? -q : q] * sT;
+          _Float16 a, b;
+          split16(t, a, b);
+          hi[j] = a;
+          lo[j] = b;
+        }
+        *(h4*)&th[(int64_t)k1 * NP + u4] = hi;
+        *(h4*)&tl[(int64_t)k1 * NP + u4] = lo;
+      }
+    }
+    return;
+  }
+  const int dd0 = blockIdx.x * 32, u0 = blockIdx.y * 64;
+  if (dd0 >= 2 * NP || u0 >= NP) return;
+  const Bufs B = bufs(d);
+  const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : B.D2;
+  const double sX = pow2_scale(ws[L.red + 8 + m]);
+  // source rows r = u - d for dd in [dd0, dd0 + 32), u in [u0, u0 + 64): r = rb + rr, rr < 95
+  const int rb = u0 - (dd0 + 31 - (Nk - 1));
+  __shared__ _Float16 S[4][95][64];
+  for (int e = threadIdx.x; e < 95 * 64; e += 256) {
+    const int rr = e >> 6, uu = e & 63;
+    const int r = rb + rr, u = u0 + uu;
+    cplx v = mk(0, 0);
+    if (r >= 0 && r < Nk && u < Nk) v = X[(int64_t)r * Nk + u];
+    _Float16 h0, h1, h2, h3;
+    split16(v.x * sX, h0, h1);
+    split16(v.y * sX, h2, h3);
+    S[0][rr][uu] = h0;
+    S[1][rr][uu] = h1;
+    S[2][rr][uu] = h2;
+    S[3][rr][uu] = h3;
+  }
+  __syncthreads();
+  // element (dd0 + ddl, u0 + uu) is S[.][uu + 31 - ddl][uu]: zero where r or u leaves the matrix
+  for (int e = threadIdx.x; e < 32 * 16; e += 256) {
+    const int ddl = e >> 4, u4 = (e & 15) * 4;
+    #pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      h4 v;
+      #pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = S[q][u4 + j + 31 - ddl][u4 + j];
+      *(h4*)&xt_plane(ws, L, NP, m, q)[(int64_t)(dd0 + ddl) * NP + u0 + u4] = v;
+    }
+  }
+}
+
+// ---- mixed-precision exchange: Y[k1][dd] = sum_u T[k1][u] X~[u][dd] of k_exch as
+// a GEMM over the k_xsplit planes, 64 x 64 output tiles (k1 x dd), 4 waves of 32 x
+// 32, 32-wide u stages staged in LDS as in k_rhs_mx; re and im each Th Xh + Th Xl +
+// Tl Xh on v_mfma_f32_16x16x32_f16, FP32 partials folded into FP64 every 4 stages.
+// Only the stages that hold the tile's parallelogram u range are visited.  The
+// epilogue adds +-Y to H on the diagonal k2 = k1 - d.  Grid (2 NP / 64, NP / 64, 3 nwires).
+__global__ void __launch_bounds__(256) k_exch_mx(const pss_wire_desc* __restrict__ descs) {
+  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!exch_active(d, m)) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1, NP = np64(Nk);
+  const int dd0 = blockIdx.x * 64, k10 = blockIdx.y * 64;
+  if (dd0 >= nd || k10 >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const double fold_f = 1.0 / (pow2_scale(ws[L.red + 16 + m]) * pow2_scale(ws[L.red + 8 + m]));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv >> 1, wc = wv & 1;
+  __shared__ _Float16 As[2][64][kLdk];
+  __shared__ _Float16 Bs[4][64][kLdk];
+  double accr[2][2][4], acci[2][2][4];
+  f4 fr[2][2], fi[2][2];
+  #pragma unroll
+  for (int a = 0; a < 2; ++a)
+    #pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      fr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      fi[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      #pragma unroll
+      for (int v = 0; v < 4; ++v) { accr[a][b][v] = 0.0; acci[a][b][v] = 0.0; }
+    }
+  int nacc = 0;
+  auto fold = [&]() {
+    #pragma unroll
+    for (int a = 0; a < 2; ++a)
+      #pragma unroll
+      for (int b = 0; b < 2; ++b)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          accr[a][b][v] += (double)fr[a][b][v] * fold_f;
+          acci[a][b][v] += (double)fi[a][b][v] * fold_f;
+          fr[a][b][v] = 0.f;
+          fi[a][b][v] = 0.f;
+        }
+    nacc = 0;
+  };
+  // u range with any nonzero X~[dd][u], dd in the tile: u - d in [0, Nk)
+  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + 63, nd - 1) - (Nk - 1);
+  const int ulo = max(0, dlo), uhi = min(Nk - 1, Nk - 1 + dhi);
+  const int s0 = ulo / kKs, s1 = uhi / kKs + 1;
+  // loader: quarter t & 3 of row t >> 2 of both T planes and all four X~ planes
+  const int lrow = threadIdx.x >> 2, lqt = threadIdx.x & 3;
+  const _Float16* ga = tp_plane(ws, L, NP, m, 0) + (int64_t)(k10 + lrow) * NP + 8 * lqt;
+  const _Float16* gb = xt_plane(ws, L, NP, m, 0) + (int64_t)(dd0 + lrow) * NP + 8 * lqt;
+  const int64_t PL = (int64_t)NP * NP;
+  uint4 pa0, pa1, pb0, pb1, pb2, pb3;  // named registers, not an array
+  auto prefetch = [&](int st) {
+    const int k0 = st * kKs;
+    pa0 = *(const uint4*)(ga + k0);
+    pa1 = *(const uint4*)(ga + PL + k0);
+    pb0 = *(const uint4*)(gb + k0);
+    pb1 = *(const uint4*)(gb + 2 * PL + k0);
+    pb2 = *(const uint4*)(gb + 4 * PL + k0);
+    pb3 = *(const uint4*)(gb + 6 * PL + k0);
+  };
+  if (s0 < s1) prefetch(s0);
+  for (int st = s0; st < s1; ++st) {
+    *(uint4*)&As[0][lrow][8 * lqt] = pa0;
+    *(uint4*)&As[1][lrow][8 * lqt] = pa1;
+    *(uint4*)&Bs[0][lrow][8 * lqt] = pb0;
+    *(uint4*)&Bs[1][lrow][8 * lqt] = pb1;
+    *(uint4*)&Bs[2][lrow][8 * lqt] = pb2;
+    *(uint4*)&Bs[3][lrow][8 * lqt] = pb3;
+    __syncthreads();
+    if (st + 1 < s1) prefetch(st + 1);
+    {
+      const int li = lane & 15, kq = (lane >> 4) * 8;
+      h8 a[2][2], b[2][4];
+      #pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        a[x][0] = *(const h8*)&As[0][wr * 32 + x * 16 + li][kq];
+        a[x][1] = *(const h8*)&As[1][wr * 32 + x * 16 + li][kq];
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) b[x][q] = *(const h8*)&Bs[q][wc * 32 + x * 16 + li][kq];
+      }
+      #pragma unroll
+      for (int x = 0; x < 2; ++x)
+        #pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          f4 r = fr[x][y], im = fi[x][y];
+          r = mfma32(a[x][0], b[y][0], r);
+          r = mfma32(a[x][0], b[y][1], r);
+          r = mfma32(a[x][1], b[y][0], r);
+          im = mfma32(a[x][0], b[y][2], im);
+          im = mfma32(a[x][0], b[y][3], im);
+          im = mfma32(a[x][1], b[y][2], im);
+          fr[x][y] = r;
+          fi[x][y] = im;
+        }
+    }
+    if (++nacc == kFold) fold();
+    __syncthreads();
+  }
+  fold();
+  // result element (x, y, v): row k1 = k10 + 32 wr + 16 x + 4 (lane / 16) + v, column dd0 + 32 wc + 16 y + lane % 16
+  cplx* H = (cplx*)(ws + L.H) + (int64_t)m * Nk * Nk;
+  const double sg = m == 0 ? 1.0 : -1.0;
+  #pragma unroll
+  for (int x = 0; x < 2; ++x)
+    #pragma unroll
+    for (int y = 0; y < 2; ++y)
+      #pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k1 = k10 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const int dd = dd0 + wc * 32 + y * 16 + (lane & 15);
+        if (k1 >= Nk || dd >= nd) continue;
+        const int k2 = k1 - (dd - (Nk - 1));
+        if (k2 < 0 || k2 >= Nk) continue;
+        cplx& h = H[(int64_t)k1 * Nk + k2];
+        h = h + mk(sg * accr[x][y][v], sg * acci[x][y][v]);
       }
 }
 
@@ -1752,11 +2009,17 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   {
     const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
+    const size_t lds = (size_t)max_Nk * sizeof(double);
     if ((int64_t)tn * tm * 3 * nwires >= 512)
-      hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), (size_t)max_Nk * sizeof(double), s, d_descs);
+      hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), lds, s, d_descs);
     else
       hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires), dim3(256),
-                         (size_t)max_Nk * sizeof(double), s, d_descs);
+                         lds, s, d_descs);
+    // mixed wires: max |p|, |C|, |D| -> exchange planes -> exchange GEMM
+    const int np = np64(max_Nk);
+    hipLaunchKernelGGL(k_maxabs, dim3(64, 3, nwires), dim3(256), 0, s, d_descs, 0);
+    hipLaunchKernelGGL(k_xsplit, dim3(np / 16, np / 64 + 1, 3 * nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
   }
   {
     const int np = np64(max_Nk), t64 = np / 64;
@@ -1769,7 +2032,7 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
     }
     ks = std::min(ks, ksplit_max(max_Nk));
     hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
-    hipLaunchKernelGGL(k_maxabs, dim3(64, 8, nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_maxabs, dim3(64, 5, nwires), dim3(256), 0, s, d_descs, 3);  // Heh .. hbar De
     hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
     hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
     if (ks > 1) {

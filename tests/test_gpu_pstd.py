@@ -168,6 +168,43 @@ def test_coupled_step_matches_oracle(grid, nk, steps):
     assert np.linalg.norm(jl) > 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_overlapped_step_is_identical(precision):
+    """The wire step on a side stream beside the B half step gives bit-identical
+    fields, wire state and line current to the serial order (same launches, same
+    operands).  The wire's current is fed back (the point-current feedback grows by
+    ~1.5e16 per second, so 15 steps keep every value finite)."""
+    import torch
+
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.rhoPJ import QWArray
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    N, nk, dx, epsr = 64, 64, 20e-9, 9.1
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    out = []
+    for overlap in (False, True):
+        sp = ss(3, N, N, N, dx, dx, dx, epsr)
+        qa = QWArray(QWParams(L=L_of_nk(nk)), MBFlags(Screened=False), Nw=1, Emax0=0.0, precision=precision)
+        prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), ps(800e-9, 3e8, 5e-15, 4e-15, 0.0), source_type="soft",
+                                boundary_type="cpml", calc_j=qa, npml=8, precision=precision, overlap=overlap)
+        assert prop.overlap == overlap
+        for _ in range(15):
+            prop.step()
+        torch.cuda.synchronize()
+        st = qa.wires.get_state(0)
+        out.append([getattr(prop, n).cpu().numpy() for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz")] +
+                   [st["p2"], st["C2"], st["D2"], qa.jl.cpu().numpy()])
+    assert all(bool(np.isfinite(a).all()) for a in out[0])
+    assert np.linalg.norm(out[0][1]) > 0 and np.linalg.norm(out[0][6]) > 0 and np.linalg.norm(out[0][9]) > 0
+    for i, (a, b) in enumerate(zip(*out)):
+        ok = bool(np.array_equal(a, b))
+        assert ok, f"array {i} differs between serial and overlapped steps"
+
+
 def test_cpml_128_matches_oracle():
     """One CPML E and one CPML B half step on a 128^3 grid (the bench grid) with
     12-cell layers: every FFT kernel path the bench uses, with the i*k premultiply."""
