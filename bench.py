@@ -54,10 +54,12 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--batched-nk-max", type=int, default=1024)
     ap.add_argument("--batched-eager", action="store_true", help="batched run without autotune or graph capture")
-    ap.add_argument("--workloads", default="coupled,batched,serving",
-                    help="comma list of coupled, batched, serving (the JSON line needs coupled)")
-    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--workloads", default="coupled,batched,serving,precision256",
+                    help="comma list of coupled, batched, serving, precision256 (the JSON line needs coupled)")
+    ap.add_argument("--p256-steps", type=int, default=1000, help="timed steps of the 256^3 / Nk=1024 precision form")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0,
+                    help="Poisson arrival rate (1/s) of the latency run; a burst run (all at t=0) follows")
     return ap.parse_args()
 
 
@@ -179,17 +181,51 @@ SERVE = dict(nk_lo=64, nk_hi=512, st_lo=200, st_hi=2000, e_lo=1e6, e_hi=1e8, see
 
 
 def serving(nreq, rate, precision="mixed"):
-    """Continuous-batching run: a seeded Poisson stream of wire + 1-D PSTD requests
-    (1024 cells, 16-cell CPML), admitted by HBM budget; throughput and latency."""
+    """Continuous-batching runs of a seeded stream of wire + 1-D PSTD requests (1024
+    cells, 16-cell CPML), admitted by HBM budget: latency at a Poisson rate below the
+    server's capacity, then the same requests all arriving at once (peak throughput)."""
     from pulsesuite_amd.serve import Server, poisson_requests
 
-    reqs = poisson_requests(nreq, rate, **SERVE)
-    srv = Server(hbm_budget=64 << 30, max_slots=64, precision=precision)
-    srv.run(reqs)
-    out = srv.summary()
-    out.update(requests=nreq, rate_per_s=rate, precision=precision, max_in_flight=srv.max_active,
-               workload=f"Poisson {rate}/s, Nk log-uniform [{SERVE['nk_lo']},{SERVE['nk_hi']}], steps log-uniform "
+    out = {}
+    for tag, r in (("poisson", rate), ("burst", float("inf"))):
+        print(f"[bench] serving {tag}", file=sys.stderr, flush=True)
+        reqs = poisson_requests(nreq, r, **SERVE)
+        srv = Server(hbm_budget=64 << 30, max_slots=64, precision=precision)
+        srv.run(reqs)
+        o = srv.summary()
+        o.update(requests=nreq, max_in_flight=srv.max_active,
+                 arrivals=f"Poisson {r}/s" if tag == "poisson" else "all at t=0")
+        out[tag] = o
+        del srv
+    out.update(precision=precision,
+               workload=f"{nreq} requests, Nk log-uniform [{SERVE['nk_lo']},{SERVE['nk_hi']}], steps log-uniform "
                         f"[{SERVE['st_lo']},{SERVE['st_hi']}], wire + 1-D PSTD 1024 cells npml 16, seed {SERVE['seed']}")
+    return out
+
+
+def precision_form(grid=256, nk=1024, npml=12, steps=100, warmup=2, feedback=False):
+    """The larger precision form: the coupled step on a grid^3 field grid with one
+    Nk-point wire, run in FP64 and in the mixed form for the same steps; throughput
+    of both and the mixed run's relative error per output against the FP64 run."""
+    res = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[bench] precision form {grid}^3 Nk={nk} {prec}", file=sys.stderr, flush=True)
+        prop, qa = setup(grid, nk, npml, prec, feedback)
+        el = time_steps(prop, steps, warmup)
+        res[prec] = dict(prop=prop, qa=qa, value=nk * steps / el, ms=1e3 * el / steps)
+    errs = precision_errors(res["fp64"]["prop"], res["fp64"]["qa"], res["mixed"]["prop"], res["mixed"]["qa"])
+    out = {"workload": f"3-D PSTD {grid}^3 + {npml}-cell CPML + soft source, 1 wire Nk={nk}, "
+                       f"{warmup} + {steps} leapfrog steps",
+           "steps": steps,
+           "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms"]},
+           "mixed": {"value": res["mixed"]["value"], "ms_per_step": res["mixed"]["ms"]},
+           "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+           "mixed_rel_err_vs_fp64": errs,
+           "mixed_within_1e-6": all(v <= 1e-6 for v in errs.values())}
+    del res
+    import torch
+
+    torch.cuda.empty_cache()
     return out
 
 
@@ -282,6 +318,8 @@ def main():
             line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max, eager=a.batched_eager)
         if "serving" in wl and a.serve_requests > 0:
             line["serving"] = serving(a.serve_requests, a.serve_rate)
+        if "precision256" in wl:
+            line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback)
         print(json.dumps(line))
         return
     results = {}
@@ -335,8 +373,9 @@ def main():
     if "batched" in wl and a.batched_wires > 0:
         line["batched"] = batched(a.batched_wires, a.batched_steps, a.batched_nk_max, eager=a.batched_eager)
     if "serving" in wl and a.serve_requests > 0:
-        print("[bench] serving", file=sys.stderr, flush=True)
         line["serving"] = serving(a.serve_requests, a.serve_rate)
+    if "precision256" in wl:
+        line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback)
     if a.cpu_steps > 0:
         v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps, a.feedback)
         line["cpu_baseline"] = {"value": v, "unit": "k-point*steps/s", "cores": len(os.sched_getaffinity(0)),
