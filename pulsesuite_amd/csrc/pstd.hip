@@ -92,6 +92,8 @@ struct PassArgs {
   int Nx, Ny, Nz, axis, mul_axis, dir;  // dir -1 forward, +1 inverse
   double scale;
   int accum;  // 0: out = result, 1: out += result (F_k += FFT(dF)), 2: real part to ((double*)out)[i]
+  int slab;   // > 0 (two-level passes only): write only positions k < slab or k >= N - slab along the
+              // axis, into an array whose axis length is 2 slab (k -> k, or k - (N - 2 slab))
 };
 
 template <int N, int MODE>
@@ -246,7 +248,7 @@ __device__ __forceinline__ void store_out(const PassArgs& a, int64_t o, cpx<T> v
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
 // per element instead of log2(N) staged radix-2 sweeps.
-template <int N1, int N2, int MODE, class T>
+template <int N1, int N2, int MODE, class T, bool SLAB = false>
 __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
@@ -258,6 +260,8 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   int64_t base0, sa, st;
   int nvalid;
   int cx = 0, cy = 0, cz0 = 0;
+  const int P2 = SLAB ? 2 * a.slab : 0;
+  int64_t obase0, ost, osa;  // output geometry (differs from the input's only for slab output)
   if (a.axis == 2) {
     const int64_t nlines = (int64_t)Nx * Ny;
     const int64_t l0 = (int64_t)blockIdx.x * TB;
@@ -268,6 +272,9 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
     st = N;
     cx = (int)(l0 / Ny);
     cy = (int)(l0 % Ny);
+    obase0 = SLAB ? l0 * P2 : base0;
+    ost = SLAB ? P2 : st;
+    osa = 1;
   } else {
     const int nch = (Nz + TB - 1) / TB;
     const int other = (int)(blockIdx.x / nch);
@@ -277,14 +284,18 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
       cx = other;
       base0 = (int64_t)other * Ny * Nz + cz0;
       sa = Nz;
+      obase0 = SLAB ? (int64_t)other * P2 * Nz + cz0 : base0;
     } else {
       if (other >= Ny) return;
       cy = other;
       base0 = (int64_t)other * Nz + cz0;
       sa = (int64_t)Ny * Nz;
+      obase0 = base0;
     }
     st = 1;
     nvalid = min(TB, Nz - cz0);
+    ost = st;
+    osa = sa;
   }
   for (int m = threadIdx.x; m < N / 2; m += TB * G) {
     const double wr = a.tw[2 * m], wi = a.tw[2 * m + 1];
@@ -343,7 +354,12 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
         const int k = k1 + N1 * k2;
-        const int64_t o = base0 + (int64_t)t * st + (int64_t)k * sa;
+        int kk = k;
+        if (SLAB) {
+          if (k >= a.slab && k < N - a.slab) continue;
+          kk = k < a.slab ? k : k - (N - P2);
+        }
+        const int64_t o = obase0 + (int64_t)t * ost + (int64_t)kk * osa;
         store_out<MODE, T>(a, o, u[k2] * (T)a.scale);
       }
     }
@@ -360,6 +376,16 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
   const dim3 gr((unsigned)blocks), bl(TB * G);
+  if (a.slab) {  // first pass of a slab derivative: store (complex) or real part
+    if (a.out32) {
+      if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float, true>), gr, bl, 0, s, a);
+      else hipLaunchKernelGGL((k_fft2<N1, N2, 0, float, true>), gr, bl, 0, s, a);
+    } else {
+      if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, double, true>), gr, bl, 0, s, a);
+      else hipLaunchKernelGGL((k_fft2<N1, N2, 0, double, true>), gr, bl, 0, s, a);
+    }
+    return;
+  }
   if (a.in32 || a.out32) {  // FP32 arithmetic whenever either side is single precision
     if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, float>), gr, bl, 0, s, a);
     else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float>), gr, bl, 0, s, a);
@@ -370,7 +396,7 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
 }
 
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
-                hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0) {
+                hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0, int slab = 0) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
   if (N == 1) {
     if (in != out) return -1;
@@ -389,6 +415,7 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   a.accum = accum;
   a.in32 = in32;
   a.out32 = out32;
+  a.slab = slab;
   switch (N) {
     case 64: launch_fft2<8, 8>(a, g, s); return 0;
     case 128: launch_fft2<8, 16>(a, g, s); return 0;
@@ -397,7 +424,7 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
     case 1024: launch_fft2<32, 32>(a, g, s); return 0;
     default: break;
   }
-  if (in32 || out32) return -3;  // single-precision passes exist for N = 64 ... 1024 only
+  if (in32 || out32 || slab) return -3;  // single-precision and slab passes exist for N = 64 ... 1024 only
   const int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   int64_t blocks;
   if (axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
@@ -502,6 +529,67 @@ int deriv_pair(const pss_grid* g, const cplx* G, int c, int a1, int a2, void* R1
   return 0;
 }
 
+// The CPML correction reads dG/da only inside the PML slabs along a: outside them
+// the profile gives sigma = 0 and kappa = 1, so c_a = 0 and 1/kappa_a - 1 = 0 multiply
+// it (PSTD3D/cpml.py:109-140,455-488).  R_q = Re IFFT(i k_a G) is therefore computed
+// only there: the pass along a (with the factor i k_a) runs first and writes only the
+// 2 P_a slab planes, into an array whose a-axis is 2 P_a long; the passes along the
+// other axes run on that array (2 P_a / N_a of the grid), the last writing the real
+// part.  Derivative q: 0 dGz/dy, 1 dGy/dz, 2 dGx/dz, 3 dGz/dx, 4 dGy/dx, 5 dGx/dy.
+struct SlabR {
+  int64_t off[6];  // element offset of derivative q in R
+  int on[6];       // q computed (else it is never read: axis of length 1 or no PML along a)
+  int full;        // 1: full-grid arrays R[q n + o] (fallback path), 0: slab arrays
+};
+
+constexpr int kDerivField[6] = {2, 1, 0, 2, 1, 0};
+constexpr int kDerivAxis[6] = {1, 2, 2, 0, 0, 1};
+
+int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int fp32, SlabR& sr, hipStream_t s) {
+  const int len[3] = {g->Nx, g->Ny, g->Nz};
+  const int pml[3] = {g->npml_x, g->npml_y, g->npml_z};
+  const double* ks[3] = {g->kx, g->ky, g->kz};
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  int64_t off = 0;
+  sr.full = 0;
+  for (int q = 0; q < 6; ++q) {
+    const int a = kDerivAxis[q];
+    sr.on[q] = 0;
+    sr.off[q] = 0;
+    const int P = pml[a];
+    if (len[a] == 1 || P == 0) continue;
+    sr.on[q] = 1;
+    sr.off[q] = off;
+    off += n / len[a] * 2 * P;
+    void* Rq = fp32 ? (void*)((float*)R + sr.off[q]) : (void*)((double*)R + sr.off[q]);
+    pss_grid gc = *g;  // the slab array's geometry
+    if (a == 0) gc.Nx = 2 * P;
+    else if (a == 1) gc.Ny = 2 * P;
+    else gc.Nz = 2 * P;
+    int order[3], np = 0;
+    order[np++] = a;
+    for (int b = 0; b < 3; ++b)
+      if (b != a && len[b] > 1) order[np++] = b;
+    for (int i = 0; i < np; ++i) {
+      const bool last = i == np - 1;
+      cplx* out = last ? (cplx*)Rq : S;
+      int rc;
+      if (i == 0)
+        rc = launch_pass(g, G[kDerivField[q]], out, a, +1, ks[a], a, s, last ? 2 : 0, 0, fp32, P);
+      else
+        rc = launch_pass(&gc, S, out, order[i], +1, nullptr, -1, s, last ? 2 : 0, fp32, fp32);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int64_t slab_index(int a, int i, int j, int k, int Nx, int Ny, int Nz, int P) {
+  if (a == 0) return ((int64_t)(i < P ? i : i - (Nx - 2 * P)) * Ny + j) * Nz + k;
+  if (a == 1) return ((int64_t)i * (2 * P) + (j < P ? j : j - (Ny - 2 * P))) * Nz + k;
+  return ((int64_t)i * Ny + j) * (2 * P) + (k < P ? k : k - (Nz - 2 * P));
+}
+
 // ------------------------------------------------------------ CPML
 // Real-space correction of PSTD3D/cpml.py:150-334 (psi only in PML cells) and the
 // soft-source add of PSTD3D/tfsf.py:106-136 on component src_comp, written as a
@@ -514,7 +602,7 @@ struct SrcPar {
 };
 
 template <class TR, class TF>
-__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* __restrict__ R,
+__global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* __restrict__ R, SlabR sr,
                                               TF* __restrict__ dFx, TF* __restrict__ dFy, TF* __restrict__ dFz,
                                               double* psi, int src_comp, SrcPar sp,
                                               const double* __restrict__ srcx, const double* __restrict__ rx,
@@ -531,9 +619,19 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
   const bool inz = k < g.npml_z || k >= g.Nz - g.npml_z;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (inx || iny || inz) {
-    // R: [6][n] real parts of d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy
-    const double gzy = (double)R[o], gyz = (double)R[n + o], gxz = (double)R[2 * n + o],
-                 gzx = (double)R[3 * n + o], gyx = (double)R[4 * n + o], gxy = (double)R[5 * n + o];
+    // real parts of d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy;
+    // outside the slab of its axis a derivative is multiplied by zero coefficients only
+    const bool in_a[3] = {inx, iny, inz};
+    const int P[3] = {g.npml_x, g.npml_y, g.npml_z};
+    double d[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int a = kDerivAxis[q];
+      d[q] = 0.0;
+      if (sr.full) d[q] = (double)R[q * n + o];
+      else if (sr.on[q] && in_a[a]) d[q] = (double)R[sr.off[q] + slab_index(a, i, j, k, g.Nx, g.Ny, g.Nz, P[a])];
+    }
+    const double gzy = d[0], gyz = d[1], gxz = d[2], gzx = d[3], gyx = d[4], gxy = d[5];
     double* pxy = psi;
     double* pxz = psi + n;
     double* pyx = psi + 2 * n;
@@ -763,11 +861,28 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   double* R = (double*)scratch;
   float* Rf = (float*)scratch;
   const cplx *gx = (const cplx*)Gx, *gy = (const cplx*)Gy, *gz = (const cplx*)Gz;
-  // R: 0 = dGz/dy, 1 = dGy/dz, 2 = dGx/dz, 3 = dGz/dx, 4 = dGy/dx, 5 = dGx/dy
-  auto rq = [&](int q) -> void* { return fp32 ? (void*)(Rf + q * n) : (void*)(R + q * n); };
-  int rc = deriv_pair(g, gz, 2, 1, 0, rq(0), rq(3), sc + 3 * n, sc + 4 * n, fp32, s);
-  if (!rc) rc = deriv_pair(g, gy, 1, 2, 0, rq(1), rq(4), sc + 3 * n, sc + 4 * n, fp32, s);
-  if (!rc) rc = deriv_pair(g, gx, 0, 2, 1, rq(2), rq(5), sc + 3 * n, sc + 4 * n, fp32, s);
+  // derivatives on the PML slabs when every axis has a two-level pass (64 ... 1024),
+  // else full-grid derivative pairs.  R: 0 = dGz/dy, 1 = dGy/dz, 2 = dGx/dz, 3 = dGz/dx,
+  // 4 = dGy/dx, 5 = dGx/dy
+  bool slab = true;
+  for (int N : {g->Nx, g->Ny, g->Nz})
+    if (N > 1 && (N < 64 || N > 1024)) slab = false;
+  SlabR sr{};
+  int rc = 0;
+  if (slab) {
+    const cplx* G3[3] = {gx, gy, gz};
+    rc = slab_derivs(g, G3, scratch, sc + 3 * n, fp32, sr, s);
+  } else {
+    sr.full = 1;
+    for (int q = 0; q < 6; ++q) {
+      sr.off[q] = q * n;
+      sr.on[q] = 1;
+    }
+    auto rq = [&](int q) -> void* { return fp32 ? (void*)(Rf + q * n) : (void*)(R + q * n); };
+    rc = deriv_pair(g, gz, 2, 1, 0, rq(0), rq(3), sc + 3 * n, sc + 4 * n, fp32, s);
+    if (!rc) rc = deriv_pair(g, gy, 1, 2, 0, rq(1), rq(4), sc + 3 * n, sc + 4 * n, fp32, s);
+    if (!rc) rc = deriv_pair(g, gx, 0, 2, 1, rq(2), rq(5), sc + 3 * n, sc + 4 * n, fp32, s);
+  }
   if (rc) return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
   SrcPar sp{};
   if (src_comp >= 0) {
@@ -778,10 +893,10 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   cflt* dFf = (cflt*)dF;       // complex64 dF (mixed: transformed in FP32, added to the FP64 fields)
   if (fp32)
     hipLaunchKernelGGL((k_cpml<float, cflt>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, Rf,
-                       dFf, dFf + n, dFf + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+                       sr, dFf, dFf + n, dFf + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   else
     hipLaunchKernelGGL((k_cpml<double, cplx>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R,
-                       dF, dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+                       sr, dF, dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
   cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
   if (sample_idx) {
     // partial sums live in the (now free) derivative slots
