@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="coupled steps as eager launches instead of graph replays")
     ap.add_argument("--feedback", action="store_true", help="keep the wire's P/rho in the current (diverges)")
     ap.add_argument("--batched-wires", type=int, default=64, help="wires of the batched run (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
@@ -66,7 +67,7 @@ def parse():
 AMP = 1e7  # soft-source amplitude (V/m)
 
 
-def setup(grid, nk, npml, precision, feedback=False):
+def setup(grid, nk, npml, precision, feedback=False, device_time=True):
     from pulsesuite_amd.PSTD3D import PSTD3DPropagator
     from pulsesuite_amd.rhoPJ import QWArray
     from pulsesuite_amd.typepulse import ps
@@ -82,19 +83,26 @@ def setup(grid, nk, npml, precision, feedback=False):
     mb = MBFlags(Screened=False, ignorewire=not feedback)  # bare Coulomb for this config
     qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, precision=precision)
     prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type="cpml",
-                            calc_j=qa, npml=npml, check_every=0, precision=precision)
+                            calc_j=qa, npml=npml, check_every=0, precision=precision, device_time=device_time)
     return prop, qa
 
 
-def time_steps(prop, steps, warmup):
+def time_steps(prop, steps, warmup, graph=True):
+    """Wall time of `steps` steps after `warmup` eager ones: replays of the step
+    recorded as a HIP graph (default), or eager launches."""
     import torch
 
     for _ in range(warmup):
         prop.step()
+    if graph:
+        prop.capture_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        prop.step()
+    if graph:
+        prop.run_graph(steps)
+    else:
+        for _ in range(steps):
+            prop.step()
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -203,7 +211,7 @@ def serving(nreq, rate, precision="mixed"):
     return out
 
 
-def precision_form(grid=256, nk=1024, npml=12, steps=100, warmup=2, feedback=False):
+def precision_form(grid=256, nk=1024, npml=12, steps=100, warmup=2, feedback=False, graph=True):
     """The larger precision form: the coupled step on a grid^3 field grid with one
     Nk-point wire, run in FP64 and in the mixed form for the same steps; throughput
     of both and the mixed run's relative error per output against the FP64 run."""
@@ -211,7 +219,7 @@ def precision_form(grid=256, nk=1024, npml=12, steps=100, warmup=2, feedback=Fal
     for prec in ("fp64", "mixed"):
         print(f"[bench] precision form {grid}^3 Nk={nk} {prec}", file=sys.stderr, flush=True)
         prop, qa = setup(grid, nk, npml, prec, feedback)
-        el = time_steps(prop, steps, warmup)
+        el = time_steps(prop, steps, warmup, graph=graph)
         res[prec] = dict(prop=prop, qa=qa, value=nk * steps / el, ms=1e3 * el / steps)
     errs = precision_errors(res["fp64"]["prop"], res["fp64"]["qa"], res["mixed"]["prop"], res["mixed"]["qa"])
     out = {"workload": f"3-D PSTD {grid}^3 + {npml}-cell CPML + soft source, 1 wire Nk={nk}, "
@@ -319,7 +327,7 @@ def main():
         if "serving" in wl and a.serve_requests > 0:
             line["serving"] = serving(a.serve_requests, a.serve_rate)
         if "precision256" in wl:
-            line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback)
+            line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback, graph=not a.eager)
         print(json.dumps(line))
         return
     results = {}
@@ -327,7 +335,7 @@ def main():
     for prec in precs:
         print(f"[bench] coupled step {prec}", file=sys.stderr, flush=True)
         prop, qa = setup(a.grid, a.nk, a.npml, prec, a.feedback)
-        el = time_steps(prop, a.steps, a.warmup)
+        el = time_steps(prop, a.steps, a.warmup, graph=not a.eager)
         results[prec] = dict(seconds=el, value=a.nk * a.steps / el, ms=1e3 * el / a.steps, prop=prop, qa=qa)
     main_prec = precs[-1]
     r = results[main_prec]
@@ -352,7 +360,8 @@ def main():
         "data": "synthetic (seeded soft-source pulse, reference initial wire state)",
         "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
                                f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,
-                   "wires": 1, "precision": main_prec},
+                   "wires": 1, "precision": main_prec,
+                   "launch": "eager" if a.eager else "HIP graph replay of the step (wire step on a second stream)"},
     }
     if errs is not None:
         line["fp64_value"] = results["fp64"]["value"]
@@ -375,7 +384,7 @@ def main():
     if "serving" in wl and a.serve_requests > 0:
         line["serving"] = serving(a.serve_requests, a.serve_rate)
     if "precision256" in wl:
-        line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback)
+        line["precision_form_256"] = precision_form(steps=a.p256_steps, feedback=a.feedback, graph=not a.eager)
     if a.cpu_steps > 0:
         v, el = cpu_baseline(a.grid, a.nk, a.npml, a.cpu_steps, a.feedback)
         line["cpu_baseline"] = {"value": v, "unit": "k-point*steps/s", "cores": len(os.sched_getaffinity(0)),

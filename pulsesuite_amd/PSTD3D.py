@@ -145,11 +145,12 @@ class PSTD3DPropagator:
         # overlap: the calc_j hook (reads E, writes J) runs on a side stream beside the
         # B half step (reads E, writes B); both join before the next E update, which
         # reads J and B.  Same launches and operands as the serial order, so identical
-        # results.  Off with device time (the B half advances the device clock).
+        # results.  The hook never reads the device clock, so this holds with device time.
         if overlap is None:
             overlap = calc_j is not None and getattr(calc_j, "separable", False)
-        self.overlap = bool(overlap) and calc_j is not None and self._tdev is None
+        self.overlap = bool(overlap) and calc_j is not None
         self._side = None
+        self._graph = None
 
     def _src(self, comp, amp, t):
         if self._tfsf is None:
@@ -171,9 +172,7 @@ class PSTD3DPropagator:
             self.step_b_half(stream)
         else:
             cur = stream if stream is not None else torch.cuda.current_stream(self.device)
-            if self._side is None:
-                self._side = torch.cuda.Stream(device=self.device)  # default priority
-                self._ev_e, self._ev_j = torch.cuda.Event(), torch.cuda.Event()
+            self._ensure_side()
             self._ev_e.record(cur)
             self._side.wait_event(self._ev_e)
             with torch.cuda.stream(self._side):
@@ -181,6 +180,50 @@ class PSTD3DPropagator:
             self._ev_j.record(self._side)
             self.step_b_half(stream)
             cur.wait_event(self._ev_j)
+
+    def _ensure_side(self):
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)  # default priority
+            self._ev_e, self._ev_j = torch.cuda.Event(), torch.cuda.Event()
+
+    def capture_step(self):
+        """Record one step (both streams when overlapped) as a HIP graph.  Needs
+        device_time=True: the soft source reads the time from the device counter that
+        the recorded step advances, so every replay is the next step.  The recording
+        executes nothing; the host bookkeeping of the recorded launches is rolled back."""
+        if self._tdev is None:
+            raise RuntimeError("capture_step needs a propagator built with device_time=True")
+        if self.overlap:
+            self._ensure_side()
+        tm = self.time
+        saved = (tm.t, tm.n, self._nstep, self.check_every)
+        self.check_every = 0
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        try:
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self.step()
+        finally:
+            torch.cuda.current_stream(self.device).wait_stream(s)
+            tm.t, tm.n, self._nstep, self.check_every = saved
+        self._graph = g
+        return g
+
+    def run_graph(self, nsteps):
+        """Advance nsteps by replaying the recorded step (recorded on first use)."""
+        if self._graph is None:
+            self.capture_step()
+        tm = self.time
+        for _ in range(nsteps):
+            self._graph.replay()
+            tm.t += tm.dt / 2.0  # the two half steps, as step() advances the host clock
+            tm.t += tm.dt / 2.0
+            tm.n += 1
+            self._nstep += 1
+            if self.check_every and self._nstep % self.check_every == 0:
+                self.check_finite()
 
     def step_e_half(self, stream=None):
         """E update + CPML-E half step (with the soft source on Ey and the wire's point

@@ -84,8 +84,10 @@ struct cflt {
 };
 
 struct PassArgs {
-  const cplx* in;
+  const cplx* in;   // batch element 0 (blockIdx.y = 0)
   cplx* out;
+  const cplx *in1, *in2;  // two-level passes: batch elements 1, 2 (same geometry, blockIdx.y = 1, 2)
+  cplx *out1, *out2;
   int in32, out32;      // element types of in / out: 0 complex128 (real double), 1 complex64 (real float)
   const double* tw;     // exp(-2 pi i m/N), m < N/2, interleaved
   const double* kmul;   // wavenumbers along mul_axis (null = none)
@@ -221,26 +223,26 @@ __device__ __forceinline__ void dft_reg(cpx<T>* v, const cpx<T>* tw) {
 }
 
 template <class T>
-__device__ __forceinline__ cpx<T> load_in(const PassArgs& a, int64_t o) {
+__device__ __forceinline__ cpx<T> load_in(const PassArgs& a, const cplx* in, int64_t o) {
   if (a.in32) {
-    const cflt v = ((const cflt*)a.in)[o];
+    const cflt v = ((const cflt*)in)[o];
     return {(T)v.x, (T)v.y};
   }
-  const cplx v = a.in[o];
+  const cplx v = in[o];
   return {(T)v.x, (T)v.y};
 }
 
 // MODE 0: out = v, MODE 1: out += v (complex128), MODE 2: out = Re v (real array)
 template <int MODE, class T>
-__device__ __forceinline__ void store_out(const PassArgs& a, int64_t o, cpx<T> v) {
+__device__ __forceinline__ void store_out(const PassArgs& a, cplx* out, int64_t o, cpx<T> v) {
   if (MODE == 1) {
-    a.out[o] = a.out[o] + mk((double)v.x, (double)v.y);
+    out[o] = out[o] + mk((double)v.x, (double)v.y);
   } else if (MODE == 2) {
-    if (a.out32) ((float*)a.out)[o] = (float)v.x;
-    else ((double*)a.out)[o] = (double)v.x;
+    if (a.out32) ((float*)out)[o] = (float)v.x;
+    else ((double*)out)[o] = (double)v.x;
   } else {
-    if (a.out32) ((cflt*)a.out)[o] = cflt{(float)v.x, (float)v.y};
-    else a.out[o] = mk((double)v.x, (double)v.y);
+    if (a.out32) ((cflt*)out)[o] = cflt{(float)v.x, (float)v.y};
+    else out[o] = mk((double)v.x, (double)v.y);
   }
 }
 
@@ -260,6 +262,9 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   int64_t base0, sa, st;
   int nvalid;
   int cx = 0, cy = 0, cz0 = 0;
+  const int bz = blockIdx.y;  // batch element: a uniform choice among scalar arguments
+  const cplx* in = bz == 0 ? a.in : bz == 1 ? a.in1 : a.in2;
+  cplx* out = bz == 0 ? a.out : bz == 1 ? a.out1 : a.out2;
   const int P2 = SLAB ? 2 * a.slab : 0;
   int64_t obase0, ost, osa;  // output geometry (differs from the input's only for slab output)
   if (a.axis == 2) {
@@ -325,7 +330,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
       cpx<T> x = {(T)0, (T)0};
       if (t < nvalid) {
         const int idx = n1 * N2 + n2;
-        x = load_in<T>(a, base0 + (int64_t)t * st + (int64_t)idx * sa);
+        x = load_in<T>(a, in, base0 + (int64_t)t * st + (int64_t)idx * sa);
         if (a.kmul) {
           const T k = (T)(kconst ? kf : a.kmul[idx]);
           x = cpx<T>{-k * x.y, k * x.x};  // i k x
@@ -360,14 +365,14 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
           kk = k < a.slab ? k : k - (N - P2);
         }
         const int64_t o = obase0 + (int64_t)t * ost + (int64_t)kk * osa;
-        store_out<MODE, T>(a, o, u[k2] * (T)a.scale);
+        store_out<MODE, T>(a, out, o, u[k2] * (T)a.scale);
       }
     }
   }
 }
 
 template <int N1, int N2>
-void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
+void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s, int nb) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
   constexpr int TB0 = 256 / G, TBL = 2048 / N;
@@ -375,7 +380,7 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
-  const dim3 gr((unsigned)blocks), bl(TB * G);
+  const dim3 gr((unsigned)blocks, (unsigned)nb), bl(TB * G);
   if (a.slab) {  // first pass of a slab derivative: store (complex) or real part
     if (a.out32) {
       if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float, true>), gr, bl, 0, s, a);
@@ -395,16 +400,41 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s) {
   else hipLaunchKernelGGL((k_fft2<N1, N2, 0, double>), gr, bl, 0, s, a);
 }
 
+// One pass along `axis` over nb <= 3 arrays of the same geometry (ins[b] -> outs[b]):
+// one launch for the two-level sizes, one launch per array otherwise.
+int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
+                  const double* kmul, int mul_axis, hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0,
+                  int slab = 0);
+
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
                 hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0, int slab = 0) {
+  return launch_pass_b(g, &in, &out, 1, axis, dir, kmul, mul_axis, s, accum, in32, out32, slab);
+}
+
+int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
+                  const double* kmul, int mul_axis, hipStream_t s, int accum, int in32, int out32, int slab) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
+  if (nb < 1 || nb > 3) return -1;
   if (N == 1) {
-    if (in != out) return -1;
+    for (int b = 0; b < nb; ++b)
+      if (ins[b] != outs[b]) return -1;
+    return 0;
+  }
+  const bool two_level = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  if (nb > 1 && !two_level) {
+    for (int b = 0; b < nb; ++b) {
+      const int rc = launch_pass_b(g, ins + b, outs + b, 1, axis, dir, kmul, mul_axis, s, accum, in32, out32, slab);
+      if (rc) return rc;
+    }
     return 0;
   }
   PassArgs a;
-  a.in = in;
-  a.out = out;
+  a.in = ins[0];
+  a.out = outs[0];
+  a.in1 = nb > 1 ? ins[1] : nullptr;
+  a.out1 = nb > 1 ? outs[1] : nullptr;
+  a.in2 = nb > 2 ? ins[2] : nullptr;
+  a.out2 = nb > 2 ? outs[2] : nullptr;
   a.tw = axis == 0 ? g->twx : axis == 1 ? g->twy : g->twz;
   a.kmul = kmul;
   a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
@@ -417,11 +447,11 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
   a.out32 = out32;
   a.slab = slab;
   switch (N) {
-    case 64: launch_fft2<8, 8>(a, g, s); return 0;
-    case 128: launch_fft2<8, 16>(a, g, s); return 0;
-    case 256: launch_fft2<16, 16>(a, g, s); return 0;
-    case 512: launch_fft2<16, 32>(a, g, s); return 0;
-    case 1024: launch_fft2<32, 32>(a, g, s); return 0;
+    case 64: launch_fft2<8, 8>(a, g, s, nb); return 0;
+    case 128: launch_fft2<8, 16>(a, g, s, nb); return 0;
+    case 256: launch_fft2<16, 16>(a, g, s, nb); return 0;
+    case 512: launch_fft2<16, 32>(a, g, s, nb); return 0;
+    case 1024: launch_fft2<32, 32>(a, g, s, nb); return 0;
     default: break;
   }
   if (in32 || out32 || slab) return -3;  // single-precision and slab passes exist for N = 64 ... 1024 only
@@ -464,7 +494,9 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
 // out += FFT(tmp) along every axis of length > 1; tmp is overwritten (used in place
 // for all passes but the last, which accumulates into out).  tmp32: tmp holds
 // complex64 and the passes compute in FP32 (out stays complex128).
-int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s, int tmp32 = 0) {
+// nb arrays at once (tmps[b] -> outs[b]), one launch per axis for the two-level sizes.
+int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, int dir, hipStream_t s,
+             int tmp32 = 0) {
   int last = -1;
   for (int axis = 0; axis < 3; ++axis)
     if ((axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz) > 1) last = axis;
@@ -473,8 +505,9 @@ int fft3_acc(const pss_grid* g, cplx* tmp, cplx* out, int dir, hipStream_t s, in
     const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
     if (N == 1) continue;
     const bool fin = axis == last;
-    const int rc = launch_pass(g, tmp, fin ? out : tmp, axis, dir, nullptr, -1, s, fin ? 1 : 0, tmp32,
-                               fin ? 0 : tmp32);
+    const cplx* ins[3] = {tmps[0], nb > 1 ? tmps[1] : nullptr, nb > 2 ? tmps[2] : nullptr};
+    const int rc = launch_pass_b(g, ins, fin ? outs : tmps, nb, axis, dir, nullptr, -1, s, fin ? 1 : 0, tmp32,
+                                 fin ? 0 : tmp32);
     if (rc) return rc;
   }
   return 0;
@@ -545,6 +578,8 @@ struct SlabR {
 constexpr int kDerivField[6] = {2, 1, 0, 2, 1, 0};
 constexpr int kDerivAxis[6] = {1, 2, 2, 0, 0, 1};
 
+// The two derivatives along one axis (of two fields) share every launch: batch of 2.
+// S holds two slab arrays of at most n / 2 complex each (2 P < N).
 int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int fp32, SlabR& sr, hipStream_t s) {
   const int len[3] = {g->Nx, g->Ny, g->Nz};
   const int pml[3] = {g->npml_x, g->npml_y, g->npml_z};
@@ -556,12 +591,25 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
     const int a = kDerivAxis[q];
     sr.on[q] = 0;
     sr.off[q] = 0;
-    const int P = pml[a];
-    if (len[a] == 1 || P == 0) continue;
+    if (len[a] == 1 || pml[a] == 0) continue;
     sr.on[q] = 1;
     sr.off[q] = off;
-    off += n / len[a] * 2 * P;
-    void* Rq = fp32 ? (void*)((float*)R + sr.off[q]) : (void*)((double*)R + sr.off[q]);
+    off += n / len[a] * 2 * pml[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    const int P = pml[a];
+    if (len[a] == 1 || P == 0) continue;
+    int qs[2], nq = 0;
+    for (int q = 0; q < 6; ++q)
+      if (kDerivAxis[q] == a) qs[nq++] = q;
+    const int64_t m = n / len[a] * 2 * P;  // elements of one slab array
+    cplx* Sq[2] = {S, S + m};
+    cplx* Rq[2];
+    const cplx* Gq[2];
+    for (int h = 0; h < 2; ++h) {
+      Rq[h] = fp32 ? (cplx*)((float*)R + sr.off[qs[h]]) : (cplx*)((double*)R + sr.off[qs[h]]);
+      Gq[h] = G[kDerivField[qs[h]]];
+    }
     pss_grid gc = *g;  // the slab array's geometry
     if (a == 0) gc.Nx = 2 * P;
     else if (a == 1) gc.Ny = 2 * P;
@@ -572,12 +620,14 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
       if (b != a && len[b] > 1) order[np++] = b;
     for (int i = 0; i < np; ++i) {
       const bool last = i == np - 1;
-      cplx* out = last ? (cplx*)Rq : S;
+      cplx* const* outs = last ? Rq : Sq;
       int rc;
       if (i == 0)
-        rc = launch_pass(g, G[kDerivField[q]], out, a, +1, ks[a], a, s, last ? 2 : 0, 0, fp32, P);
-      else
-        rc = launch_pass(&gc, S, out, order[i], +1, nullptr, -1, s, last ? 2 : 0, fp32, fp32);
+        rc = launch_pass_b(g, Gq, outs, 2, a, +1, ks[a], a, s, last ? 2 : 0, 0, fp32, P);
+      else {
+        const cplx* ins[2] = {Sq[0], Sq[1]};
+        rc = launch_pass_b(&gc, ins, outs, 2, order[i], +1, nullptr, -1, s, last ? 2 : 0, fp32, fp32);
+      }
       if (rc) return rc;
     }
   }
@@ -911,8 +961,10 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
       hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, part, nblk, dF, dF + n,
                          dF + 2 * n, sample_idx, sample_w, sample_out);
   }
-  for (int c = 0; c < 3; ++c) {
-    rc = fp32 ? fft3_acc(g, (cplx*)(dFf + c * n), F[c], -1, s, 1) : fft3_acc(g, dF + c * n, F[c], -1, s);
+  {
+    cplx* tmps[3];
+    for (int c = 0; c < 3; ++c) tmps[c] = fp32 ? (cplx*)(dFf + c * n) : dF + c * n;
+    rc = fft3_acc(g, tmps, F, 3, -1, s, fp32 ? 1 : 0);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
   PSS_CHECK(hipGetLastError());
