@@ -405,8 +405,10 @@ __global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __res
 // a_p = n_pm (1-n_p), b_p = (1-n_pm) n_p are shared by every k, so a workgroup
 // owns one q and 256 k, stages (G, a, b) of a p chunk in LDS (broadcast reads)
 // and accumulates Sa = sum L(F + G_p) a_p, Sb = sum L(F + G_p) b_p with F =
-// E[k+q] - E[k]; D = pi/hbar V^2 ((1-n_k) Sa + n_k Sb).
-// grid (ceil(Nk/256), 2Nk+1 (q + Nk), 2 nwires (species));  electrons -> De.
+// E[k+q] - E[k]; D = pi/hbar V^2 ((1-n_k) Sa + n_k Sb).  For the same-species term
+// the sums read a per-workgroup table of the line shape over m = k - p + q (see below).
+// grid (ceil(Nk/256), 2Nk+1 (q + Nk), 2 nwires (species)), dynamic LDS (256 + max Nk)
+// doubles;  electrons -> De.
 __device__ inline void offdiag_sum64(const double* __restrict__ G, const double* __restrict__ A,
                                      const double* __restrict__ Bv, int n, double F, double pre, double b2,
                                      double& Sa, double& Sb) {
@@ -429,6 +431,32 @@ __device__ inline void offdiag_sum32(const float4* __restrict__ P, int n, float 
       const float4 v = P[j];
       const float z = Fs + v.x;
       const float l = __frcp_rn(fmaf(z, z, 1.0f));
+      sa = fmaf(l, v.y, sa);
+      sb = fmaf(l, v.z, sb);
+    }
+    Sa += (double)sa;
+    Sb += (double)sb;
+  }
+}
+
+// table forms: T[-j] is the line shape of partner j (T points at the entry of j = 0)
+__device__ inline void offdiag_tab64(const double* __restrict__ T, const double* __restrict__ A,
+                                     const double* __restrict__ Bv, int n, double& Sa, double& Sb) {
+  for (int j = 0; j < n; ++j) {
+    const double l = T[-j];
+    Sa = fma(l, A[j], Sa);
+    Sb = fma(l, Bv[j], Sb);
+  }
+}
+
+__device__ inline void offdiag_tab32(const float* __restrict__ T, const float4* __restrict__ P, int n, double& Sa,
+                                     double& Sb) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j1 = min(n, j0 + 64);
+    float sa = 0.f, sb = 0.f;
+    for (int j = j0; j < j1; ++j) {
+      const float l = T[-j];
+      const float4 v = P[j];
       sa = fmaf(l, v.y, sa);
       sb = fmaf(l, v.z, sb);
     }
@@ -463,6 +491,8 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
   constexpr int CH = 512;
   __shared__ double sG[CH], sa[CH], sb[CH];
   __shared__ float4 sP[CH];
+  extern __shared__ double sT[];  // 256 + Nk - 1 line-shape values (FP64, or FP32 in mixed)
+  float* sTf = (float*)sT;
   // g = [gam_eh, gam_e, gam_h]; E2 gets (g0, g2), H2 gets (g1, g2)  (dephasing.py:735-736)
   const double bAA = kHbar * (holes ? d.gam_e : d.gam_eh);
   const double bAB = kHbar * d.gam_h;
@@ -490,6 +520,25 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
     const int p0 = term == 0 ? max(q + 1, 0) : max(-q, 0);
     const int p1 = term == 0 ? min(q + Nk - 1, Nk - 1) : min(Nk - 1 - q, Nk - 1);
     double Sa = 0.0, Sb = 0.0;
+    // same species: the bands are parabolic on a uniform grid (E[j] = c kr[j]^2, kr[j] =
+    // dkr (j - j0)), so F_k + G_p = D (k - p + q) with D = 2 c dkr^2 q and the line shape
+    // depends on m = k - p + q only: tabulated once per workgroup for the m it meets,
+    // m in [mlo, mlo + nm)
+    const bool tab = term == 0;
+    const int mlo = kb - p1 + q, nm = 256 + (p1 - p0);
+    if (tab) {
+      const double D = 2.0 * (E[0] / (d.kr[0] * d.kr[0])) * d.dkr * d.dkr * (double)q;
+      __syncthreads();
+      for (int i = threadIdx.x; i < nm; i += 256) {
+        const double Ed = D * (double)(mlo + i);
+        if (fast) {
+          const double z = Ed * ib;
+          sTf[i] = (float)(1.0 / (1.0 + z * z));
+        } else {
+          sT[i] = ddiv(pre, Ed * Ed + b2);
+        }
+      }
+    }
     for (int c0 = p0; c0 <= p1; c0 += CH) {
       const int n = min(CH, p1 - c0 + 1);
       __syncthreads();
@@ -508,8 +557,15 @@ __global__ void __launch_bounds__(256) k_offdiag(const pss_wire_desc* __restrict
       }
       __syncthreads();
       if (valid) {
-        if (fast) offdiag_sum32(sP, n, (float)(F * ib), Sa, Sb);
-        else offdiag_sum64(sG, sa, sb, n, F, pre, b2, Sa, Sb);
+        if (tab) {
+          const int base = k - c0 + q - mlo;  // table index of p = c0 + j is base - j
+          if (fast) offdiag_tab32(sTf + base, sP, n, Sa, Sb);
+          else offdiag_tab64(sT + base, sa, sb, n, Sa, Sb);
+        } else if (fast) {
+          offdiag_sum32(sP, n, (float)(F * ib), Sa, Sb);
+        } else {
+          offdiag_sum64(sG, sa, sb, n, F, pre, b2, Sa, Sb);
+        }
       }
     }
     if (valid) {
@@ -2004,7 +2060,8 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prep, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
+                     (size_t)(256 + max_Nk) * sizeof(double), s, d_descs);
   const int t16 = (max_Nk + 15) / 16;
   hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
   {
