@@ -147,7 +147,9 @@ int pss_fft3d(const pss_grid* g, double* F, int32_t dir, void* stream);
  * One CPML half step on fields F (k-space in, k-space out) driven by G:
  * which = 0: F = E, G = B (ApplyCPML_E, PSTD3D/cpml.py:640-702, scale v2*dt, sign +1);
  * which = 1: F = B, G = E (ApplyCPML_B, PSTD3D/cpml.py:575-638, scale dt, sign -1).
- * psi: 6 real arrays [Nx*Ny*Nz] (order xy, xz, yx, yz, zx, zy).  scratch: 6 complex [N].
+ * psi: the 6 auxiliary arrays (order xy, xz, yx, yz, zx, zy), each stored on the PML
+ * slab of its derivative axis b (shape with axis b of length 2 npml_b), back to back:
+ * pss_cpml_psi_len(g) doubles.  scratch: 6 complex [N].
  * src_comp >= 0 fuses UpdateTFSC (PSTD3D/tfsf.py:63-142) on that component: the
  * additive S(x) E_inc(x,y,z,t) dx is applied in real space before the forward FFT;
  * src_par (host, 9 doubles) = {t, amp, omega0, tau_G, chirp, v, Tp, 1/w0^2, dx};
@@ -161,6 +163,10 @@ int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, doub
                   int32_t src_comp, const double* src_par, const double* srcx, const double* rx,
                   const double* ry, const double* rz, const int32_t* sample_idx, const double* sample_w,
                   double* sample_out, void* stream);
+
+/* Length in doubles of the psi buffer of pss_cpml_half for this grid (sizing helper of
+ * the CPML state of PSTD3D/cpml.py:340-450, which keeps full-grid psi arrays). */
+int64_t pss_cpml_psi_len(const pss_grid* g);
 
 /* One FFT pass along `axis` of the grid's shape (the other axes batch), in -> out
  * (may alias).  Used for 1-D transforms of the separable current factors. */

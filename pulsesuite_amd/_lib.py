@@ -64,6 +64,7 @@ def _load():
     sig = {
         "pss_sbe_workspace_bytes": (C.c_int64, [_i, _i, _i, _i]),
         "pss_sbe_tune": (C.c_int, [_i]),
+        "pss_cpml_psi_len": (C.c_int64, [C.POINTER(Grid)]),
         "pss_sbe_step": (C.c_int, [_vp, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
@@ -90,7 +91,7 @@ def _load():
 lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_vint", "pss_update_e3d", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
+           "pss_fft3d", "pss_fft_line", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 
 

@@ -1,7 +1,8 @@
 """CPML for the PSTD solver on the GPU (drop-in for PSTD3D/cpml.py:340-818).
 
 ``CPML(Nx, Ny, Nz, dx, dy, dz, dt, epsr, npml=None)`` keeps the 12 auxiliary psi
-fields and 6 derivative buffers on the device; ``ApplyCPML_E`` /
+fields (each on the PML slabs of its derivative axis, where it can be nonzero) and a
+6-field scratch on the device; ``ApplyCPML_E`` /
 ``ApplyCPML_B`` take k-space complex128 CUDA tensors and correct them in place
 (spectral derivatives, real-space recursion, FFT back), as the reference does.
 ``npml`` overrides the reference's 5 %-of-the-grid rule on every axis that has
@@ -23,8 +24,10 @@ class CPML:
         self.npml_x, self.npml_y, self.npml_z = self.ctx.npml
         n = self.ctx.n
         dev = self.ctx.device
-        self.psi_B = torch.zeros(6 * n, dtype=torch.float64, device=dev)
-        self.psi_E = torch.zeros(6 * n, dtype=torch.float64, device=dev)
+        # psi on the PML slabs of their derivative axes only (zero everywhere else)
+        npsi = max(1, int(_lib.lib.pss_cpml_psi_len(self.ctx.gp)))
+        self.psi_B = torch.zeros(npsi, dtype=torch.float64, device=dev)
+        self.psi_E = torch.zeros(npsi, dtype=torch.float64, device=dev)
         self.scratch = torch.empty(6 * n, dtype=torch.complex128, device=dev)
 
     def GetCPMLInfo(self):

@@ -23,9 +23,10 @@ __global__ void __launch_bounds__(256) k_update_e(pss_grid g, const cplx* __rest
   const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (o >= n) return;
-  const int iz = (int)(o % g.Nz);
-  const int iy = (int)((o / g.Nz) % g.Ny);
-  const int ix = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
+  const int iz = (int)(o & (g.Nz - 1));
+  const int iy = (int)((o >> lz) & (g.Ny - 1));
+  const int ix = (int)(o >> (lz + ly));
   const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
   const double coeff = g.v2 * g.dt;
   const cplx bx = Bx[o], by = By[o], bz = Bz[o];
@@ -51,9 +52,10 @@ __global__ void __launch_bounds__(256) k_update_b(pss_grid g, const cplx* __rest
   const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (o >= n) return;
-  const int iz = (int)(o % g.Nz);
-  const int iy = (int)((o / g.Nz) % g.Ny);
-  const int ix = (int)(o / ((int64_t)g.Nz * g.Ny));
+  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
+  const int iz = (int)(o & (g.Nz - 1));
+  const int iy = (int)((o >> lz) & (g.Ny - 1));
+  const int ix = (int)(o >> (lz + ly));
   const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
   const cplx ex = Ex[o], ey = Ey[o], ez = Ez[o];
   const cplx cx = qy * ez - qz * ey, cy = qz * ex - qx * ez, cz = qx * ey - qy * ex;
@@ -570,10 +572,29 @@ int deriv_pair(const pss_grid* g, const cplx* G, int c, int a1, int a2, void* R1
 // other axes run on that array (2 P_a / N_a of the grid), the last writing the real
 // part.  Derivative q: 0 dGz/dy, 1 dGy/dz, 2 dGx/dz, 3 dGz/dx, 4 dGy/dx, 5 dGx/dy.
 struct SlabR {
-  int64_t off[6];  // element offset of derivative q in R
-  int on[6];       // q computed (else it is never read: axis of length 1 or no PML along a)
-  int full;        // 1: full-grid arrays R[q n + o] (fallback path), 0: slab arrays
+  int64_t off[6];   // element offset of derivative q in R
+  int on[6];        // q computed (else it is never read: axis of length 1 or no PML along a)
+  int full;         // 1: full-grid arrays R[q n + o] (fallback path), 0: slab arrays
+  int64_t poff[6];  // element offsets of psi_xy, psi_xz, psi_yx, psi_yz, psi_zx, psi_zy, each
+                    // stored on the slab of its derivative axis (elsewhere c = 0 keeps it 0)
 };
+
+// derivative axis of psi_xy, psi_xz, psi_yx, psi_yz, psi_zx, psi_zy
+constexpr int kPsiAxis[6] = {1, 2, 0, 2, 0, 1};
+
+// offsets of the six slab-stored psi arrays; returns their total length in doubles
+int64_t psi_layout(const pss_grid* g, int64_t poff[6]) {
+  const int len[3] = {g->Nx, g->Ny, g->Nz};
+  const int pml[3] = {g->npml_x, g->npml_y, g->npml_z};
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  int64_t off = 0;
+  for (int j = 0; j < 6; ++j) {
+    const int b = kPsiAxis[j];
+    poff[j] = off;
+    if (len[b] > 1 && pml[b] > 0) off += n / len[b] * 2 * pml[b];
+  }
+  return off;
+}
 
 constexpr int kDerivField[6] = {2, 1, 0, 2, 1, 0};
 constexpr int kDerivAxis[6] = {1, 2, 2, 0, 0, 1};
@@ -682,19 +703,23 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
       else if (sr.on[q] && in_a[a]) d[q] = (double)R[sr.off[q] + slab_index(a, i, j, k, g.Nx, g.Ny, g.Nz, P[a])];
     }
     const double gzy = d[0], gyz = d[1], gxz = d[2], gzx = d[3], gyx = d[4], gxy = d[5];
-    double* pxy = psi;
-    double* pxz = psi + n;
-    double* pyx = psi + 2 * n;
-    double* pyz = psi + 3 * n;
-    double* pzx = psi + 4 * n;
-    double* pzy = psi + 5 * n;
-    const double vxy = g.by[j] * pxy[o] + g.cy[j] * gzy;
-    const double vxz = g.bz[k] * pxz[o] + g.cz[k] * gyz;
-    const double vyx = g.bx[i] * pyx[o] + g.cx[i] * gzx;
-    const double vyz = g.bz[k] * pyz[o] + g.cz[k] * gxz;
-    const double vzx = g.bx[i] * pzx[o] + g.cx[i] * gyx;
-    const double vzy = g.by[j] * pzy[o] + g.cy[j] * gxy;
-    pxy[o] = vxy; pxz[o] = vxz; pyx[o] = vyx; pyz[o] = vyz; pzx[o] = vzx; pzy[o] = vzy;
+    // psi_ab = b_b psi_ab + c_b dG/db on the slab of b; outside it psi_ab and c_b are 0
+    const double gb[6] = {gzy, gyz, gzx, gxz, gyx, gxy};  // the derivative each psi reads
+    const double* bb[3] = {g.bx, g.by, g.bz};
+    const double* cb[3] = {g.cx, g.cy, g.cz};
+    const int ib[3] = {i, j, k};
+    double v[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int b = kPsiAxis[q];
+      v[q] = 0.0;
+      if (in_a[b]) {
+        double* ps = psi + sr.poff[q] + slab_index(b, i, j, k, g.Nx, g.Ny, g.Nz, P[b]);
+        v[q] = bb[b][ib[b]] * *ps + cb[b][ib[b]] * gb[q];
+        *ps = v[q];
+      }
+    }
+    const double vxy = v[0], vxz = v[1], vyx = v[2], vyz = v[3], vzx = v[4], vzy = v[5];
     const double cx_ = g.iky[j] * gzy + vxy - g.ikz[k] * gyz - vxz;
     const double cy_ = g.ikz[k] * gxz + vyz - g.ikx[i] * gzx - vyx;
     const double cz_ = g.ikx[i] * gyx + vzx - g.iky[j] * gxy - vzy;
@@ -856,9 +881,16 @@ __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, c
 }  // namespace
 
 // ================================================================== C ABI
+static bool pow2_axes(const pss_grid* g) {
+  for (int N : {g->Nx, g->Ny, g->Nz})
+    if (N < 1 || (N & (N - 1))) return false;
+  return true;
+}
+
 extern "C" int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
                               const double* Jx, const double* Jy, const double* Jz, const double* jline,
                               const double* gy, const double* gz, double* Ex, double* Ey, double* Ez, void* stream) {
+  if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d: axis lengths must be powers of two");
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   hipLaunchKernelGGL(k_update_e, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g,
                      (const cplx*)Bx, (const cplx*)By, (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy,
@@ -870,6 +902,7 @@ extern "C" int pss_update_e3d(const pss_grid* g, const double* Bx, const double*
 
 extern "C" int pss_update_b3d(const pss_grid* g, const double* Ex, const double* Ey, const double* Ez, double* Bx,
                               double* By, double* Bz, void* stream) {
+  if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_b3d: axis lengths must be powers of two");
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   hipLaunchKernelGGL(k_update_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g,
                      (const cplx*)Ex, (const cplx*)Ey, (const cplx*)Ez, (cplx*)Bx, (cplx*)By, (cplx*)Bz);
@@ -918,6 +951,7 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   for (int N : {g->Nx, g->Ny, g->Nz})
     if (N > 1 && (N < 64 || N > 1024)) slab = false;
   SlabR sr{};
+  psi_layout(g, sr.poff);
   int rc = 0;
   if (slab) {
     const cplx* G3[3] = {gx, gy, gz};
@@ -973,6 +1007,11 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
 
 __global__ void k_add_scalar(double* x, double v) {
   if (threadIdx.x == 0) x[0] += v;
+}
+
+extern "C" int64_t pss_cpml_psi_len(const pss_grid* g) {
+  int64_t poff[6];
+  return psi_layout(g, poff);
 }
 
 extern "C" int pss_add_scalar(double* x, double v, void* stream) {
