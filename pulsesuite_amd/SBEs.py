@@ -135,6 +135,16 @@ class DeviceWires:
     def _setup_wire(self, i, m, w):
         dev = self.device
         Nk, Nr = m.Nk, m.Nr
+        # the dephasing table and the collision-line recurrence need parabolic bands on a
+        # uniform grid: E[k] = c kr[k]^2 with kr[k] = dkr (k - k0)
+        kr = m.arrays["kr"]
+        for name in ("Ee", "Eh"):
+            E = m.arrays[name]
+            c = E[0] / kr[0] ** 2
+            if not np.allclose(E, c * kr**2, rtol=1e-12, atol=0.0):
+                raise ValueError(f"{name} is not parabolic in kr; the device kernels need E = c kr^2")
+        if Nk > 1 and not np.allclose(np.diff(kr), kr[1] - kr[0], rtol=1e-12, atol=0.0):
+            raise ValueError("kr is not uniform; the device kernels need a uniform k grid")
         nrr = self.nrr[i]
         s = m.scalars
         st = {}
