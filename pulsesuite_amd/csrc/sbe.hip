@@ -1571,72 +1571,81 @@ __device__ inline CollCtx coll_ctx(const pss_wire_desc& d) {
   return x;
 }
 
+// One species' line data: energies E (global, E(k) = E[k-2] for 1-based k >= 2), and in
+// LDS m = 1 - n and n indexed by 1-based momentum k = 0..Nk (n = |Re n3|).
+struct LineSp {
+  const double* E;
+  const double* m;
+  const double* n;
+};
+__device__ __forceinline__ double ek(const LineSp& S, int k) { return S.E[k - 2]; }
+
 // Gaussian line (GaussDelta) by recurrence.  With parabolic bands on a uniform grid the
-// energy mismatch of a line is linear in the summed momentum, z_j = (base + P[j].E -
-// Q[j+s].E) / hw = z_lo + (j - lo) beta, so e_j = exp(-z_j^2) obeys
+// energy mismatch of a line is linear in the summed momentum, z_j = (base + E(j) -
+// E(j+s)) / hw = z_lo + (j - lo) beta, so e_j = exp(-z_j^2) obeys
 //   e_{j+1} = e_j r_j,  r_j = exp(-2 z_j beta - beta^2),  r_{j+1} = r_j exp(-2 beta^2)
 // (and the mirror image downwards).  Starting at the grid point nearest the line centre
 // (|z| <= |beta| / 2) the terms only decrease, so the recurrence neither overflows nor
 // loses the line to an underflowed start; 3 exp per line instead of one per term, FP64
 // throughout (relative error ~1e-16 per step).
-__device__ inline double coll_line_gauss(const CollCtx& x, int kind, double base, const double2* __restrict__ P,
-                                         const double2* __restrict__ Q, int s, int lo, int hi) {
+__device__ inline double coll_line_gauss(const CollCtx& x, int kind, double base, const LineSp& S, int s, int lo,
+                                         int hi) {
   if (hi < lo) return 0.0;
   const double ih = x.ihw[kind];
-  const double zlo = (base + P[lo].x - Q[lo + s].x) * ih;
-  if (hi == lo) return x.pre[kind] * exp(-(zlo * zlo)) * ((1.0 - P[lo].y) * Q[lo + s].y);
-  const double zhi = (base + P[hi].x - Q[hi + s].x) * ih;
+  const double* __restrict__ m = S.m;
+  const double* __restrict__ n = S.n;
+  const double zlo = (base + ek(S, lo) - ek(S, lo + s)) * ih;
+  if (hi == lo) return x.pre[kind] * exp(-(zlo * zlo)) * (m[lo] * n[lo + s]);
+  const double zhi = (base + ek(S, hi) - ek(S, hi + s)) * ih;
   const double beta = (zhi - zlo) / (double)(hi - lo);
   double jc = beta != 0.0 ? (double)lo - zlo / beta : (double)lo;
   jc = fmin(fmax(jc, (double)lo), (double)hi);
   const int jp = (int)rint(jc);
-  const double zp = (base + P[jp].x - Q[jp + s].x) * ih;
+  const double zp = (base + ek(S, jp) - ek(S, jp + s)) * ih;
   const double c = exp(-2.0 * beta * beta);
   const double e0 = exp(-(zp * zp));
-  double acc = e0 * ((1.0 - P[jp].y) * Q[jp + s].y);
+  double acc = e0 * (m[jp] * n[jp + s]);
   double e = e0, r = exp(-(2.0 * zp * beta + beta * beta));
   for (int j = jp + 1; j <= hi; ++j) {
     e *= r;
     r *= c;
-    acc = fma(e, (1.0 - P[j].y) * Q[j + s].y, acc);
+    acc = fma(e, m[j] * n[j + s], acc);
   }
   e = e0;
   r = exp(2.0 * zp * beta - beta * beta);
   for (int j = jp - 1; j >= lo; --j) {
     e *= r;
     r *= c;
-    acc = fma(e, (1.0 - P[j].y) * Q[j + s].y, acc);
+    acc = fma(e, m[j] * n[j + s], acc);
   }
   return x.pre[kind] * acc;
 }
 
-// sum over j in [lo, hi] of shape(base + P[j].E - Q[j + s].E) (1 - P[j].n) Q[j + s].n
-// where P / Q are (energy, population) pairs indexed by 1-based momentum
+// sum over j in [lo, hi] of shape(base + E(j) - E(j+s)) (1 - n[j]) n[j+s] along one line
 template <bool LORENTZ, bool FAST>
-__device__ inline double coll_line(const CollCtx& x, int kind, double base, const double2* __restrict__ P,
-                                   const double2* __restrict__ Q, int s, int lo, int hi) {
-  if (!LORENTZ) return coll_line_gauss(x, kind, base, P, Q, s, lo, hi);
+__device__ inline double coll_line(const CollCtx& x, int kind, double base, const LineSp& S, int s, int lo, int hi) {
+  if (!LORENTZ) return coll_line_gauss(x, kind, base, S, s, lo, hi);
   double acc = 0.0;
-  for (int j = lo; j <= hi; ++j) {
-    const double2 pj = P[j], qj = Q[j + s];
-    acc += line_shape<LORENTZ, FAST>(x, kind, base + pj.x - qj.x) * ((1.0 - pj.y) * qj.y);
-  }
+  for (int j = lo; j <= hi; ++j)
+    acc += line_shape<LORENTZ, FAST>(x, kind, base + ek(S, j) - ek(S, j + s)) * (S.m[j] * S.n[j + s]);
   return acc;
 }
 
 template <bool LORENTZ, bool FAST>
-__device__ inline void coll_pair(const CollCtx& x, bool holes, int Nk, int o, int t, const double2* Pe,
-                                 const double2* Ph, double& Sx, double& Ss) {
+__device__ inline void coll_pair(const CollCtx& x, bool holes, int Nk, int o, int t, const LineSp& Se,
+                                 const LineSp& Sh, double& Sx, double& Ss) {
   if (!holes) {
+    const double base = ek(Se, o) - ek(Se, t);
     // e-h: a = o, c = t, loop b, d = b + t - o: Ed = Ee[o-2] - Ee[t-2] + Eh[b-2] - Eh[d-2]
-    Sx = coll_line<LORENTZ, FAST>(x, 0, Pe[o].x - Pe[t].x, Ph, Ph, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
+    Sx = coll_line<LORENTZ, FAST>(x, 0, base, Sh, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
     // e-e: a = o, c = t, loop b, d = o + b - t
-    Ss = coll_line<LORENTZ, FAST>(x, 1, Pe[o].x - Pe[t].x, Pe, Pe, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
+    Ss = coll_line<LORENTZ, FAST>(x, 1, base, Se, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
   } else {
+    const double base = ek(Sh, o) - ek(Sh, t);
     // e-h: b = o, d = t, loop a, c = t + a - o: Ed = Ee[a-2] - Ee[c-2] + Eh[o-2] - Eh[t-2]
-    Sx = coll_line<LORENTZ, FAST>(x, 0, Ph[o].x - Ph[t].x, Pe, Pe, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
+    Sx = coll_line<LORENTZ, FAST>(x, 0, base, Se, t - o, max(2, 2 + o - t), min(Nk, Nk + o - t));
     // h-h: a = o, c = t, loop b, d = o + b - t
-    Ss = coll_line<LORENTZ, FAST>(x, 2, Ph[o].x - Ph[t].x, Ph, Ph, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
+    Ss = coll_line<LORENTZ, FAST>(x, 2, base, Sh, o - t, max(2, 2 + t - o), min(Nk, Nk + t - o));
   }
 }
 
@@ -1648,39 +1657,45 @@ __global__ void __launch_bounds__(256) k_coll(const pss_wire_desc* __restrict__ 
   if (o > Nk) return;
   const bool holes = blockIdx.y == 1;
   __shared__ double sh[16];
-  extern __shared__ double2 pairs[];  // (E[k-2], n[k]) for k = 0..Nk, electrons then holes
+  extern __shared__ double lsp[];  // m = 1 - n, n for k = 0..Nk: electrons (m, n), then holes (m, n)
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
-  double2* Pe = pairs;
-  double2* Ph = pairs + (Nk + 1);
+  double* mE = lsp;
+  double* nE = lsp + (Nk + 1);
+  double* mH = lsp + 2 * (Nk + 1);
+  double* nH = lsp + 3 * (Nk + 1);
   for (int k = threadIdx.x; k <= Nk; k += 256) {
-    const bool in = k >= 2;
-    Pe[k] = make_double2(in ? d.Ee[k - 2] : 0.0, k == 0 ? 0.0 : fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x));
-    Ph[k] = make_double2(in ? d.Eh[k - 2] : 0.0, k == 0 ? 0.0 : fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x));
+    const double ne = k == 0 ? 0.0 : fabs(B.Cn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    const double nh = k == 0 ? 0.0 : fabs(B.Dn[(int64_t)(k - 1) * Nk + (k - 1)].x);
+    mE[k] = 1.0 - ne;
+    nE[k] = ne;
+    mH[k] = 1.0 - nh;
+    nH[k] = nh;
   }
   __syncthreads();
   const CollCtx x = coll_ctx(d);
   const double* vc = ws + L.vc;
-  const double2* Po = holes ? Ph : Pe;  // output species
-  const double no = Po[o].y;
+  const LineSp Se{d.Ee, mE, nE}, Sh{d.Eh, mH, nH};
+  const double* nout = holes ? nH : nE;  // output species
+  const double no = nout[o];
   double win = 0.0;
   double* pout = ws + L.cp + (holes ? (int64_t)Nk * Nk : 0);
   for (int t = (int)threadIdx.x + 1; t <= Nk; t += 256) {
     double Sx = 0.0, Ss = 0.0;  // cross-species (e-h) and same-species (e-e / h-h) sums
     if (o >= 2 && t >= 2 && t != o) {
       if (x.lorentz) {
-        if (x.fast) coll_pair<true, true>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
-        else coll_pair<true, false>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+        if (x.fast) coll_pair<true, true>(x, holes, Nk, o, t, Se, Sh, Sx, Ss);
+        else coll_pair<true, false>(x, holes, Nk, o, t, Se, Sh, Sx, Ss);
       } else {
-        if (x.fast) coll_pair<false, true>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
-        else coll_pair<false, false>(x, holes, Nk, o, t, Pe, Ph, Sx, Ss);
+        if (x.fast) coll_pair<false, true>(x, holes, Nk, o, t, Se, Sh, Sx, Ss);
+        else coll_pair<false, false>(x, holes, Nk, o, t, Se, Sh, Sx, Ss);
       }
       const int q = abs(o - t);
       const double vx = vc[q], vs = vc[(holes ? 2 : 1) * Nk + q];
       Sx = vx * vx * Sx + vs * vs * Ss;
     }
-    win += Po[t].y * Sx;
+    win += nout[t] * Sx;
     pout[(int64_t)(t - 1) * Nk + (o - 1)] = (1.0 - no) * Sx;
   }
   win = block_sum<256>(win, sh);
