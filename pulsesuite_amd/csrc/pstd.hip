@@ -869,12 +869,27 @@ __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, c
     drho[x] = mk(dr.x / dt, dr.y / dt);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    cplx acc = mk(0, 0);
-    for (int x = 1; x < Nx; ++x) {
-      acc = acc - drho[x] * dx;
-      jl[x] = jl[x] + mk(acc.x / Nw, acc.y / Nw);
-    }
+  // CalcJfx: jl_x[x] += -dx sum_{x'=1..x} drho[x'] / Nw for x >= 1, as a block scan: each
+  // thread sums a contiguous run, the runs' sums are scanned across the block (wave
+  // shuffles, then the 4 wave totals), and each thread replays its run from its offset
+  __shared__ cplx wsum[4];
+  const int per = (Nx + 255) / 256;
+  const int x0 = threadIdx.x * per, x1 = min(Nx, x0 + per);
+  cplx loc = mk(0, 0);
+  for (int x = max(x0, 1); x < x1; ++x) loc = loc + drho[x];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  cplx inc = loc;  // inclusive scan of the run sums within the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const double ur = __shfl_up(inc.x, off, 64), ui = __shfl_up(inc.y, off, 64);
+    if (lane >= off) inc = inc + mk(ur, ui);
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  cplx pre = inc - loc;  // exclusive prefix within the wave
+  for (int w = 0; w < wid; ++w) pre = pre + wsum[w];
+  for (int x = max(x0, 1); x < x1; ++x) {
+    pre = pre + drho[x];
+    jl[x] = jl[x] + mk(-(pre.x * dx) / Nw, -(pre.y * dx) / Nw);
   }
 }
 

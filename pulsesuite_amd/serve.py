@@ -204,7 +204,10 @@ class Server:
         host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
         self._d_descs = host.to(self.device)
         nk = np.array([s.req.Nk for s in self.active])
-        self._ranges = balanced_ranges(nk, self.buckets)
+        # few wires: one launch sequence over all of them fills the chip best; more wires:
+        # up to `buckets` sequences side by side (one per 8 wires)
+        nb = max(1, min(self.buckets, n // 8))
+        self._ranges = balanced_ranges(nk, nb)
         self._dirty = False
 
     def _sbe_step(self):
