@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--eager", action="store_true", help="coupled steps as eager launches instead of graph replays")
+    ap.add_argument("--traffic-reps", type=int, default=3, help="copies and 3-D FFTs of the traffic workload")
     ap.add_argument("--feedback", action="store_true", help="keep the wire's P/rho in the current (diverges)")
     ap.add_argument("--batched-wires", type=int, default=64, help="wires of the batched run (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=5)
@@ -318,7 +319,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     wl = set(a.workloads.split(","))
     if "traffic" in wl:  # kernels for the HBM-counter runs: a plain copy of known size, then 3-D FFTs
-        traffic_kernels(a.grid)
+        traffic_kernels(a.grid, a.traffic_reps)
         return
     if "coupled" not in wl:  # partial runs (profiling): no headline value
         line = {"metric": METRIC, "value": None, "note": "coupled workload skipped"}
