@@ -672,57 +672,91 @@ struct SrcPar {
   double t, amp, omega0, tau_G, chirp, v, Tp, w0inv2, dx;
 };
 
-template <class TR, class TF>
+// ROWS (Nz >= 64): the 64 lanes of a wave lie in one (i, j) row, so i, j, the x- and
+// y-slab tests and their coefficients are wave-uniform (scalar); only the z slab splits
+// a wave.  Every load of a cell (derivatives, old psi, b/c and 1/kappa - 1 tables) is
+// issued before the first use, so a wave waits on memory once.
+template <class TR, class TF, bool ROWS>
 __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* __restrict__ R, SlabR sr,
                                               TF* __restrict__ dFx, TF* __restrict__ dFy, TF* __restrict__ dFz,
-                                              double* psi, int src_comp, SrcPar sp,
+                                              double* __restrict__ psi, int src_comp, SrcPar sp,
                                               const double* __restrict__ srcx, const double* __restrict__ rx,
                                               const double* __restrict__ ry, const double* __restrict__ rz) {
-  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  const int64_t n = (int64_t)Nx * Ny * Nz;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (o >= n) return;
-  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
-  const int k = (int)(o & (g.Nz - 1));
-  const int j = (int)((o >> lz) & (g.Ny - 1));
-  const int i = (int)(o >> (lz + ly));
-  const bool inx = i < g.npml_x || i >= g.Nx - g.npml_x;
-  const bool iny = j < g.npml_y || j >= g.Ny - g.npml_y;
-  const bool inz = k < g.npml_z || k >= g.Nz - g.npml_z;
+  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);  // axis lengths are powers of two
+  const int k = (int)(o & (Nz - 1));
+  int j = (int)((o >> lz) & (Ny - 1));
+  int i = (int)(o >> (lz + ly));
+  if (ROWS) {
+    i = __builtin_amdgcn_readfirstlane(i);
+    j = __builtin_amdgcn_readfirstlane(j);
+  }
+  const int Px = g.npml_x, Py = g.npml_y, Pz = g.npml_z;  // 0 on axes without a PML (and length 1)
+  const bool inx = Px > 0 && (i < Px || i >= Nx - Px);
+  const bool iny = Py > 0 && (j < Py || j >= Ny - Py);
+  const bool inz = Pz > 0 && (k < Pz || k >= Nz - Pz);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (inx || iny || inz) {
-    // real parts of d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy;
-    // outside the slab of its axis a derivative is multiplied by zero coefficients only
-    const bool in_a[3] = {inx, iny, inz};
-    const int P[3] = {g.npml_x, g.npml_y, g.npml_z};
-    double d[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int a = kDerivAxis[q];
-      d[q] = 0.0;
-      if (sr.full) d[q] = (double)R[q * n + o];
-      else if (sr.on[q] && in_a[a]) d[q] = (double)R[sr.off[q] + slab_index(a, i, j, k, g.Nx, g.Ny, g.Nz, P[a])];
+    // d0 = dGz/dy, d1 = dGy/dz, d2 = dGx/dz, d3 = dGz/dx, d4 = dGy/dx, d5 = dGx/dy (real parts)
+    // and the old psi_xy, psi_xz, psi_yx, psi_yz, psi_zx, psi_zy, each read only on the slab
+    // of its axis (elsewhere the coefficients that multiply it are 0, and psi stays 0)
+    const double ikx = g.ikx[i], iky = g.iky[j], ikz = g.ikz[k];
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0, d4 = 0.0, d5 = 0.0;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0, p5 = 0.0;
+    double bX = 0.0, cX = 0.0, bY = 0.0, cY = 0.0, bZ = 0.0, cZ = 0.0;
+    int64_t sx = 0, sy = 0, sz = 0;  // positions on the x, y, z slab arrays
+    if (inx) {
+      sx = ((int64_t)(i < Px ? i : i - (Nx - 2 * Px)) * Ny + j) * Nz + k;
+      const int64_t r = sr.full ? o : sx;
+      d3 = (double)R[sr.off[3] + r];
+      d4 = (double)R[sr.off[4] + r];
+      p2 = psi[sr.poff[2] + sx];
+      p4 = psi[sr.poff[4] + sx];
+      bX = g.bx[i];
+      cX = g.cx[i];
     }
-    const double gzy = d[0], gyz = d[1], gxz = d[2], gzx = d[3], gyx = d[4], gxy = d[5];
-    // psi_ab = b_b psi_ab + c_b dG/db on the slab of b; outside it psi_ab and c_b are 0
-    const double gb[6] = {gzy, gyz, gzx, gxz, gyx, gxy};  // the derivative each psi reads
-    const double* bb[3] = {g.bx, g.by, g.bz};
-    const double* cb[3] = {g.cx, g.cy, g.cz};
-    const int ib[3] = {i, j, k};
-    double v[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int b = kPsiAxis[q];
-      v[q] = 0.0;
-      if (in_a[b]) {
-        double* ps = psi + sr.poff[q] + slab_index(b, i, j, k, g.Nx, g.Ny, g.Nz, P[b]);
-        v[q] = bb[b][ib[b]] * *ps + cb[b][ib[b]] * gb[q];
-        *ps = v[q];
-      }
+    if (iny) {
+      sy = ((int64_t)i * (2 * Py) + (j < Py ? j : j - (Ny - 2 * Py))) * Nz + k;
+      const int64_t r = sr.full ? o : sy;
+      d0 = (double)R[sr.off[0] + r];
+      d5 = (double)R[sr.off[5] + r];
+      p0 = psi[sr.poff[0] + sy];
+      p5 = psi[sr.poff[5] + sy];
+      bY = g.by[j];
+      cY = g.cy[j];
     }
-    const double vxy = v[0], vxz = v[1], vyx = v[2], vyz = v[3], vzx = v[4], vzy = v[5];
-    const double cx_ = g.iky[j] * gzy + vxy - g.ikz[k] * gyz - vxz;
-    const double cy_ = g.ikz[k] * gxz + vyz - g.ikx[i] * gzx - vyx;
-    const double cz_ = g.ikx[i] * gyx + vzx - g.iky[j] * gxy - vzy;
+    if (inz) {
+      sz = ((int64_t)i * Ny + j) * (2 * Pz) + (k < Pz ? k : k - (Nz - 2 * Pz));
+      const int64_t r = sr.full ? o : sz;
+      d1 = (double)R[sr.off[1] + r];
+      d2 = (double)R[sr.off[2] + r];
+      p1 = psi[sr.poff[1] + sz];
+      p3 = psi[sr.poff[3] + sz];
+      bZ = g.bz[k];
+      cZ = g.cz[k];
+    }
+    // psi_ab = b_b psi_ab + c_b dG/db (0 outside the slab of b)
+    const double vxy = bY * p0 + cY * d0, vxz = bZ * p1 + cZ * d1;
+    const double vyx = bX * p2 + cX * d3, vyz = bZ * p3 + cZ * d2;
+    const double vzx = bX * p4 + cX * d4, vzy = bY * p5 + cY * d5;
+    if (inx) {
+      psi[sr.poff[2] + sx] = vyx;
+      psi[sr.poff[4] + sx] = vzx;
+    }
+    if (iny) {
+      psi[sr.poff[0] + sy] = vxy;
+      psi[sr.poff[5] + sy] = vzy;
+    }
+    if (inz) {
+      psi[sr.poff[1] + sz] = vxz;
+      psi[sr.poff[3] + sz] = vyz;
+    }
+    const double cx_ = iky * d0 + vxy - ikz * d1 - vxz;
+    const double cy_ = ikz * d2 + vyz - ikx * d3 - vyx;
+    const double cz_ = ikx * d4 + vzx - iky * d5 - vzy;
     const double sc = which == 0 ? g.v2 * g.dt : -g.dt;
     fx = sc * cx_;
     fy = sc * cy_;
@@ -949,6 +983,12 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   if (g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
     return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
+  {
+    const int len[3] = {g->Nx, g->Ny, g->Nz}, pml[3] = {g->npml_x, g->npml_y, g->npml_z};
+    for (int a = 0; a < 3; ++a)
+      if (pml[a] < 0 || (pml[a] > 0 && (len[a] == 1 || 2 * pml[a] > len[a])))
+        return pss_set_msg(2, "pss_cpml_half: a PML needs an axis longer than 1 and 2 npml <= N");
+  }
   // scratch: 6 complex fields.  Slots 0-2 hold the six real derivative arrays,
   // slots 3-4 the complex temporaries of the derivative passes, then slots 3-5 dF.
   cplx* sc = (cplx*)scratch;
@@ -990,12 +1030,25 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   }
   cplx* dF = sc + 3 * n;       // complex128 dF (FP64), or
   cflt* dFf = (cflt*)dF;       // complex64 dF (mixed: transformed in FP32, added to the FP64 fields)
-  if (fp32)
-    hipLaunchKernelGGL((k_cpml<float, cflt>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, Rf,
-                       sr, dFf, dFf + n, dFf + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
-  else
-    hipLaunchKernelGGL((k_cpml<double, cplx>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, which, R,
-                       sr, dF, dF + n, dF + 2 * n, psi, src_comp, sp, srcx, rx, ry, rz);
+  {
+    const dim3 gr((unsigned)((n + 255) / 256)), bl(256);
+    const bool rows = g->Nz >= 64;
+    if (fp32) {
+      if (rows)
+        hipLaunchKernelGGL((k_cpml<float, cflt, true>), gr, bl, 0, s, *g, which, Rf, sr, dFf, dFf + n, dFf + 2 * n,
+                           psi, src_comp, sp, srcx, rx, ry, rz);
+      else
+        hipLaunchKernelGGL((k_cpml<float, cflt, false>), gr, bl, 0, s, *g, which, Rf, sr, dFf, dFf + n, dFf + 2 * n,
+                           psi, src_comp, sp, srcx, rx, ry, rz);
+    } else {
+      if (rows)
+        hipLaunchKernelGGL((k_cpml<double, cplx, true>), gr, bl, 0, s, *g, which, R, sr, dF, dF + n, dF + 2 * n,
+                           psi, src_comp, sp, srcx, rx, ry, rz);
+      else
+        hipLaunchKernelGGL((k_cpml<double, cplx, false>), gr, bl, 0, s, *g, which, R, sr, dF, dF + n, dF + 2 * n,
+                           psi, src_comp, sp, srcx, rx, ry, rz);
+    }
+  }
   cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
   if (sample_idx) {
     // partial sums live in the (now free) derivative slots
