@@ -171,6 +171,12 @@ int64_t pss_cpml_psi_len(const pss_grid* g);
 /* One FFT pass along `axis` of the grid's shape (the other axes batch), in -> out
  * (may alias).  Used for 1-D transforms of the separable current factors. */
 int pss_fft_line(const pss_grid* g, const double* in, double* out, int32_t axis, int32_t dir, void* stream);
+/* The same pass over nb <= 3 arrays of the grid's shape in one launch: in[b] -> out[b]
+ * (host arrays of device pointers; each pair may alias).  Used for the three components
+ * of the separable current line J(x), whose product with the transformed transverse
+ * gates is the 3-D FFT of the deposited current (QWPlacement, PSTD3D/rhoPJ.py:736-781). */
+int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out, int32_t nb, int32_t axis,
+                       int32_t dir, void* stream);
 
 /* Fill three uniform wire lines [3][n] with the point sample (complex [3]): the
  * wire sees E at one point (QWArray.QuantumWire, PSTD3D/rhoPJ.py:1160-1164). */

@@ -791,28 +791,30 @@ __device__ inline cplx twp(const cplx* tw, int N, int r) {  // exp(+2 pi i r / N
 __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restrict__ Fx, const cplx* __restrict__ Fy,
                                                  const cplx* __restrict__ Fz, const int32_t* __restrict__ idx,
                                                  const double* __restrict__ w, double* partial) {
-  extern __shared__ cplx tws[];  // Nx/2 + Ny/2 + Nz/2 twiddles
+  // Phi(k) = sum_c w_c exp(+2 pi i (mx cx/Nx + my cy/Ny + mz cz/Nz)).  The axis lengths are
+  // powers of two, so with L = max(Nx, Ny, Nz) the phase is exp(+2 pi i r/L) with the exact
+  // integer r = (mx cx L/Nx + my cy L/Ny + mz cz L/Nz) mod L: one twiddle of the longest
+  // axis per corner instead of a product of three.
+  extern __shared__ cplx tws[];  // L/2 twiddles exp(-2 pi i m/L)
   __shared__ double red[6][4];
   __shared__ int cx[8], cy[8], cz[8];
   __shared__ double cw[8];
   const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
-  cplx* tx = tws;
-  cplx* ty = tx + Nx / 2;
-  cplx* tz = ty + Ny / 2;
-  for (int m = threadIdx.x; m < Nx / 2; m += 256) tx[m] = mk(g.twx[2 * m], g.twx[2 * m + 1]);
-  for (int m = threadIdx.x; m < Ny / 2; m += 256) ty[m] = mk(g.twy[2 * m], g.twy[2 * m + 1]);
-  for (int m = threadIdx.x; m < Nz / 2; m += 256) tz[m] = mk(g.twz[2 * m], g.twz[2 * m + 1]);
+  const int L = max(Nx, max(Ny, Nz));
+  const double* twL = L == Nx ? g.twx : L == Ny ? g.twy : g.twz;
+  for (int m = threadIdx.x; m < L / 2; m += 256) tws[m] = mk(twL[2 * m], twL[2 * m + 1]);
   if (threadIdx.x < 8) {
     const int64_t o = idx[threadIdx.x];
     const bool ok = o >= 0;
-    cx[threadIdx.x] = ok ? (int)(o / ((int64_t)Ny * Nz)) : 0;
-    cy[threadIdx.x] = ok ? (int)((o / Nz) % Ny) : 0;
-    cz[threadIdx.x] = ok ? (int)(o % Nz) : 0;
+    // corner coordinates pre-scaled by L/N of their axis
+    cx[threadIdx.x] = ok ? (int)(o / ((int64_t)Ny * Nz)) * (L / Nx) : 0;
+    cy[threadIdx.x] = ok ? (int)((o / Nz) % Ny) * (L / Ny) : 0;
+    cz[threadIdx.x] = ok ? (int)(o % Nz) * (L / Nz) : 0;
     cw[threadIdx.x] = ok ? w[threadIdx.x] : 0.0;
   }
   __syncthreads();
   const int64_t n = (int64_t)Nx * Ny * Nz;
-  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);  // axis lengths are powers of two
+  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);
   cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
   for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
     const cplx f0 = Fx[o], f1 = Fy[o], f2 = Fz[o];
@@ -822,10 +824,12 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
     cplx phi = mk(0, 0);
     for (int c = 0; c < 8; ++c) {
       if (cw[c] == 0.0) continue;
-      const cplx ex = Nx > 1 ? twp(tx, Nx, mx * cx[c]) : mk(1, 0);
-      const cplx ey = Ny > 1 ? twp(ty, Ny, my * cy[c]) : mk(1, 0);
-      const cplx ez = Nz > 1 ? twp(tz, Nz, mz * cz[c]) : mk(1, 0);
-      phi = phi + cw[c] * (ex * ey * ez);
+      if (L == 1) {
+        phi.x += cw[c];
+        continue;
+      }
+      const int r = (mx * cx[c] + my * cy[c] + mz * cz[c]) & (L - 1);
+      phi = phi + cw[c] * twp(tws, L, r);
     }
     a0 = a0 + f0 * phi;
     a1 = a1 + f1 * phi;
@@ -843,26 +847,39 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
         ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
 }
 
+// One workgroup: the six sums over the per-block partials side by side (one reduction),
+// the 24 corner reads of dF issued together.
 template <class TF>
 __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
                                                        const TF* __restrict__ dFx, const TF* __restrict__ dFy,
                                                        const TF* __restrict__ dFz, const int32_t* __restrict__ idx,
                                                        const double* __restrict__ w, double* out) {
-  __shared__ double sh[16];
+  __shared__ double red[6][4];
+  __shared__ double dfw[3][8];  // w_c dF_comp(x_c), 0 for unused corners
   const double invn = 1.0 / ((double)g.Nx * g.Ny * g.Nz);
+  if (threadIdx.x < 24) {
+    const int c = threadIdx.x & 7, comp = threadIdx.x >> 3;
+    const TF* dF = comp == 0 ? dFx : comp == 1 ? dFy : dFz;
+    const int32_t o = idx[c];
+    dfw[comp][c] = o >= 0 ? w[c] * (double)dF[o].x : 0.0;
+  }
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] += partial[(int64_t)b * 6 + q];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
   for (int q = 0; q < 6; ++q) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += 256) acc += partial[(int64_t)b * 6 + q];
-    acc = block_sum<256>(acc, sh);
-    if (threadIdx.x == 0) {
-      double r = acc * invn;
-      if ((q & 1) == 0) {  // dF is real: add the sampled correction to the real parts
-        const TF* dF = q == 0 ? dFx : q == 2 ? dFy : dFz;
-        for (int c = 0; c < 8; ++c)
-          if (idx[c] >= 0) r += w[c] * (double)dF[idx[c]].x;
-      }
-      out[q] = r;
-    }
+    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_down(acc[q], off, 64);
+    if (lane == 0) red[q][wid] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    double r = (((red[q][0] + red[q][1]) + red[q][2]) + red[q][3]) * invn;
+    if ((q & 1) == 0)  // dF is real: the sampled correction adds to the real parts
+      for (int c = 0; c < 8; ++c) r += dfw[q >> 1][c];
+    out[q] = r;
   }
 }
 
@@ -974,6 +991,18 @@ extern "C" int pss_fft_line(const pss_grid* g, const double* in, double* out, in
   return 0;
 }
 
+extern "C" int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out, int32_t nb,
+                                  int32_t axis, int32_t dir, void* stream) {
+  if (nb < 1 || nb > 3 || axis < 0 || axis > 2 || (dir != 1 && dir != -1))
+    return pss_set_msg(2, "pss_fft_line_batch: 1 <= nb <= 3, axis 0..2, dir +-1");
+  const cplx* ins[3] = {(const cplx*)in[0], nb > 1 ? (const cplx*)in[1] : nullptr, nb > 2 ? (const cplx*)in[2] : nullptr};
+  cplx* outs[3] = {(cplx*)out[0], nb > 1 ? (cplx*)out[1] : nullptr, nb > 2 ? (cplx*)out[2] : nullptr};
+  const int rc = launch_pass_b(g, ins, outs, nb, axis, dir, nullptr, -1, (hipStream_t)stream);
+  if (rc) return pss_set_msg(2, "pss_fft_line_batch: bad axis length");
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
                              const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
                              const double* src_par, const double* srcx, const double* rx, const double* ry,
@@ -1054,7 +1083,7 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
     // partial sums live in the (now free) derivative slots
     const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
     double* part = R;
-    const size_t lds = (size_t)(g->Nx / 2 + g->Ny / 2 + g->Nz / 2 + 1) * sizeof(cplx);
+    const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
     hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), lds, s, *g, F[0], F[1], F[2], sample_idx, sample_w, part);
     if (fp32)
       hipLaunchKernelGGL((k_ksample_final<cflt>), dim3(1), dim3(256), 0, s, *g, part, nblk, dFf, dFf + n,

@@ -133,9 +133,8 @@ class QWArray:
         _lib.check(_lib.lib.pss_jline(_lib.ptr(self.P), _lib.ptr(self.Rho), _lib.ptr(self.Pold), _lib.ptr(self.RhoOld),
                                       self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), st), "jline")
         if space.Nx > 1:
-            for c in range(3):
-                _lib.check(_lib.lib.pss_fft_line(self._gline_x.gp, _lib.ptr(self.jl[c]), _lib.ptr(self.jl[c]), 0, -1,
-                                                 st), "fft jline")
+            lines = (_lib.C.c_void_p * 3)(*(_lib.ptr(self.jl[c]) for c in range(3)))
+            _lib.check(_lib.lib.pss_fft_line_batch(self._gline_x.gp, lines, lines, 3, 0, -1, st), "fft jline")
 
 
 _line_cache = {}

@@ -60,3 +60,19 @@ def test_struct_layout_matches_ctypes(struct):
     assert out[0] == ctypes.sizeof(cls)
     for f, off in zip(fields, out[1:]):
         assert getattr(cls, f).offset == off, f
+
+
+@pytest.mark.parametrize("shape,pml", [((64, 1, 1), (6, 3, 0)), ((32, 32, 32), (17, 6, 6)), ((32, 32, 32), (6, -1, 6))])
+def test_cpml_half_rejects_pml_that_does_not_fit(shape, pml):
+    """A PML needs an axis longer than 1 and 2 npml <= N (the slab arrays of the
+    derivatives and psi assume it); the check runs before any device work, so a bad
+    grid is refused on a machine without a GPU too (null buffers are never touched)."""
+    from pulsesuite_amd import _lib
+
+    g = _lib.Grid()
+    g.Nx, g.Ny, g.Nz = shape
+    g.npml_x, g.npml_y, g.npml_z = pml
+    rc = _lib.lib.pss_cpml_half(ctypes.byref(g), 0, None, None, None, None, None, None, None, None, -1, None, None,
+                                None, None, None, None, None, None, None)
+    assert rc != 0
+    assert b"npml" in _lib.lib.pss_last_error()

@@ -38,6 +38,40 @@ def test_fft3d_matches_numpy(shape):
     assert rel(T, A) < 1e-14
 
 
+@pytest.mark.parametrize("shape,axis", [((128, 1, 1), 0), ((8, 64, 4), 1), ((2, 4, 1024), 2), ((4, 16, 2), 1),
+                                        ((2048, 1, 1), 0)])
+@pytest.mark.parametrize("nb", [1, 2, 3])
+def test_fft_line_batch_matches_numpy(shape, axis, nb):
+    """One launch over nb arrays (in place and out of place) = numpy's FFT along the axis;
+    lengths 16 and 2048 take the one-launch-per-array path."""
+    import ctypes as C
+
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd._grid import GridContext
+    from pulsesuite_amd.typespace import ss
+
+    ctx = GridContext(ss(3, *shape, 1.0, 1.0, 1.0, 1.0), 1.0, with_cpml=False)
+    rng = np.random.default_rng(nb)
+    A = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(nb)]
+    for inplace in (True, False):
+        T = [cu(a) for a in A]
+        O = T if inplace else [torch.empty_like(t) for t in T]
+        ins = (C.c_void_p * nb)(*(t.data_ptr() for t in T))
+        outs = (C.c_void_p * nb)(*(t.data_ptr() for t in O))
+        _lib.check(_lib.lib.pss_fft_line_batch(ctx.gp, ins, outs, nb, axis, -1, _lib.stream_ptr()), "batch")
+        for b in range(nb):
+            assert rel(O[b], np.fft.fft(A[b], axis=axis)) < 1e-14
+        ins = (C.c_void_p * nb)(*(t.data_ptr() for t in O))
+        _lib.check(_lib.lib.pss_fft_line_batch(ctx.gp, ins, ins, nb, axis, +1, _lib.stream_ptr()), "batch")
+        for b in range(nb):
+            assert rel(O[b], A[b]) < 1e-14
+    bad = (C.c_void_p * 1)(T[0].data_ptr())
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.lib.pss_fft_line_batch(ctx.gp, bad, bad, 4, axis, -1, _lib.stream_ptr()), "batch")
+
+
 def test_update_e_b_match_oracle():
     from pulsesuite_amd import PSTD3D as G
     from pulsesuite_amd.typespace import ss
