@@ -89,17 +89,23 @@ def setup(grid, nk, npml, precision, feedback=False, device_time=True):
 
 
 def time_steps(prop, steps, warmup, graph=True):
-    """Wall time of `steps` steps after `warmup` eager ones: replays of the step
-    recorded as a HIP graph (default), or eager launches."""
+    """Wall time of `steps` steps after `warmup` eager ones: graph replays (default) of
+    the pipelined step (the next step's CPML-E correction beside the wire step) when the
+    propagator supports it, else of the plain step; or eager launches."""
     import torch
 
     for _ in range(warmup):
         prop.step()
-    if graph:
+    pipe = graph and prop.pipelined()
+    if pipe:
+        prop.capture_pipelined()
+    elif graph:
         prop.capture_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    if graph:
+    if pipe:
+        prop.run_pipelined(steps)
+    elif graph:
         prop.run_graph(steps)
     else:
         for _ in range(steps):
@@ -362,7 +368,8 @@ def main():
         "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
                                f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,
                    "wires": 1, "precision": main_prec,
-                   "launch": "eager" if a.eager else "HIP graph replay of the step (wire step on a second stream)"},
+                   "launch": "eager" if a.eager else ("HIP graph replays of the pipelined step (wire step on a second "
+                                                        "stream beside the B half and the next CPML-E correction)")},
     }
     if errs is not None:
         line["fp64_value"] = results["fp64"]["value"]

@@ -164,6 +164,12 @@ int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, doub
                   const double* ry, const double* rz, const int32_t* sample_idx, const double* sample_w,
                   double* sample_out, void* stream);
 
+/* The trilinear point sample of pss_cpml_half taken on its own, from k-space fields that
+ * already hold the whole half step (EAtXYZ of the IFFT, PSTD3D/usefulsubs.py:2016-2085):
+ * sample_out[6] = (Re, Im) of each component.  partial: 6144 doubles of device scratch. */
+int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz, const int32_t* sample_idx,
+                const double* sample_w, double* partial, double* sample_out, void* stream);
+
 /* Length in doubles of the psi buffer of pss_cpml_half for this grid (sizing helper of
  * the CPML state of PSTD3D/cpml.py:340-450, which keeps full-grid psi arrays). */
 int64_t pss_cpml_psi_len(const pss_grid* g);

@@ -151,6 +151,7 @@ class PSTD3DPropagator:
         self.overlap = bool(overlap) and calc_j is not None
         self._side = None
         self._graph = None
+        self._pgraphs = None
 
     def _src(self, comp, amp, t):
         if self._tfsf is None:
@@ -229,6 +230,15 @@ class PSTD3DPropagator:
         """E update + CPML-E half step (with the soft source on Ey and the wire's point
         sample), then t += dt/2: the part of a step before the calc_j hook."""
         tm = self.time
+        self._update_e(stream)
+        sample = self.calc_j.sample_args() if getattr(self.calc_j, "separable", False) else None
+        self._cpml_e(stream, sample)
+        self._advance(tm.dt / 2.0, _lib.stream_ptr(stream))
+        tm.t += tm.dt / 2.0
+
+    def _update_e(self, stream):
+        """E += i v2 dt (k x B) - mu0 v2 dt J (J from the last calc_j call)."""
+        tm = self.time
         g = self.ctx.g
         g.dt = tm.dt
         g.v2 = self.v2
@@ -236,15 +246,97 @@ class PSTD3DPropagator:
         sep = getattr(self.calc_j, "separable", False)
         jl, gy, gz = self.calc_j.jsep() if sep else (None, None, None)
         Jd = (None, None, None) if sep else (self.Jx, self.Jy, self.Jz)
-        st = _lib.stream_ptr(stream)
         _lib.check(_lib.lib.pss_update_e3d(self.ctx.gp, P(self.Bx), P(self.By), P(self.Bz), *(P(t) for t in Jd),
-                                           P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), st),
-                   "pss_update_e3d")
-        sample = self.calc_j.sample_args() if sep else None
+                                           P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez),
+                                           _lib.stream_ptr(stream)), "pss_update_e3d")
+
+    def _cpml_e(self, stream, sample=None):
+        """CPML correction of E from B and psi_E, with the soft source on Ey at the current
+        (host or device) time; optionally the wire's point sample of the corrected E."""
+        tm = self.time
         self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
                               src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
+
+    # ---- pipelined steps (graph replays of several steps)
+    # The CPML correction of an E half step reads only B, psi_E and the clock, not J, and it
+    # adds to E like the curl/current update does, so the two commute.  Applied right after
+    # the B half step of the step before, it runs while the wire step (calc_j) still works on
+    # the side stream; what waits for the wire's current is only the curl/current update and
+    # the point sample.  Same operations on the same operands in another order: results agree
+    # with eager steps to rounding.  The state between replays holds the next step's
+    # correction, so the sequence is closed: a prologue applies the first correction and the
+    # last step applies none.
+    def _pipelined_step(self, pre_next, stream=None):
+        tm = self.time
+        P = _lib.ptr
+        st = _lib.stream_ptr(stream)
+        self._update_e(stream)
+        sidx, sw, sout = self.calc_j.sample_args()
+        _lib.check(_lib.lib.pss_ksample(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(sidx), P(sw),
+                                        P(self.cpml.scratch), P(sout), st), "pss_ksample")
         self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
+        cur = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self._ensure_side()
+        self._ev_e.record(cur)
+        self._side.wait_event(self._ev_e)
+        with torch.cuda.stream(self._side):
+            self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+        self._ev_j.record(self._side)
+        self.step_b_half(stream)
+        if pre_next:
+            self._cpml_e(stream)
+        cur.wait_event(self._ev_j)
+
+    def pipelined(self):
+        """True when run_pipelined applies: device time and an overlapped, separable calc_j."""
+        return self._tdev is not None and self.overlap and getattr(self.calc_j, "separable", False)
+
+    def capture_pipelined(self):
+        """Record the pipelined step (with and without the next step's CPML-E correction)
+        as two HIP graphs; like capture_step, the recording executes nothing."""
+        if not self.pipelined():
+            raise RuntimeError("pipelined steps need device_time=True and an overlapped separable calc_j")
+        self._ensure_side()
+        tm = self.time
+        saved = (tm.t, tm.n, self._nstep, self.check_every)
+        self.check_every = 0
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        graphs = []
+        try:
+            for pre in (True, False):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.stream(s):
+                    with torch.cuda.graph(g, stream=s):
+                        self._pipelined_step(pre)
+                graphs.append(g)
+                tm.t, tm.n, self._nstep = saved[:3]
+        finally:
+            torch.cuda.current_stream(self.device).wait_stream(s)
+            tm.t, tm.n, self._nstep, self.check_every = saved
+        self._pgraphs = tuple(graphs)
+        return self._pgraphs
+
+    def run_pipelined(self, nsteps):
+        """Advance nsteps: the first step's CPML-E correction, then nsteps - 1 replays of
+        the pipelined step and one of the closing step.  Afterwards the state is that of
+        nsteps eager steps (to rounding)."""
+        if nsteps <= 0:
+            return
+        if self._pgraphs is None:
+            self.capture_pipelined()
+        gp, gl = self._pgraphs
+        self._cpml_e(None)
+        tm = self.time
+        for k in range(nsteps):
+            (gp if k < nsteps - 1 else gl).replay()
+            tm.t += tm.dt / 2.0
+            tm.t += tm.dt / 2.0
+            tm.n += 1
+            self._nstep += 1
+            if self.check_every and self._nstep % self.check_every == 0:
+                self.check_finite()
 
     def step_b_half(self, stream=None):
         """B update + CPML-B half step (soft source on Bz), then t += dt/2: the part of a

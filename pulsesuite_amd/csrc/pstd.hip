@@ -260,6 +260,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   constexpr int TB = TB0 < TBL ? TB0 : TBL;
   __shared__ cpx<T> Y[TB * N1 * (N2 + 1)];
   __shared__ cpx<T> tws[N / 2];
+  __shared__ T kl[N];  // wavenumbers along the pass axis (derivative passes)
   const int Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
   int64_t base0, sa, st;
   int nvalid;
@@ -308,6 +309,8 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
     const double wr = a.tw[2 * m], wi = a.tw[2 * m + 1];
     tws[m] = cpx<T>{(T)wr, (T)(a.dir > 0 ? -wi : wi)};
   }
+  if (a.kmul && a.mul_axis == a.axis)
+    for (int m = threadIdx.x; m < N; m += TB * G) kl[m] = (T)a.kmul[m];
   __syncthreads();
   const int tid = threadIdx.x;
   // ---- level 1: thread (t, n2) transforms x[n1 N2 + n2], n1 < N1
@@ -334,7 +337,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
         const int idx = n1 * N2 + n2;
         x = load_in<T>(a, in, base0 + (int64_t)t * st + (int64_t)idx * sa);
         if (a.kmul) {
-          const T k = (T)(kconst ? kf : a.kmul[idx]);
+          const T k = kconst ? (T)kf : kl[idx];
           x = cpx<T>{-k * x.y, k * x.x};  // i k x
         }
       }
@@ -848,7 +851,7 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
 }
 
 // One workgroup: the six sums over the per-block partials side by side (one reduction),
-// the 24 corner reads of dF issued together.
+// the 24 corner reads of dF issued together (no real-space correction when dF is null).
 template <class TF>
 __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
                                                        const TF* __restrict__ dFx, const TF* __restrict__ dFy,
@@ -861,7 +864,7 @@ __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double*
     const int c = threadIdx.x & 7, comp = threadIdx.x >> 3;
     const TF* dF = comp == 0 ? dFx : comp == 1 ? dFy : dFz;
     const int32_t o = idx[c];
-    dfw[comp][c] = o >= 0 ? w[c] * (double)dF[o].x : 0.0;
+    dfw[comp][c] = o >= 0 && dF ? w[c] * (double)dF[o].x : 0.0;
   }
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblk; b += 256)
@@ -987,6 +990,21 @@ extern "C" int pss_fft_line(const pss_grid* g, const double* in, double* out, in
                             void* stream) {
   int rc = launch_pass(g, (const cplx*)in, (cplx*)out, axis, dir, nullptr, -1, (hipStream_t)stream);
   if (rc) return pss_set_msg(2, "pss_fft_line: bad axis length");
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
+                           const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
+                           void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
+  hipLaunchKernelGGL(k_ksample, dim3(nblk), dim3(256), lds, s, *g, (const cplx*)Fx, (const cplx*)Fy, (const cplx*)Fz,
+                     sample_idx, sample_w, partial);
+  hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
+                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

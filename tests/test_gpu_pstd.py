@@ -287,6 +287,64 @@ def test_captured_step_matches_eager(precision):
     assert np.linalg.norm(a[1]) > 0 and np.linalg.norm(a[6]) > 0
 
 
+@pytest.mark.parametrize("precision,feedback,steps,tol", [("fp64", False, 12, 1e-12), ("mixed", False, 12, 1e-6),
+                                                          ("fp64", True, 8, 1e-12)])
+def test_pipelined_run_matches_eager(precision, feedback, steps, tol):
+    """run_pipelined (the next step's CPML-E correction beside the wire step, graph
+    replays) against as many eager steps, in chunks of 1, 2 and the rest.  The same
+    updates are summed in another order, so agreement is to rounding, measured on each
+    field as a vector (its three components together: with the current fed back, Ey
+    grows to ~1e12 while Ex stays ~1e4, and a rounding difference of the order of Ey's
+    last bits is a large part of Ex).  FP64 1e-12; mixed: the point sample is taken from
+    the field after the FP32 correction instead of before it plus the correction, 1e-6."""
+    import torch
+
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.rhoPJ import QWArray
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    N, nk, dx, epsr = 64, 64, 20e-9, 9.1
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+
+    def make():
+        sp = ss(3, N, N, N, dx, dx, dx, epsr)
+        qa = QWArray(QWParams(L=L_of_nk(nk)), MBFlags(Screened=False, ignorewire=not feedback), Nw=1, Emax0=0.0,
+                     precision=precision)
+        prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), ps(800e-9, 3e8, 5e-15, 4e-15, 0.0), source_type="soft",
+                                boundary_type="cpml", calc_j=qa, npml=8, precision=precision, device_time=True)
+        return prop, qa
+
+    def snap(prop, qa):
+        torch.cuda.synchronize()
+        st = qa.wires.get_state(0)
+        return [np.stack([getattr(prop, F + c).cpu().numpy() for c in "xyz"]) for F in "EB"] + \
+               [st["p2"], st["C2"], qa.jl.cpu().numpy(), qa.sample.cpu().numpy()]
+
+    pe, qe = make()
+    pp, qp = make()
+    assert pp.pipelined()
+    for _ in range(2):
+        pe.step()
+        pp.step()
+    done = 0
+    for chunk in (1, 2, steps - 3):
+        for _ in range(chunk):
+            pe.step()
+        pp.run_pipelined(chunk)
+        done += chunk
+        a, b = snap(pe, qe), snap(pp, qp)
+        assert all(bool(np.isfinite(x).all()) for x in b)
+        for i, (x, y) in enumerate(zip(a, b)):
+            assert rel(y, x) <= tol, f"after {done} steps: array {i} differs by {rel(y, x):.3e}"
+    assert pe.time.n == pp.time.n and abs(pe.time.t - pp.time.t) <= 1e-12 * abs(pe.time.t)
+    assert np.linalg.norm(a[0]) > 0 and np.linalg.norm(a[3]) > 0 and np.linalg.norm(a[5]) > 0
+    if feedback:
+        assert np.linalg.norm(a[4]) > 0
+
+
 def test_cpml_128_matches_oracle():
     """One CPML E and one CPML B half step on a 128^3 grid (the bench grid) with
     12-cell layers: every FFT kernel path the bench uses, with the i*k premultiply."""
