@@ -94,6 +94,12 @@ int pss_sbe_tune(int32_t ksplit);
  */
 int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
                  int32_t max_nrr, void* stream);
+/* The same step for a batch whose wires hold only the precisions in the mask
+ * `precisions` (1: PSS_FP64 wires, 2: PSS_MIXED wires, 3: both): the kernels of an absent
+ * precision are not launched (pss_sbe_step launches all of them; each returns at once for
+ * wires of the other precision). */
+int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+                   int32_t max_nrr, int32_t precisions, void* stream);
 
 /*
  * Uniform drive of batched wires: Exx/Eyy/Ezz of every wire are set along the wire to
@@ -163,6 +169,17 @@ int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, doub
                   int32_t src_comp, const double* src_par, const double* srcx, const double* rx,
                   const double* ry, const double* rz, const int32_t* sample_idx, const double* sample_w,
                   double* sample_out, void* stream);
+
+/* The B half step of a leapfrog step, UpdateB3D followed by ApplyCPML_B
+ * (PSTD3D/PSTD3D.py:79-113 and PSTD3D/cpml.py:575-638, as PSTD3D/PSTD3D.py:523-627 calls
+ * them): B -= i dt (k x E), then the CPML-B correction with the soft source on src_comp.
+ * Both updates read only E, so the curl term is added in the last pass of the
+ * correction's forward transform when that axis is 64 ... 1024 long (one read-modify-
+ * write of B instead of two); other lengths run the two updates one after the other.
+ * Arguments as for pss_update_b3d and pss_cpml_half (which = 1, no sample). */
+int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz, const double* Ex, const double* Ey,
+               const double* Ez, double* psi, double* scratch, int32_t src_comp, const double* src_par,
+               const double* srcx, const double* rx, const double* ry, const double* rz, void* stream);
 
 /* The trilinear point sample of pss_cpml_half taken on its own, from k-space fields that
  * already hold the whole half step (EAtXYZ of the IFFT, PSTD3D/usefulsubs.py:2016-2085):

@@ -279,13 +279,15 @@ class PSTD3DPropagator:
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ensure_side()
         self._ev_e.record(cur)
+        # recorded first, the field branch is the one the graph starts at once (the other
+        # branch's queue starts later); it is the longer one here
+        self.step_b_half(stream)
+        if pre_next:
+            self._cpml_e(stream)
         self._side.wait_event(self._ev_e)
         with torch.cuda.stream(self._side):
             self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
         self._ev_j.record(self._side)
-        self.step_b_half(stream)
-        if pre_next:
-            self._cpml_e(stream)
         cur.wait_event(self._ev_j)
 
     def pipelined(self):
@@ -342,12 +344,9 @@ class PSTD3DPropagator:
         """B update + CPML-B half step (soft source on Bz), then t += dt/2: the part of a
         step after the calc_j hook."""
         tm = self.time
-        P = _lib.ptr
         st = _lib.stream_ptr(stream)
-        _lib.check(_lib.lib.pss_update_b3d(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(self.Bx), P(self.By),
-                                           P(self.Bz), st), "pss_update_b3d")
-        self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, dt=tm.dt,
-                              src=self._src(2, self.pulse.Amp / self.v, tm.t), stream=stream)
+        self.cpml.UpdateB_ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, tm.dt,
+                                      src=self._src(2, self.pulse.Amp / self.v, tm.t), stream=stream)
         self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
         tm.n += 1

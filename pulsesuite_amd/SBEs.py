@@ -205,8 +205,9 @@ class DeviceWires:
         sz = C.sizeof(_lib.WireDesc)
         ptr = C.c_void_p(self.d_descs.data_ptr() + lo * sz)
         sub = self.models[lo:hi]
-        _lib.check(_lib.lib.pss_sbe_step(ptr, hi - lo, max(m.Nk for m in sub), max(m.Nr for m in sub),
-                                         max(self.nrr[lo:hi]), _lib.stream_ptr(stream)), "pss_sbe_step")
+        _lib.check(_lib.lib.pss_sbe_step_p(ptr, hi - lo, max(m.Nk for m in sub), max(m.Nr for m in sub),
+                                           max(self.nrr[lo:hi]), precision_mask(self.prec_code),
+                                           _lib.stream_ptr(stream)), "pss_sbe_step_p")
 
     # ------------------------------------------------------------ state access
     def _cur(self, i):
@@ -240,6 +241,11 @@ class DeviceWires:
                  "rho": 2 * Nr * 2, "er": 3 * Nr}
         o = lay[name]
         return self.ws[i][o:o + sizes[name]]
+
+
+def precision_mask(prec_code):
+    """pss_sbe_step_p's mask for a batch of wires of one precision."""
+    return 2 if prec_code == _lib.MIXED else 1
 
 
 def ws_layout(Nk, Nr, nrr):

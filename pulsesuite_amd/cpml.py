@@ -46,6 +46,17 @@ class CPML:
                                           _lib.ptr(rx), _lib.ptr(ry), _lib.ptr(rz), _lib.ptr(sidx), _lib.ptr(sw),
                                           _lib.ptr(sout), _lib.stream_ptr(stream)), "pss_cpml_half")
 
+    def UpdateB_ApplyCPML_B(self, Bx, By, Bz, Ex, Ey, Ez, dt, src=None, stream=None):
+        """B -= i dt (k x E) and ApplyCPML_B in one call (pss_b_half: the curl term rides on
+        the correction's last forward pass)."""
+        g = self.ctx.g
+        g.dt = dt
+        comp, par, srcx, rx, ry, rz = src if src is not None else (-1, None, None, None, None, None)
+        P = _lib.ptr
+        _lib.check(_lib.lib.pss_b_half(self.ctx.gp, P(Bx), P(By), P(Bz), P(Ex), P(Ey), P(Ez), P(self.psi_B),
+                                       P(self.scratch), comp, par, P(srcx), P(rx), P(ry), P(rz),
+                                       _lib.stream_ptr(stream)), "pss_b_half")
+
     def ApplyCPML_E(self, Ex, Ey, Ez, Bx, By, Bz, kx=None, ky=None, kz=None, v2=None, dt=None, **kw):
         """PSTD3D/cpml.py:640-702 (kx, ky, kz are implied by the grid)."""
         self._half(0, (Ex, Ey, Ez), (Bx, By, Bz), self.ctx.dt if dt is None else dt, v2, **kw)

@@ -39,6 +39,30 @@ PSS_INLINE double block_sum(double v, double* sh) {
   return r;
 }
 
+// K sums at once, each by the same tree as block_sum (bit-identical results), one set of
+// barriers; sh holds K * NT / 64 doubles.
+template <int NT, int K>
+PSS_INLINE void block_sum_n(double* v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < K; ++q) sh[q * (NT / 64) + wid] = v[q];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double r = 0.0;
+    for (int i = 0; i < NT / 64; ++i) r += sh[threadIdx.x * (NT / 64) + i];
+    sh[K * (NT / 64) + threadIdx.x] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = sh[K * (NT / 64) + q];
+  __syncthreads();
+}
+
 template <int NT>
 PSS_INLINE double block_max(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));

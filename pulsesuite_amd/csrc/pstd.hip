@@ -98,6 +98,12 @@ struct PassArgs {
   int accum;  // 0: out = result, 1: out += result (F_k += FFT(dF)), 2: real part to ((double*)out)[i]
   int slab;   // > 0 (two-level passes only): write only positions k < slab or k >= N - slab along the
               // axis, into an array whose axis length is 2 slab (k -> k, or k - (N - 2 slab))
+  // accum 3 (two-level passes, complex128 out): out_c += result + ccoef i (q x G)_c, the
+  // spectral curl update of the other field G fused into the accumulating last pass
+  // (batch element c = component); q from the grid's wavenumber tables
+  const cplx *cg0, *cg1, *cg2;
+  const double *qx, *qy, *qz;
+  double ccoef;
 };
 
 template <int N, int MODE>
@@ -234,7 +240,8 @@ __device__ __forceinline__ cpx<T> load_in(const PassArgs& a, const cplx* in, int
   return {(T)v.x, (T)v.y};
 }
 
-// MODE 0: out = v, MODE 1: out += v (complex128), MODE 2: out = Re v (real array)
+// MODE 0: out = v, MODE 1: out += v (complex128), MODE 2: out = Re v (real array);
+// MODE 3 (out += v + curl term) is stored by the pass itself
 template <int MODE, class T>
 __device__ __forceinline__ void store_out(const PassArgs& a, cplx* out, int64_t o, cpx<T> v) {
   if (MODE == 1) {
@@ -270,10 +277,12 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   cplx* out = bz == 0 ? a.out : bz == 1 ? a.out1 : a.out2;
   const int P2 = SLAB ? 2 * a.slab : 0;
   int64_t obase0, ost, osa;  // output geometry (differs from the input's only for slab output)
+  int64_t lb = 0;            // first line of the block (axis 2)
   if (a.axis == 2) {
     const int64_t nlines = (int64_t)Nx * Ny;
     const int64_t l0 = (int64_t)blockIdx.x * TB;
     if (l0 >= nlines) return;
+    lb = l0;
     nvalid = (int)min((int64_t)TB, nlines - l0);
     base0 = l0 * N;
     sa = 1;
@@ -370,7 +379,29 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
           kk = k < a.slab ? k : k - (N - P2);
         }
         const int64_t o = obase0 + (int64_t)t * ost + (int64_t)kk * osa;
-        store_out<MODE, T>(a, out, o, u[k2] * (T)a.scale);
+        if (MODE == 3) {
+          int ix, iy, iz;
+          if (a.axis == 2) {
+            const int64_t l = lb + t;
+            ix = (int)(l / Ny);
+            iy = (int)(l % Ny);
+            iz = k;
+          } else if (a.axis == 1) {
+            ix = cx; iy = k; iz = cz0 + t;
+          } else {
+            ix = k; iy = cy; iz = cz0 + t;
+          }
+          // component c = bz: (q x G)_c = q_{c+1} G_{c+2} - q_{c+2} G_{c+1}
+          const double q0 = a.qx[ix], q1 = a.qy[iy], q2 = a.qz[iz];
+          const double qa = bz == 0 ? q1 : bz == 1 ? q2 : q0, qb = bz == 0 ? q2 : bz == 1 ? q0 : q1;
+          const cplx* ga = bz == 0 ? a.cg1 : bz == 1 ? a.cg2 : a.cg0;
+          const cplx* gb = bz == 0 ? a.cg2 : bz == 1 ? a.cg0 : a.cg1;
+          const cplx cr = qa * gb[o] - qb * ga[o];
+          const cpx<T> v = u[k2] * (T)a.scale;
+          out[o] = out[o] + (mk((double)v.x, (double)v.y) + mk(-cr.y, cr.x) * a.ccoef);
+        } else {
+          store_out<MODE, T>(a, out, o, u[k2] * (T)a.scale);
+        }
       }
     }
   }
@@ -397,19 +428,27 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s, int nb) {
     return;
   }
   if (a.in32 || a.out32) {  // FP32 arithmetic whenever either side is single precision
-    if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, float>), gr, bl, 0, s, a);
+    if (a.accum == 3) hipLaunchKernelGGL((k_fft2<N1, N2, 3, float>), gr, bl, 0, s, a);
+    else if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, float>), gr, bl, 0, s, a);
     else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float>), gr, bl, 0, s, a);
     else hipLaunchKernelGGL((k_fft2<N1, N2, 0, float>), gr, bl, 0, s, a);
-  } else if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, double>), gr, bl, 0, s, a);
+  } else if (a.accum == 3) hipLaunchKernelGGL((k_fft2<N1, N2, 3, double>), gr, bl, 0, s, a);
+  else if (a.accum == 1) hipLaunchKernelGGL((k_fft2<N1, N2, 1, double>), gr, bl, 0, s, a);
   else if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, double>), gr, bl, 0, s, a);
   else hipLaunchKernelGGL((k_fft2<N1, N2, 0, double>), gr, bl, 0, s, a);
 }
+
+// Curl update fused into an accumulating last pass (accum 3): out_c += ... + coef i (q x G)_c.
+struct CurlArgs {
+  const cplx* G[3];
+  double coef;
+};
 
 // One pass along `axis` over nb <= 3 arrays of the same geometry (ins[b] -> outs[b]):
 // one launch for the two-level sizes, one launch per array otherwise.
 int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
                   const double* kmul, int mul_axis, hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0,
-                  int slab = 0);
+                  int slab = 0, const CurlArgs* cu = nullptr);
 
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
                 hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0, int slab = 0) {
@@ -417,9 +456,12 @@ int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir,
 }
 
 int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
-                  const double* kmul, int mul_axis, hipStream_t s, int accum, int in32, int out32, int slab) {
+                  const double* kmul, int mul_axis, hipStream_t s, int accum, int in32, int out32, int slab,
+                  const CurlArgs* cu) {
   const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
   if (nb < 1 || nb > 3) return -1;
+  if (accum == 3 && (!cu || nb != 3 || !(N == 64 || N == 128 || N == 256 || N == 512 || N == 1024)))
+    return -4;  // the fused curl update: the three components, two-level lengths only
   if (N == 1) {
     for (int b = 0; b < nb; ++b)
       if (ins[b] != outs[b]) return -1;
@@ -451,6 +493,15 @@ int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, 
   a.in32 = in32;
   a.out32 = out32;
   a.slab = slab;
+  if (cu) {
+    a.cg0 = cu->G[0]; a.cg1 = cu->G[1]; a.cg2 = cu->G[2];
+    a.qx = g->kx; a.qy = g->ky; a.qz = g->kz;
+    a.ccoef = cu->coef;
+  } else {
+    a.cg0 = a.cg1 = a.cg2 = nullptr;
+    a.qx = a.qy = a.qz = nullptr;
+    a.ccoef = 0.0;
+  }
   switch (N) {
     case 64: launch_fft2<8, 8>(a, g, s, nb); return 0;
     case 128: launch_fft2<8, 16>(a, g, s, nb); return 0;
@@ -501,7 +552,7 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
 // complex64 and the passes compute in FP32 (out stays complex128).
 // nb arrays at once (tmps[b] -> outs[b]), one launch per axis for the two-level sizes.
 int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, int dir, hipStream_t s,
-             int tmp32 = 0) {
+             int tmp32 = 0, const CurlArgs* cu = nullptr) {
   int last = -1;
   for (int axis = 0; axis < 3; ++axis)
     if ((axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz) > 1) last = axis;
@@ -511,8 +562,8 @@ int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, in
     if (N == 1) continue;
     const bool fin = axis == last;
     const cplx* ins[3] = {tmps[0], nb > 1 ? tmps[1] : nullptr, nb > 2 ? tmps[2] : nullptr};
-    const int rc = launch_pass_b(g, ins, fin ? outs : tmps, nb, axis, dir, nullptr, -1, s, fin ? 1 : 0, tmp32,
-                                 fin ? 0 : tmp32);
+    const int rc = launch_pass_b(g, ins, fin ? outs : tmps, nb, axis, dir, nullptr, -1, s, fin ? (cu ? 3 : 1) : 0,
+                                 tmp32, fin ? 0 : tmp32, 0, fin ? cu : nullptr);
     if (rc) return rc;
   }
   return 0;
@@ -1021,12 +1072,12 @@ extern "C" int pss_fft_line_batch(const pss_grid* g, double* const* in, double* 
   return 0;
 }
 
-extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
-                             const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
-                             const double* src_par, const double* srcx, const double* rx, const double* ry,
-                             const double* rz, const int32_t* sample_idx, const double* sample_w,
-                             double* sample_out, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+// cu: the curl update of F from G, fused into the last forward pass (null: none)
+static int cpml_half_impl(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
+                   const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
+                   const double* src_par, const double* srcx, const double* rx, const double* ry, const double* rz,
+                   const int32_t* sample_idx, const double* sample_w, double* sample_out, const CurlArgs* cu,
+                   hipStream_t s) {
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   if (g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
     return pss_set_msg(2, "pss_cpml_half: axis length must be a power of two <= 2048");
@@ -1113,11 +1164,40 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
   {
     cplx* tmps[3];
     for (int c = 0; c < 3; ++c) tmps[c] = fp32 ? (cplx*)(dFf + c * n) : dF + c * n;
-    rc = fft3_acc(g, tmps, F, 3, -1, s, fp32 ? 1 : 0);
+    rc = fft3_acc(g, tmps, F, 3, -1, s, fp32 ? 1 : 0, cu);
     if (rc) return pss_set_msg(2, "pss_cpml_half: fft");
   }
   PSS_CHECK(hipGetLastError());
   return 0;
+}
+
+extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
+                             const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
+                             const double* src_par, const double* srcx, const double* rx, const double* ry,
+                             const double* rz, const int32_t* sample_idx, const double* sample_w,
+                             double* sample_out, void* stream) {
+  return cpml_half_impl(g, which, Fx, Fy, Fz, Gx, Gy, Gz, psi, scratch, src_comp, src_par, srcx, rx, ry, rz,
+                        sample_idx, sample_w, sample_out, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz, const double* Ex, const double* Ey,
+                          const double* Ez, double* psi, double* scratch, int32_t src_comp, const double* src_par,
+                          const double* srcx, const double* rx, const double* ry, const double* rz, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  int last = 0;
+  for (int a = 0; a < 3; ++a)
+    if ((a == 0 ? g->Nx : a == 1 ? g->Ny : g->Nz) > 1) last = a;
+  const int N = last == 0 ? g->Nx : last == 1 ? g->Ny : g->Nz;
+  const bool fuse = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  if (!fuse) {
+    const int rc = pss_update_b3d(g, Ex, Ey, Ez, Bx, By, Bz, stream);
+    if (rc) return rc;
+    return cpml_half_impl(g, 1, Bx, By, Bz, Ex, Ey, Ez, psi, scratch, src_comp, src_par, srcx, rx, ry, rz, nullptr,
+                          nullptr, nullptr, nullptr, s);
+  }
+  const CurlArgs cu{{(const cplx*)Ex, (const cplx*)Ey, (const cplx*)Ez}, -g->dt};
+  return cpml_half_impl(g, 1, Bx, By, Bz, Ex, Ey, Ez, psi, scratch, src_comp, src_par, srcx, rx, ry, rz, nullptr,
+                        nullptr, nullptr, &cu, s);
 }
 
 __global__ void k_add_scalar(double* x, double v) {

@@ -1883,7 +1883,7 @@ __global__ void __launch_bounds__(256) k_pol_rho(const pss_wire_desc* __restrict
 __global__ void __launch_bounds__(320) k_rho_spline(const pss_wire_desc* __restrict__ descs) {
   const pss_wire_desc& d = descs[blockIdx.x];
   if (!wire_on(d)) return;
-  __shared__ double sh[16];
+  __shared__ double sh[4 * 5 + 4];
   const int Nk = d.Nk, Nr = d.Nr;
   const WS L = ws_layout(Nk, Nr, d.nrr);
   double* ws = d.ws;
@@ -1892,40 +1892,46 @@ __global__ void __launch_bounds__(320) k_rho_spline(const pss_wire_desc* __restr
   double* y5 = ws + L.pr;  // Px, Py, Pz already here; re, rh appended below as real arrays
   if (lf) {
     const double dr = d.r[1] - d.r[0];
-    for (int s = 0; s < 2; ++s) {
-      cplx* R = rho + s * Nr;
-      double edge = 0.0;
-      for (int j = threadIdx.x; j < 10; j += 320) edge += R[j].x + R[Nr - 10 + j].x;
-      edge = block_sum<320>(edge, sh);
-      const double sub = edge / 20.0;
-      double sa = 0.0;
-      for (int j = threadIdx.x; j < Nr; j += 320) {
-        R[j].x -= sub;
-        sa += cabs(R[j]);
-      }
-      sa = block_sum<320>(sa, sh);
-      double ntot = 0.0;
-      for (int k = threadIdx.x; k < Nk; k += 320) ntot += fabs(ws[L.n3 + s * Nk + k]);
-      ntot = block_sum<320>(ntot, sh) + 1e-50;
-      const double sc = ntot / (sa * dr + 1e-100);
-      for (int j = threadIdx.x; j < Nr; j += 320) {
-        cplx v = R[j] * sc;
-        v = mk(2.0 * kE0 * v.x / d.area * d.ehint, 2.0 * kE0 * v.y / d.area * d.ehint);
-        R[j] = v;
-      }
+    cplx* R0 = rho;
+    cplx* R1 = rho + Nr;
+    // the two species side by side: edge values and population totals, then the sums
+    // of |rho - edge mean|, then (after normalisation) the plasmonic totals -- three
+    // reductions of several values instead of eight
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < 10; j += 320) {
+      v[0] += R0[j].x + R0[Nr - 10 + j].x;
+      v[1] += R1[j].x + R1[Nr - 10 + j].x;
     }
-    __syncthreads();
-    // QW2Prop plasmonic step: re = |re|, normalise both to the common total
-    double s0 = 0.0, s1 = 0.0;
+    for (int k = threadIdx.x; k < Nk; k += 320) {
+      v[2] += fabs(ws[L.n3 + k]);
+      v[3] += fabs(ws[L.n3 + Nk + k]);
+    }
+    block_sum_n<320, 4>(v, sh);
+    const double sub0 = v[0] / 20.0, sub1 = v[1] / 20.0;
+    const double nt0 = v[2] + 1e-50, nt1 = v[3] + 1e-50;
+    double w[2] = {0.0, 0.0};
     for (int j = threadIdx.x; j < Nr; j += 320) {
-      const double a = cabs(rho[j]), b = cabs(rho[Nr + j]);
+      R0[j].x -= sub0;
+      R1[j].x -= sub1;
+      w[0] += cabs(R0[j]);
+      w[1] += cabs(R1[j]);
+    }
+    block_sum_n<320, 2>(w, sh);
+    const double sc0 = nt0 / (w[0] * dr + 1e-100), sc1 = nt1 / (w[1] * dr + 1e-100);
+    // QW2Prop plasmonic step: re = |re|, normalise both to the common total
+    double t[2] = {0.0, 0.0};
+    for (int j = threadIdx.x; j < Nr; j += 320) {
+      cplx v0 = R0[j] * sc0, v1 = R1[j] * sc1;
+      v0 = mk(2.0 * kE0 * v0.x / d.area * d.ehint, 2.0 * kE0 * v0.y / d.area * d.ehint);
+      v1 = mk(2.0 * kE0 * v1.x / d.area * d.ehint, 2.0 * kE0 * v1.y / d.area * d.ehint);
+      const double a = cabs(v0), b = cabs(v1);
       ws[L.spb + 3 * Nr + j] = a;  // temporary
       ws[L.spb + 4 * Nr + j] = b;
-      s0 += a;
-      s1 += b;
+      t[0] += a;
+      t[1] += b;
     }
-    s0 = block_sum<320>(s0, sh);
-    s1 = block_sum<320>(s1, sh);
+    block_sum_n<320, 2>(t, sh);
+    const double s0 = t[0], s1 = t[1];
     const double total = (s0 * dr + s1 * dr + 1e-100) / 2.0;
     const double f0 = total / (s0 * dr + 1e-100), f1 = total / (s1 * dr + 1e-100);
     for (int j = threadIdx.x; j < Nr; j += 320) {
@@ -2094,9 +2100,12 @@ extern "C" int pss_sbe_tune(int32_t ksplit) {
   return 0;
 }
 
-extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
-                            int32_t max_nrr, void* stream) {
+extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+                              int32_t max_nrr, int32_t precisions, void* stream) {
   if (nwires <= 0) return 0;
+  if (precisions < 1 || precisions > 3)
+    return pss_set_msg(2, "pss_sbe_step_p: precisions is a mask of 1 (FP64 wires) and 2 (mixed wires)");
+  const bool any64 = precisions & 1, anymx = precisions & 2;
   if (max_Nk < 4 || max_Nk > 4096 || max_Nr > 8194 || max_nrr < 4)
     return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
   hipStream_t s = (hipStream_t)stream;
@@ -2122,16 +2131,22 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   {
     const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
     const size_t lds = (size_t)max_Nk * sizeof(double);
-    if ((int64_t)tn * tm * 3 * nwires >= 512)
-      hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), lds, s, d_descs);
-    else
-      hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires), dim3(256),
-                         lds, s, d_descs);
+    // each kernel below returns at once for wires of the other precision; launched only
+    // when the batch holds wires of its own
+    if (any64) {
+      if ((int64_t)tn * tm * 3 * nwires >= 512)
+        hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), lds, s, d_descs);
+      else
+        hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires),
+                           dim3(256), lds, s, d_descs);
+    }
     // mixed wires: max |p|, |C|, |D| -> exchange planes -> exchange GEMM
     const int np = np64(max_Nk);
-    hipLaunchKernelGGL(k_maxabs, dim3(64, 3, nwires), dim3(256), 0, s, d_descs, 0);
-    hipLaunchKernelGGL(k_xsplit, dim3(np / 16, np / 64 + 1, 3 * nwires), dim3(256), 0, s, d_descs);
-    hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
+    if (anymx) {
+      hipLaunchKernelGGL(k_maxabs, dim3(64, 3, nwires), dim3(256), 0, s, d_descs, 0);
+      hipLaunchKernelGGL(k_xsplit, dim3(np / 16, np / 64 + 1, 3 * nwires), dim3(256), 0, s, d_descs);
+      hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
+    }
   }
   {
     const int np = np64(max_Nk), t64 = np / 64;
@@ -2143,10 +2158,12 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
       while (ks < ksplit_max(max_Nk) && tiles * ks < 1024) ks *= 2;
     }
     ks = std::min(ks, ksplit_max(max_Nk));
-    hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
-    hipLaunchKernelGGL(k_maxabs, dim3(64, 5, nwires), dim3(256), 0, s, d_descs, 3);  // Heh .. hbar De
-    hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
-    hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    if (any64) hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    if (anymx) {
+      hipLaunchKernelGGL(k_maxabs, dim3(64, 5, nwires), dim3(256), 0, s, d_descs, 3);  // Heh .. hbar De
+      hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
+      hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    }
     if (ks > 1) {
       const int64_t nn = (int64_t)max_Nk * max_Nk;
       hipLaunchKernelGGL(k_rhs_epi, dim3((unsigned)((nn + 255) / 256), 3, nwires), dim3(256), 0, s, d_descs, ks);
@@ -2161,6 +2178,11 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   hipLaunchKernelGGL(k_qw2prop, dim3(nwires), dim3(256), 0, s, d_descs);
   PSS_CHECK(hipGetLastError());
   return 0;
+}
+
+extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+                            int32_t max_nrr, void* stream) {
+  return pss_sbe_step_p(d_descs, nwires, max_Nk, max_Nr, max_nrr, 3, stream);
 }
 
 extern "C" int pss_drive_uniform(const pss_wire_desc* d_descs, const double* d_pulses, int32_t nwires,

@@ -39,6 +39,7 @@ import numpy as np
 
 from . import _lib
 from .batch import balanced_ranges, wire_bytes
+from .SBEs import precision_mask
 from .wire_model import L_of_nk, MBFlags, QWParams
 
 # 1-D line: 1024 cells of 10 nm with 16-cell CPML.  On an Nx x 1 x 1 grid the
@@ -168,6 +169,7 @@ class Server:
         self.budget = int(hbm_budget)
         self.max_slots = max_slots
         self.precision = precision
+        self._pmask = precision_mask(_lib.MIXED if precision == "mixed" else _lib.FP64)
         self.mb = mb_flags or MBFlags()
         self.buckets = buckets
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -228,8 +230,8 @@ class Server:
             sub = self.active[lo:hi]
             m = [s.qa.model for s in sub]
             st = _lib.stream_ptr(self._pool[b % ns] if ns > 1 else main)
-            _lib.check(_lib.lib.pss_sbe_step(C.c_void_p(base + lo * sz), hi - lo, max(x.Nk for x in m),
-                                             max(x.Nr for x in m), NX, st), "pss_sbe_step")
+            _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(base + lo * sz), hi - lo, max(x.Nk for x in m),
+                                               max(x.Nr for x in m), NX, self._pmask, st), "pss_sbe_step_p")
         if ns > 1:
             for j in range(ns):
                 self._ev_join[j].record(self._pool[j])
