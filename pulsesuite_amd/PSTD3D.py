@@ -270,10 +270,15 @@ class PSTD3DPropagator:
         tm = self.time
         P = _lib.ptr
         st = _lib.stream_ptr(stream)
-        self._update_e(stream)
+        g = self.ctx.g
+        g.dt = tm.dt
+        g.v2 = self.v2
+        jl, gy, gz = self.calc_j.jsep()
         sidx, sw, sout = self.calc_j.sample_args()
-        _lib.check(_lib.lib.pss_ksample(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(sidx), P(sw),
-                                        P(self.cpml.scratch), P(sout), st), "pss_ksample")
+        # E update and the point sample of the new E in one pass over E
+        _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, P(self.Bx), P(self.By), P(self.Bz), None, None, None,
+                                                  P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), P(sidx),
+                                                  P(sw), P(self.cpml.scratch), P(sout), st), "pss_update_e3d_sample")
         self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)

@@ -69,6 +69,7 @@ def _load():
         "pss_sbe_step_p": (C.c_int, [_vp, _i, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
+        "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp]),
         "pss_update_b3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 6 + [_vp]),
         "pss_fft3d": (C.c_int, [C.POINTER(Grid), _vp, _i, _vp]),
         "pss_fft_line": (C.c_int, [C.POINTER(Grid), _vp, _vp, _i, _i, _vp]),
@@ -95,7 +96,7 @@ def _load():
 
 lib = _load()
 
-EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_sbe_step_p", "pss_vint", "pss_update_e3d", "pss_update_b3d",
+EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_sbe_step_p", "pss_vint", "pss_update_e3d", "pss_update_e3d_sample", "pss_update_b3d",
            "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_ksample", "pss_b_half", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 

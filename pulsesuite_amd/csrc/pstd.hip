@@ -14,19 +14,14 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 // ------------------------------------------------- spectral curl updates
-// PSTD3D/PSTD3D.py:29-76
-__global__ void __launch_bounds__(256) k_update_e(pss_grid g, const cplx* __restrict__ Bx, const cplx* __restrict__ By,
-                                                  const cplx* __restrict__ Bz, const cplx* __restrict__ Jx,
-                                                  const cplx* __restrict__ Jy, const cplx* __restrict__ Jz,
-                                                  const cplx* __restrict__ jl, const cplx* __restrict__ gy,
-                                                  const cplx* __restrict__ gz, cplx* Ex, cplx* Ey, cplx* Ez) {
-  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
-  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (o >= n) return;
-  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
-  const int iz = (int)(o & (g.Nz - 1));
-  const int iy = (int)((o >> lz) & (g.Ny - 1));
-  const int ix = (int)(o >> (lz + ly));
+// PSTD3D/PSTD3D.py:29-76, at cell o = (ix, iy, iz); the new values are returned too
+__device__ __forceinline__ void update_e_cell(const pss_grid& g, int64_t o, int ix, int iy, int iz,
+                                              const cplx* __restrict__ Bx, const cplx* __restrict__ By,
+                                              const cplx* __restrict__ Bz, const cplx* __restrict__ Jx,
+                                              const cplx* __restrict__ Jy, const cplx* __restrict__ Jz,
+                                              const cplx* __restrict__ jl, const cplx* __restrict__ gy,
+                                              const cplx* __restrict__ gz, cplx* Ex, cplx* Ey, cplx* Ez, cplx& ex,
+                                              cplx& ey, cplx& ez) {
   const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
   const double coeff = g.v2 * g.dt;
   const cplx bx = Bx[o], by = By[o], bz = Bz[o];
@@ -41,9 +36,28 @@ __global__ void __launch_bounds__(256) k_update_e(pss_grid g, const cplx* __rest
   }
   const cplx cx = qy * bz - qz * by, cy = qz * bx - qx * bz, cz = qx * by - qy * bx;
   // i * c * coeff - mu0 * J * coeff
-  Ex[o] = Ex[o] + (mk(-cx.y, cx.x) * coeff - (g.mu0 * jx) * coeff);
-  Ey[o] = Ey[o] + (mk(-cy.y, cy.x) * coeff - (g.mu0 * jy) * coeff);
-  Ez[o] = Ez[o] + (mk(-cz.y, cz.x) * coeff - (g.mu0 * jz) * coeff);
+  ex = Ex[o] + (mk(-cx.y, cx.x) * coeff - (g.mu0 * jx) * coeff);
+  ey = Ey[o] + (mk(-cy.y, cy.x) * coeff - (g.mu0 * jy) * coeff);
+  ez = Ez[o] + (mk(-cz.y, cz.x) * coeff - (g.mu0 * jz) * coeff);
+  Ex[o] = ex;
+  Ey[o] = ey;
+  Ez[o] = ez;
+}
+
+__global__ void __launch_bounds__(256) k_update_e(pss_grid g, const cplx* __restrict__ Bx, const cplx* __restrict__ By,
+                                                  const cplx* __restrict__ Bz, const cplx* __restrict__ Jx,
+                                                  const cplx* __restrict__ Jy, const cplx* __restrict__ Jz,
+                                                  const cplx* __restrict__ jl, const cplx* __restrict__ gy,
+                                                  const cplx* __restrict__ gz, cplx* Ex, cplx* Ey, cplx* Ez) {
+  const int64_t n = (int64_t)g.Nx * g.Ny * g.Nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int lz = __builtin_ctz(g.Nz), ly = __builtin_ctz(g.Ny);  // axis lengths are powers of two
+  const int iz = (int)(o & (g.Nz - 1));
+  const int iy = (int)((o >> lz) & (g.Ny - 1));
+  const int ix = (int)(o >> (lz + ly));
+  cplx ex, ey, ez;
+  update_e_cell(g, o, ix, iy, iz, Bx, By, Bz, Jx, Jy, Jz, jl, gy, gz, Ex, Ey, Ez, ex, ey, ez);
 }
 
 // PSTD3D/PSTD3D.py:79-113
@@ -842,17 +856,19 @@ __device__ inline cplx twp(const cplx* tw, int N, int r) {  // exp(+2 pi i r / N
   return r < N / 2 ? conj(tw[r]) : mk(-tw[r - N / 2].x, tw[r - N / 2].y);
 }
 
-__global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restrict__ Fx, const cplx* __restrict__ Fy,
-                                                 const cplx* __restrict__ Fz, const int32_t* __restrict__ idx,
-                                                 const double* __restrict__ w, double* partial) {
-  // Phi(k) = sum_c w_c exp(+2 pi i (mx cx/Nx + my cy/Ny + mz cz/Nz)).  The axis lengths are
-  // powers of two, so with L = max(Nx, Ny, Nz) the phase is exp(+2 pi i r/L) with the exact
-  // integer r = (mx cx L/Nx + my cy L/Ny + mz cz L/Nz) mod L: one twiddle of the longest
-  // axis per corner instead of a product of three.
-  extern __shared__ cplx tws[];  // L/2 twiddles exp(-2 pi i m/L)
-  __shared__ double red[6][4];
-  __shared__ int cx[8], cy[8], cz[8];
-  __shared__ double cw[8];
+// Phi(k) = sum_c w_c exp(+2 pi i (mx cx/Nx + my cy/Ny + mz cz/Nz)).  The axis lengths are
+// powers of two, so with L = max(Nx, Ny, Nz) the phase is exp(+2 pi i r/L) with the exact
+// integer r = (mx cx L/Nx + my cy L/Ny + mz cz L/Nz) mod L: one twiddle of the longest
+// axis per corner instead of a product of three.
+struct SampleLDS {
+  int cx[8], cy[8], cz[8];  // corner coordinates pre-scaled by L/N of their axis
+  double cw[8];
+  double red[6][4];
+};
+
+// tws: L/2 twiddles exp(-2 pi i m/L) (dynamic LDS); ends with a barrier
+__device__ inline int sample_prep(const pss_grid& g, const int32_t* __restrict__ idx, const double* __restrict__ w,
+                                  cplx* tws, SampleLDS& sl) {
   const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
   const int L = max(Nx, max(Ny, Nz));
   const double* twL = L == Nx ? g.twx : L == Ny ? g.twy : g.twz;
@@ -860,45 +876,89 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
   if (threadIdx.x < 8) {
     const int64_t o = idx[threadIdx.x];
     const bool ok = o >= 0;
-    // corner coordinates pre-scaled by L/N of their axis
-    cx[threadIdx.x] = ok ? (int)(o / ((int64_t)Ny * Nz)) * (L / Nx) : 0;
-    cy[threadIdx.x] = ok ? (int)((o / Nz) % Ny) * (L / Ny) : 0;
-    cz[threadIdx.x] = ok ? (int)(o % Nz) * (L / Nz) : 0;
-    cw[threadIdx.x] = ok ? w[threadIdx.x] : 0.0;
+    sl.cx[threadIdx.x] = ok ? (int)(o / ((int64_t)Ny * Nz)) * (L / Nx) : 0;
+    sl.cy[threadIdx.x] = ok ? (int)((o / Nz) % Ny) * (L / Ny) : 0;
+    sl.cz[threadIdx.x] = ok ? (int)(o % Nz) * (L / Nz) : 0;
+    sl.cw[threadIdx.x] = ok ? w[threadIdx.x] : 0.0;
   }
   __syncthreads();
+  return L;
+}
+
+__device__ __forceinline__ cplx sample_phi(int L, const cplx* tws, const SampleLDS& sl, int mx, int my, int mz) {
+  cplx phi = mk(0, 0);
+  for (int c = 0; c < 8; ++c) {
+    if (sl.cw[c] == 0.0) continue;
+    if (L == 1) {
+      phi.x += sl.cw[c];
+      continue;
+    }
+    const int r = (mx * sl.cx[c] + my * sl.cy[c] + mz * sl.cz[c]) & (L - 1);
+    phi = phi + sl.cw[c] * twp(tws, L, r);
+  }
+  return phi;
+}
+
+// the block's three complex sums -> partial[blockIdx.x * 6 + q]
+__device__ inline void sample_reduce(cplx a0, cplx a1, cplx a2, SampleLDS& sl, double* partial) {
+  double v[6] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y};
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < 6; ++q) {
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    if (lane == 0) sl.red[q][wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    partial[(int64_t)blockIdx.x * 6 + threadIdx.x] =
+        ((sl.red[threadIdx.x][0] + sl.red[threadIdx.x][1]) + sl.red[threadIdx.x][2]) + sl.red[threadIdx.x][3];
+}
+
+__global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restrict__ Fx, const cplx* __restrict__ Fy,
+                                                 const cplx* __restrict__ Fz, const int32_t* __restrict__ idx,
+                                                 const double* __restrict__ w, double* partial) {
+  extern __shared__ cplx tws[];
+  __shared__ SampleLDS sl;
+  const int L = sample_prep(g, idx, w, tws, sl);
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
   const int64_t n = (int64_t)Nx * Ny * Nz;
   const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);
   cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
   for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
     const cplx f0 = Fx[o], f1 = Fy[o], f2 = Fz[o];
-    const int mz = (int)(o & (Nz - 1));
-    const int my = (int)((o >> lz) & (Ny - 1));
-    const int mx = (int)(o >> (lz + ly));
-    cplx phi = mk(0, 0);
-    for (int c = 0; c < 8; ++c) {
-      if (cw[c] == 0.0) continue;
-      if (L == 1) {
-        phi.x += cw[c];
-        continue;
-      }
-      const int r = (mx * cx[c] + my * cy[c] + mz * cz[c]) & (L - 1);
-      phi = phi + cw[c] * twp(tws, L, r);
-    }
+    const cplx phi = sample_phi(L, tws, sl, (int)(o >> (lz + ly)), (int)((o >> lz) & (Ny - 1)), (int)(o & (Nz - 1)));
     a0 = a0 + f0 * phi;
     a1 = a1 + f1 * phi;
     a2 = a2 + f2 * phi;
   }
-  double v[6] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y};
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int q = 0; q < 6; ++q) {
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
-    if (lane == 0) red[q][wid] = v[q];
+  sample_reduce(a0, a1, a2, sl, partial);
+}
+
+// The E update with the point sample of the new E (partial sums as k_ksample): one pass
+// over E instead of two.
+__global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx* __restrict__ Bx,
+                                                         const cplx* __restrict__ By, const cplx* __restrict__ Bz,
+                                                         const cplx* __restrict__ Jx, const cplx* __restrict__ Jy,
+                                                         const cplx* __restrict__ Jz, const cplx* __restrict__ jl,
+                                                         const cplx* __restrict__ gy, const cplx* __restrict__ gz,
+                                                         cplx* Ex, cplx* Ey, cplx* Ez, const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ w, double* partial) {
+  extern __shared__ cplx tws[];
+  __shared__ SampleLDS sl;
+  const int L = sample_prep(g, idx, w, tws, sl);
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  const int64_t n = (int64_t)Nx * Ny * Nz;
+  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);
+  cplx a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
+  for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
+    const int iz = (int)(o & (Nz - 1)), iy = (int)((o >> lz) & (Ny - 1)), ix = (int)(o >> (lz + ly));
+    cplx e0, e1, e2;
+    update_e_cell(g, o, ix, iy, iz, Bx, By, Bz, Jx, Jy, Jz, jl, gy, gz, Ex, Ey, Ez, e0, e1, e2);
+    const cplx phi = sample_phi(L, tws, sl, ix, iy, iz);
+    a0 = a0 + e0 * phi;
+    a1 = a1 + e1 * phi;
+    a2 = a2 + e2 * phi;
   }
-  __syncthreads();
-  if (threadIdx.x < 6)
-    partial[(int64_t)blockIdx.x * 6 + threadIdx.x] =
-        ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+  sample_reduce(a0, a1, a2, sl, partial);
 }
 
 // One workgroup: the six sums over the per-block partials side by side (one reduction),
@@ -1016,6 +1076,25 @@ extern "C" int pss_update_e3d(const pss_grid* g, const double* Bx, const double*
                      (const cplx*)Bx, (const cplx*)By, (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy,
                      (const cplx*)Jz, (const cplx*)jline, (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey,
                      (cplx*)Ez);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
+                                     const double* Jx, const double* Jy, const double* Jz, const double* jline,
+                                     const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
+                                     const int32_t* sample_idx, const double* sample_w, double* partial,
+                                     double* sample_out, void* stream) {
+  if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d_sample: axis lengths must be powers of two");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
+  hipLaunchKernelGGL(k_update_e_sample, dim3(nblk), dim3(256), lds, s, *g, (const cplx*)Bx, (const cplx*)By,
+                     (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy, (const cplx*)Jz, (const cplx*)jline,
+                     (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial);
+  hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
+                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

@@ -72,6 +72,65 @@ def test_fft_line_batch_matches_numpy(shape, axis, nb):
         _lib.check(_lib.lib.pss_fft_line_batch(ctx.gp, bad, bad, 4, axis, -1, _lib.stream_ptr()), "batch")
 
 
+@pytest.mark.parametrize("shape", [(32, 64, 16), (128, 1, 1), (16, 16, 256)])
+def test_update_e_sample_and_ksample(shape):
+    """pss_update_e3d_sample: E as pss_update_e3d (the same per-cell update, to the last
+    bits), and a point sample equal to pss_ksample of the new E and to numpy's IFFT of it at the
+    eight trilinear corners (weights of a point inside the grid; -1 marks unused corners
+    on length-1 axes)."""
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd._grid import GridContext
+    from pulsesuite_amd.typespace import ss
+
+    Nx, Ny, Nz = shape
+    ctx = GridContext(ss(3, Nx, Ny, Nz, 2e-8, 2e-8, 2e-8, 9.1), 1e-17, with_cpml=False)
+    ctx.g.dt, ctx.g.v2 = 1e-17, 2.99792458e8**2 / 9.1
+    rng = np.random.default_rng(3)
+    E = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    B = [1e-8 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)]
+    jl = rng.standard_normal((3, Nx)) + 1j * rng.standard_normal((3, Nx))
+    gy, gz = rng.standard_normal(Ny) + 0j, rng.standard_normal(Nz) + 0j
+    # trilinear corners of a point (x, y, z) = (i0 + fx, j0 + fy, k0 + fz)
+    f = [0.3, 0.6, 0.8]
+    base = [n // 3 for n in shape]
+    idx, wts = [], []
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                ok = all(shape[d] > 1 or s == 0 for d, s in enumerate((a, b, c)))
+                w = np.prod([(f[d] if s else 1 - f[d]) if shape[d] > 1 else 1.0 for d, s in enumerate((a, b, c))])
+                if ok:
+                    i, j, k = base[0] + a, base[1] + b, base[2] + c
+                    idx.append((i * Ny + j) * Nz + k)
+                    wts.append(w)
+                else:
+                    idx.append(-1)
+                    wts.append(0.0)
+    dev = torch.device("cuda")
+    t_idx = torch.tensor(idx, dtype=torch.int32, device=dev)
+    t_w = torch.tensor(wts, dtype=torch.float64, device=dev)
+    part = torch.zeros(6144, dtype=torch.float64, device=dev)
+    out1 = torch.zeros(3, dtype=torch.complex128, device=dev)
+    out2 = torch.zeros(3, dtype=torch.complex128, device=dev)
+    dB, djl, dgy, dgz = [cu(a) for a in B], cu(jl), cu(gy), cu(gz)
+    E1, E2 = [cu(a) for a in E], [cu(a) for a in E]
+    P, st = _lib.ptr, _lib.stream_ptr()
+    _lib.check(_lib.lib.pss_update_e3d(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
+                                       *(P(t) for t in E1), st), "e")
+    _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
+                                              *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), st), "es")
+    _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E1), P(t_idx), P(t_w), P(part), P(out2), st), "ks")
+    torch.cuda.synchronize()
+    for a, b in zip(E1, E2):
+        assert rel(b, a.cpu().numpy()) < 1e-15
+    assert rel(out1, out2.cpu().numpy()) < 1e-14
+    real = [np.fft.ifftn(t.cpu().numpy()) for t in E1]
+    ref = [sum(w * r.reshape(-1)[o] for o, w in zip(idx, wts) if o >= 0) for r in real]
+    assert rel(out1, np.array(ref)) < 1e-12
+
+
 def test_update_e_b_match_oracle():
     from pulsesuite_amd import PSTD3D as G
     from pulsesuite_amd.typespace import ss
