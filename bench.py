@@ -57,7 +57,9 @@ def parse():
     ap.add_argument("--batched-nk-max", type=int, default=1024)
     ap.add_argument("--batched-eager", action="store_true", help="batched run without autotune or graph capture")
     ap.add_argument("--workloads", default="coupled,batched,serving,precision256",
-                    help="comma list of coupled, batched, serving, precision256 (the JSON line needs coupled)")
+                    help="comma list of coupled, batched, serving, precision256 (the JSON line needs coupled); "
+                         "alone: traffic (kernels for the HBM-counter runs), ceiling (one FFT pass against "
+                         "plain in-place / copy kernels of the same size)")
     ap.add_argument("--p256-steps", type=int, default=1000, help="timed steps of the 256^3 / Nk=1024 precision form")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests of the serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0,
@@ -292,6 +294,54 @@ def traffic_kernels(grid, reps=3):
     print(json.dumps({"traffic_kernels": {"copy_bytes": src.numel() * 8, "fft_grid": grid, "reps": reps}}))
 
 
+def pass_ceiling(grids=(128, 256), reps=100):
+    """What bounds one FFT pass at a given size: per-axis FP64 passes (in place and out of
+    place, each axis repeated) against torch's in-place elementwise scale and out-of-place
+    copy of the same complex128 field, HIP events on the launch stream.  Bytes per launch
+    = read + write of the field."""
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd._grid import GridContext
+    from pulsesuite_amd.typespace import ss
+
+    def timed(fn):
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(5):
+            fn()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return 1e3 * e0.elapsed_time(e1) / reps
+
+    out = {}
+    for grid in grids:
+        ctx = GridContext(ss(3, grid, grid, grid, 20e-9, 20e-9, 20e-9, 9.1), 1e-17, with_cpml=False, device="cuda")
+        a = torch.ones((grid, grid, grid), dtype=torch.complex128, device="cuda")
+        b = torch.empty_like(a)
+        nbytes = 2 * a.numel() * 16
+        r = {"bytes_per_launch": nbytes}
+        r["torch_inplace_scale_us"] = timed(lambda: a.mul_(1.0))
+        r["torch_copy_us"] = timed(lambda: b.copy_(a))
+        for ax in range(3):
+            r[f"pass{ax}_inplace_us"] = timed(lambda: _lib.check(_lib.lib.pss_fft_line(
+                ctx.gp, _lib.ptr(a), _lib.ptr(a), ax, -1, _lib.stream_ptr()), "pass"))
+            r[f"pass{ax}_out_of_place_us"] = timed(lambda: _lib.check(_lib.lib.pss_fft_line(
+                ctx.gp, _lib.ptr(a), _lib.ptr(b), ax, -1, _lib.stream_ptr()), "pass"))
+        r["fft3d_pass_avg_us"] = timed(lambda: _lib.check(_lib.lib.pss_fft3d(
+            ctx.gp, _lib.ptr(a), -1, _lib.stream_ptr()), "fft")) / 3
+        for k in [k for k in r if k.endswith("_us")]:
+            r[k.replace("_us", "_TBps")] = round(nbytes / (r[k] * 1e-6) / 1e12, 3)
+            r[k] = round(r[k], 2)
+        out[f"{grid}^3"] = r
+        del a, b, ctx
+        torch.cuda.empty_cache()
+    print(json.dumps({"pass_ceiling": out}))
+
+
 def cpu_baseline(grid, nk, npml, steps, feedback=False):
     """The FP64 NumPy oracle on the same workload for `steps` steps (bounded sample)."""
     from oracle import pstd as OP
@@ -326,6 +376,9 @@ def main():
     wl = set(a.workloads.split(","))
     if "traffic" in wl:  # kernels for the HBM-counter runs: a plain copy of known size, then 3-D FFTs
         traffic_kernels(a.grid, a.traffic_reps)
+        return
+    if "ceiling" in wl:  # one FFT pass against plain in-place / copy kernels of the same size
+        pass_ceiling()
         return
     if "coupled" not in wl:  # partial runs (profiling): no headline value
         line = {"metric": METRIC, "value": None, "note": "coupled workload skipped"}
