@@ -97,9 +97,12 @@ int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, i
 /* The same step for a batch whose wires hold only the precisions in the mask
  * `precisions` (1: PSS_FP64 wires, 2: PSS_MIXED wires, 3: both): the kernels of an absent
  * precision are not launched (pss_sbe_step launches all of them; each returns at once for
- * wires of the other precision). */
+ * wires of the other precision).  ksplit_cap > 0 (a power of two <= 16) caps the split-K
+ * factor of the contractions, which is otherwise chosen to fill the chip: a step that
+ * runs beside other work (the wire step of the coupled step) does better with fewer
+ * workgroups.  pss_sbe_step = precisions 3, no cap. */
 int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
-                   int32_t max_nrr, int32_t precisions, void* stream);
+                   int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, void* stream);
 
 /*
  * Uniform drive of batched wires: Exx/Eyy/Ezz of every wire are set along the wire to

@@ -199,14 +199,16 @@ class DeviceWires:
             self.upload_descs()
 
     # ------------------------------------------------------------ stepping
-    def step(self, wires=None, stream=None):
-        """One step of the wires in the contiguous range ``wires`` (default: all)."""
+    def step(self, wires=None, stream=None, ksplit_cap=0):
+        """One step of the wires in the contiguous range ``wires`` (default: all);
+        ksplit_cap > 0 caps the contractions' split-K factor (a step that shares the GPU
+        with other work does better with fewer workgroups)."""
         lo, hi = (0, self.Nw) if wires is None else wires
         sz = C.sizeof(_lib.WireDesc)
         ptr = C.c_void_p(self.d_descs.data_ptr() + lo * sz)
         sub = self.models[lo:hi]
         _lib.check(_lib.lib.pss_sbe_step_p(ptr, hi - lo, max(m.Nk for m in sub), max(m.Nr for m in sub),
-                                           max(self.nrr[lo:hi]), precision_mask(self.prec_code),
+                                           max(self.nrr[lo:hi]), precision_mask(self.prec_code), ksplit_cap,
                                            _lib.stream_ptr(stream)), "pss_sbe_step_p")
 
     # ------------------------------------------------------------ state access

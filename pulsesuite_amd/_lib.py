@@ -66,7 +66,7 @@ def _load():
         "pss_sbe_tune": (C.c_int, [_i]),
         "pss_cpml_psi_len": (C.c_int64, [C.POINTER(Grid)]),
         "pss_sbe_step": (C.c_int, [_vp, _i, _i, _i, _i, _vp]),
-        "pss_sbe_step_p": (C.c_int, [_vp, _i, _i, _i, _i, _i, _vp]),
+        "pss_sbe_step_p": (C.c_int, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
         "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp]),

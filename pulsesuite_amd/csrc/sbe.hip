@@ -2101,10 +2101,12 @@ extern "C" int pss_sbe_tune(int32_t ksplit) {
 }
 
 extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
-                              int32_t max_nrr, int32_t precisions, void* stream) {
+                              int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, void* stream) {
   if (nwires <= 0) return 0;
   if (precisions < 1 || precisions > 3)
     return pss_set_msg(2, "pss_sbe_step_p: precisions is a mask of 1 (FP64 wires) and 2 (mixed wires)");
+  if (ksplit_cap < 0 || ksplit_cap > 16 || (ksplit_cap & (ksplit_cap - 1)))
+    return pss_set_msg(2, "pss_sbe_step_p: ksplit_cap must be 0 (none) or a power of two <= 16");
   const bool any64 = precisions & 1, anymx = precisions & 2;
   if (max_Nk < 4 || max_Nk > 4096 || max_Nr > 8194 || max_nrr < 4)
     return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
@@ -2158,6 +2160,7 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
       while (ks < ksplit_max(max_Nk) && tiles * ks < 1024) ks *= 2;
     }
     ks = std::min(ks, ksplit_max(max_Nk));
+    if (ksplit_cap > 0) ks = std::min(ks, (int)ksplit_cap);
     if (any64) hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
     if (anymx) {
       hipLaunchKernelGGL(k_maxabs, dim3(64, 5, nwires), dim3(256), 0, s, d_descs, 3);  // Heh .. hbar De
@@ -2182,7 +2185,7 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
 
 extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
                             int32_t max_nrr, void* stream) {
-  return pss_sbe_step_p(d_descs, nwires, max_Nk, max_Nr, max_nrr, 3, stream);
+  return pss_sbe_step_p(d_descs, nwires, max_Nk, max_Nr, max_nrr, 3, 0, stream);
 }
 
 extern "C" int pss_drive_uniform(const pss_wire_desc* d_descs, const double* d_pulses, int32_t nwires,

@@ -116,7 +116,9 @@ class QWArray:
     def __call__(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
         """calc_j hook: wires step on the sampled field, J line -> k-space factors."""
         self.drive(space, time)
-        self.wires.step()
+        # the hook runs beside the field half steps: a split-K of at most 4 keeps the
+        # contractions' workgroups from crowding the field passes (measured at Nk = 256)
+        self.wires.step(ksplit_cap=4)
         self.deposit(space, time)
 
     def drive(self, space, time):

@@ -231,7 +231,7 @@ class Server:
             m = [s.qa.model for s in sub]
             st = _lib.stream_ptr(self._pool[b % ns] if ns > 1 else main)
             _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(base + lo * sz), hi - lo, max(x.Nk for x in m),
-                                               max(x.Nr for x in m), NX, self._pmask, st), "pss_sbe_step_p")
+                                               max(x.Nr for x in m), NX, self._pmask, 0, st), "pss_sbe_step_p")
         if ns > 1:
             for j in range(ns):
                 self._ev_join[j].record(self._pool[j])
