@@ -465,7 +465,10 @@ def test_b_half_matches_update_then_cpml(shape, fused, precision):
     """pss_b_half (B -= i dt k x E added in the correction's last forward pass when that
     axis is 64 ... 1024 long) against pss_update_b3d followed by pss_cpml_half, with the
     soft source on Bz, two half steps (psi carried): the same sums in another order,
-    FP64 rounding (1e-13 on B as a vector).  The non-fused lengths take the two calls."""
+    FP64 rounding (1e-13 on B as a vector).  Mixed: the correction's last pass is an FP32
+    transform in both, but in two different kernels (plain and curl-adding), whose FP32
+    butterflies the compiler may contract differently: FP32 rounding of the correction,
+    1e-7 on B.  The non-fused lengths take the two calls."""
     import torch
 
     from pulsesuite_amd import _lib
@@ -501,7 +504,7 @@ def test_b_half_matches_update_then_cpml(shape, fused, precision):
         torch.cuda.synchronize()
         out.append(np.stack([t.cpu().numpy() for t in dB]))
     assert np.isfinite(out[1]).all()
-    assert rel(out[1], out[0]) < 1e-13
+    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 1e-7)
     if not fused:  # the same two calls: bit-identical
         assert np.array_equal(out[0], out[1])
 
