@@ -273,8 +273,9 @@ __device__ __forceinline__ void store_out(const PassArgs& a, cplx* out, int64_t 
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
 // per element instead of log2(N) staged radix-2 sweeps.
-template <int N1, int N2, int MODE, class T, bool SLAB = false>
-__global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
+// by value: a reference to the kernel's by-value argument would put it in scratch memory
+template <int N1, int N2, int MODE, class T, bool SLAB>
+__device__ __forceinline__ void fft2_body(const PassArgs a, const int bz) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
   constexpr int TB0 = 256 / G, TBL = 2048 / N;
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   int64_t base0, sa, st;
   int nvalid;
   int cx = 0, cy = 0, cz0 = 0;
-  const int bz = blockIdx.y;  // batch element: a uniform choice among scalar arguments
+  // bz: batch element, a uniform choice among scalar arguments
   const cplx* in = bz == 0 ? a.in : bz == 1 ? a.in1 : a.in2;
   cplx* out = bz == 0 ? a.out : bz == 1 ? a.out1 : a.out2;
   const int P2 = SLAB ? 2 * a.slab : 0;
@@ -421,6 +422,61 @@ __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
   }
 }
 
+template <int N1, int N2, int MODE, class T, bool SLAB = false>
+__global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
+  fft2_body<N1, N2, MODE, T, SLAB>(a, blockIdx.y);
+}
+
+// Up to three passes of one length, each over up to two arrays, possibly of different
+// geometries and axes (one stage of the slab derivatives along the three axes) in one
+// launch: blockIdx.y = sub-pass, blockIdx.x as for a single pass (a sub-pass with fewer
+// workgroups than the grid's x extent lets the rest return).
+struct MultiPass {
+  PassArgs p[3];
+  int q[6], b[6];  // blockIdx.y -> (pass, batch element)
+};
+
+template <int N1, int N2, int MODE, class T, bool SLAB>
+__global__ void __launch_bounds__(256) k_fft2_multi(MultiPass m) {
+  const int e = blockIdx.y;
+  fft2_body<N1, N2, MODE, T, SLAB>(m.p[m.q[e]], m.b[e]);
+}
+
+// workgroups of one two-level pass along `axis` of grid (Nx, Ny, Nz)
+template <int N1, int N2>
+int64_t fft2_blocks(int axis, int Nx, int Ny, int Nz) {
+  constexpr int N = N1 * N2, G = N2, TB0 = 256 / G, TBL = 2048 / N;
+  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  if (axis == 2) return ((int64_t)Nx * Ny + TB - 1) / TB;
+  return (int64_t)(axis == 1 ? Nx : Ny) * ((Nz + TB - 1) / TB);
+}
+
+template <int N1, int N2>
+void launch_fft2_multi(const MultiPass& m, int np, int nsub, hipStream_t s) {
+  constexpr int N = N1 * N2, G = N2, TB0 = 256 / G, TBL = 2048 / N;
+  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  int64_t blocks = 0;
+  for (int q = 0; q < np; ++q)
+    blocks = std::max(blocks, fft2_blocks<N1, N2>(m.p[q].axis, m.p[q].Nx, m.p[q].Ny, m.p[q].Nz));
+  const dim3 gr((unsigned)blocks, (unsigned)nsub), bl(TB * G);
+  const PassArgs& a = m.p[0];  // the stage's flags are common to the sub-passes
+  if (a.slab) {
+    if (a.out32) {
+      if (a.accum == 2) hipLaunchKernelGGL((k_fft2_multi<N1, N2, 2, float, true>), gr, bl, 0, s, m);
+      else hipLaunchKernelGGL((k_fft2_multi<N1, N2, 0, float, true>), gr, bl, 0, s, m);
+    } else {
+      if (a.accum == 2) hipLaunchKernelGGL((k_fft2_multi<N1, N2, 2, double, true>), gr, bl, 0, s, m);
+      else hipLaunchKernelGGL((k_fft2_multi<N1, N2, 0, double, true>), gr, bl, 0, s, m);
+    }
+  } else if (a.in32 || a.out32) {
+    if (a.accum == 2) hipLaunchKernelGGL((k_fft2_multi<N1, N2, 2, float, false>), gr, bl, 0, s, m);
+    else hipLaunchKernelGGL((k_fft2_multi<N1, N2, 0, float, false>), gr, bl, 0, s, m);
+  } else {
+    if (a.accum == 2) hipLaunchKernelGGL((k_fft2_multi<N1, N2, 2, double, false>), gr, bl, 0, s, m);
+    else hipLaunchKernelGGL((k_fft2_multi<N1, N2, 0, double, false>), gr, bl, 0, s, m);
+  }
+}
+
 template <int N1, int N2>
 void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s, int nb) {
   constexpr int N = N1 * N2;
@@ -464,6 +520,12 @@ int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, 
                   const double* kmul, int mul_axis, hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0,
                   int slab = 0, const CurlArgs* cu = nullptr);
 
+PassArgs make_pass_args(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
+                        const double* kmul, int mul_axis, int accum, int in32, int out32, int slab,
+                        const CurlArgs* cu);
+int launch_pass_radix2(const pss_grid* g, const PassArgs& a, int N, int axis, int accum, int in32, int out32,
+                       int slab, hipStream_t s);
+
 int launch_pass(const pss_grid* g, const cplx* in, cplx* out, int axis, int dir, const double* kmul, int mul_axis,
                 hipStream_t s, int accum = 0, int in32 = 0, int out32 = 0, int slab = 0) {
   return launch_pass_b(g, &in, &out, 1, axis, dir, kmul, mul_axis, s, accum, in32, out32, slab);
@@ -489,6 +551,22 @@ int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, 
     }
     return 0;
   }
+  const PassArgs a = make_pass_args(g, ins, outs, nb, axis, dir, kmul, mul_axis, accum, in32, out32, slab, cu);
+  switch (N) {
+    case 64: launch_fft2<8, 8>(a, g, s, nb); return 0;
+    case 128: launch_fft2<8, 16>(a, g, s, nb); return 0;
+    case 256: launch_fft2<16, 16>(a, g, s, nb); return 0;
+    case 512: launch_fft2<16, 32>(a, g, s, nb); return 0;
+    case 1024: launch_fft2<32, 32>(a, g, s, nb); return 0;
+    default: break;
+  }
+  return launch_pass_radix2(g, a, N, axis, accum, in32, out32, slab, s);
+}
+
+PassArgs make_pass_args(const pss_grid* g, const cplx* const* ins, cplx* const* outs, int nb, int axis, int dir,
+                        const double* kmul, int mul_axis, int accum, int in32, int out32, int slab,
+                        const CurlArgs* cu) {
+  const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
   PassArgs a;
   a.in = ins[0];
   a.out = outs[0];
@@ -516,14 +594,12 @@ int launch_pass_b(const pss_grid* g, const cplx* const* ins, cplx* const* outs, 
     a.qx = a.qy = a.qz = nullptr;
     a.ccoef = 0.0;
   }
-  switch (N) {
-    case 64: launch_fft2<8, 8>(a, g, s, nb); return 0;
-    case 128: launch_fft2<8, 16>(a, g, s, nb); return 0;
-    case 256: launch_fft2<16, 16>(a, g, s, nb); return 0;
-    case 512: launch_fft2<16, 32>(a, g, s, nb); return 0;
-    case 1024: launch_fft2<32, 32>(a, g, s, nb); return 0;
-    default: break;
-  }
+  return a;
+}
+
+// radix-2 LDS passes for the lengths without a two-level kernel (FP64 only)
+int launch_pass_radix2(const pss_grid* g, const PassArgs& a, int N, int axis, int accum, int in32, int out32,
+                       int slab, hipStream_t s) {
   if (in32 || out32 || slab) return -3;  // single-precision and slab passes exist for N = 64 ... 1024 only
   const int TB = (2048 / N) > 0 ? (2048 / N) : 1;
   int64_t blocks;
@@ -667,8 +743,62 @@ int64_t psi_layout(const pss_grid* g, int64_t poff[6]) {
 constexpr int kDerivField[6] = {2, 1, 0, 2, 1, 0};
 constexpr int kDerivAxis[6] = {1, 2, 2, 0, 0, 1};
 
+// One stage (same dir, accum and element types) of up to three passes, each over two
+// arrays of one geometry: one launch when every pass has the same two-level length,
+// else one launch per pass.
+struct PassSpec {
+  const pss_grid* g;
+  const cplx* in[2];
+  cplx* out[2];
+  int axis;
+  const double* kmul;
+  int mul_axis;
+  int slab;
+};
+
+int launch_pass_multi(const PassSpec* sp, int ns, int dir, hipStream_t s, int accum, int in32, int out32) {
+  auto len = [&](int i) {
+    const pss_grid* g = sp[i].g;
+    return sp[i].axis == 0 ? g->Nx : sp[i].axis == 1 ? g->Ny : g->Nz;
+  };
+  const int N = len(0);
+  bool one = ns > 1 && (N == 64 || N == 128 || N == 256 || N == 512 || N == 1024);
+  for (int i = 1; i < ns; ++i)
+    if (len(i) != N) one = false;
+  if (!one) {
+    for (int i = 0; i < ns; ++i) {
+      const int rc = launch_pass_b(sp[i].g, sp[i].in, sp[i].out, 2, sp[i].axis, dir, sp[i].kmul, sp[i].mul_axis, s,
+                                   accum, in32, out32, sp[i].slab);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  MultiPass m{};
+  int e = 0;
+  for (int i = 0; i < ns; ++i) {
+    m.p[i] = make_pass_args(sp[i].g, sp[i].in, sp[i].out, 2, sp[i].axis, dir, sp[i].kmul, sp[i].mul_axis, accum,
+                            in32, out32, sp[i].slab, nullptr);
+    for (int b = 0; b < 2; ++b) {
+      m.q[e] = i;
+      m.b[e] = b;
+      ++e;
+    }
+  }
+  switch (N) {
+    case 64: launch_fft2_multi<8, 8>(m, ns, e, s); break;
+    case 128: launch_fft2_multi<8, 16>(m, ns, e, s); break;
+    case 256: launch_fft2_multi<16, 16>(m, ns, e, s); break;
+    case 512: launch_fft2_multi<16, 32>(m, ns, e, s); break;
+    default: launch_fft2_multi<32, 32>(m, ns, e, s); break;
+  }
+  return 0;
+}
+
 // The two derivatives along one axis (of two fields) share every launch: batch of 2.
-// S holds two slab arrays of at most n / 2 complex each (2 P < N).
+// When the temporaries of all axes fit in S (3 complex fields) at once, each stage of
+// the three axes (the slab pass, then the passes on the compact arrays) is one launch;
+// otherwise the axes run one after the other, reusing two slab arrays of at most n / 2
+// complex each (2 P <= N).
 int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int fp32, SlabR& sr, hipStream_t s) {
   const int len[3] = {g->Nx, g->Ny, g->Nz};
   const int pml[3] = {g->npml_x, g->npml_y, g->npml_z};
@@ -685,41 +815,82 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
     sr.off[q] = off;
     off += n / len[a] * 2 * pml[a];
   }
+  struct AxisPlan {
+    int a, P, np, order[3];
+    int64_t m;  // elements of one slab array
+    cplx* Sq[2];
+    cplx* Rq[2];
+    const cplx* Gq[2];
+    pss_grid gc;  // the slab array's geometry
+  };
+  AxisPlan ap[3];
+  int na = 0;
+  int64_t soff = 0;
   for (int a = 0; a < 3; ++a) {
     const int P = pml[a];
     if (len[a] == 1 || P == 0) continue;
+    AxisPlan& x = ap[na++];
+    x.a = a;
+    x.P = P;
     int qs[2], nq = 0;
     for (int q = 0; q < 6; ++q)
       if (kDerivAxis[q] == a) qs[nq++] = q;
-    const int64_t m = n / len[a] * 2 * P;  // elements of one slab array
-    cplx* Sq[2] = {S, S + m};
-    cplx* Rq[2];
-    const cplx* Gq[2];
+    x.m = n / len[a] * 2 * P;
+    x.Sq[0] = S + soff;
+    x.Sq[1] = S + soff + x.m;
+    soff += 2 * x.m;
     for (int h = 0; h < 2; ++h) {
-      Rq[h] = fp32 ? (cplx*)((float*)R + sr.off[qs[h]]) : (cplx*)((double*)R + sr.off[qs[h]]);
-      Gq[h] = G[kDerivField[qs[h]]];
+      x.Rq[h] = fp32 ? (cplx*)((float*)R + sr.off[qs[h]]) : (cplx*)((double*)R + sr.off[qs[h]]);
+      x.Gq[h] = G[kDerivField[qs[h]]];
     }
-    pss_grid gc = *g;  // the slab array's geometry
-    if (a == 0) gc.Nx = 2 * P;
-    else if (a == 1) gc.Ny = 2 * P;
-    else gc.Nz = 2 * P;
-    int order[3], np = 0;
-    order[np++] = a;
+    x.gc = *g;
+    if (a == 0) x.gc.Nx = 2 * P;
+    else if (a == 1) x.gc.Ny = 2 * P;
+    else x.gc.Nz = 2 * P;
+    x.np = 0;
+    x.order[x.np++] = a;
     for (int b = 0; b < 3; ++b)
-      if (b != a && len[b] > 1) order[np++] = b;
+      if (b != a && len[b] > 1) x.order[x.np++] = b;
+  }
+  bool together = na > 1 && soff <= 3 * n;
+  for (int k = 1; k < na; ++k)
+    if (ap[k].np != ap[0].np) together = false;
+  if (!together)
+    for (int k = 0; k < na; ++k) {  // one axis after the other, two reused slab arrays
+      ap[k].Sq[0] = S;
+      ap[k].Sq[1] = S + ap[k].m;
+    }
+  const int np = na ? ap[0].np : 0;
+  auto spec = [&](const AxisPlan& x, int i) {
+    const bool last = i == x.np - 1;
+    PassSpec p;
+    p.g = i == 0 ? g : &x.gc;
+    for (int h = 0; h < 2; ++h) {
+      p.in[h] = i == 0 ? x.Gq[h] : x.Sq[h];
+      p.out[h] = last ? x.Rq[h] : x.Sq[h];
+    }
+    p.axis = x.order[i];
+    p.kmul = i == 0 ? ks[x.a] : nullptr;
+    p.mul_axis = i == 0 ? x.a : -1;
+    p.slab = i == 0 ? x.P : 0;
+    return p;
+  };
+  if (together) {
     for (int i = 0; i < np; ++i) {
-      const bool last = i == np - 1;
-      cplx* const* outs = last ? Rq : Sq;
-      int rc;
-      if (i == 0)
-        rc = launch_pass_b(g, Gq, outs, 2, a, +1, ks[a], a, s, last ? 2 : 0, 0, fp32, P);
-      else {
-        const cplx* ins[2] = {Sq[0], Sq[1]};
-        rc = launch_pass_b(&gc, ins, outs, 2, order[i], +1, nullptr, -1, s, last ? 2 : 0, fp32, fp32);
-      }
+      PassSpec sp[3];
+      for (int k = 0; k < na; ++k) sp[k] = spec(ap[k], i);
+      const int rc = launch_pass_multi(sp, na, +1, s, i == np - 1 ? 2 : 0, i == 0 ? 0 : fp32, fp32);
       if (rc) return rc;
     }
+    return 0;
   }
+  for (int k = 0; k < na; ++k)
+    for (int i = 0; i < ap[k].np; ++i) {
+      const PassSpec p = spec(ap[k], i);
+      const int rc = launch_pass_b(p.g, p.in, p.out, 2, p.axis, +1, p.kmul, p.mul_axis, s, i == ap[k].np - 1 ? 2 : 0,
+                                   i == 0 ? 0 : fp32, fp32, p.slab);
+      if (rc) return rc;
+    }
   return 0;
 }
 
