@@ -147,12 +147,13 @@ int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const 
                    const double* gy, const double* gz, double* Ex, double* Ey, double* Ez, void* stream);
 /* pss_update_e3d followed by the point sample of the new E (as pss_ksample), in one
  * pass over E: partial needs 6144 doubles of device scratch, sample_out[6] = (Re, Im)
- * of each component at the wire point. */
+ * of each component at the wire point.  clock (may be null): a device scalar advanced by
+ * tick after the sample (the soft source's device time, as pss_add_scalar would). */
 int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
                           const double* Jx, const double* Jy, const double* Jz, const double* jline,
                           const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
                           const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
-                          void* stream);
+                          double* clock, double tick, void* stream);
 /* B -= i dt (k x E)  (PSTD3D/PSTD3D.py:79-113). */
 int pss_update_b3d(const pss_grid* g, const double* Ex, const double* Ey, const double* Ez,
                    double* Bx, double* By, double* Bz, void* stream);

@@ -203,7 +203,7 @@ class PSTD3DPropagator:
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
         try:
-            with torch.cuda.stream(s):
+            with _lib.gc_paused(), torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
                     self.step()
         finally:
@@ -275,11 +275,12 @@ class PSTD3DPropagator:
         g.v2 = self.v2
         jl, gy, gz = self.calc_j.jsep()
         sidx, sw, sout = self.calc_j.sample_args()
-        # E update and the point sample of the new E in one pass over E
+        # E update and the point sample of the new E in one pass over E; the sample's last
+        # kernel also advances the device clock by dt/2
         _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, P(self.Bx), P(self.By), P(self.Bz), None, None, None,
                                                   P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), P(sidx),
-                                                  P(sw), P(self.cpml.scratch), P(sout), st), "pss_update_e3d_sample")
-        self._advance(tm.dt / 2.0, st)
+                                                  P(sw), P(self.cpml.scratch), P(sout), P(self._tdev), tm.dt / 2.0,
+                                                  st), "pss_update_e3d_sample")
         tm.t += tm.dt / 2.0
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ensure_side()
@@ -314,7 +315,7 @@ class PSTD3DPropagator:
         try:
             for pre in (True, False):
                 g = torch.cuda.CUDAGraph()
-                with torch.cuda.stream(s):
+                with _lib.gc_paused(), torch.cuda.stream(s):
                     with torch.cuda.graph(g, stream=s):
                         self._pipelined_step(pre)
                 graphs.append(g)

@@ -5,7 +5,9 @@ without the built library raises immediately, and every call checks its
 status and raises RuntimeError with the library's message.
 """
 
+import contextlib
 import ctypes as C
+import gc
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -69,7 +71,7 @@ def _load():
         "pss_sbe_step_p": (C.c_int, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
-        "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp]),
+        "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp, _d, _vp]),
         "pss_update_b3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 6 + [_vp]),
         "pss_fft3d": (C.c_int, [C.POINTER(Grid), _vp, _i, _vp]),
         "pss_fft_line": (C.c_int, [C.POINTER(Grid), _vp, _vp, _i, _i, _vp]),
@@ -112,6 +114,24 @@ def ptr(t):
     if t is None:
         return None
     return C.c_void_p(t.data_ptr())
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """Run the cyclic garbage collector now and keep it off until the block ends.
+
+    Wraps every HIP graph capture: a collection that starts inside a capture runs the
+    destructors of dead graphs, events and pooled tensors (graph-exec destroy, pool
+    release) in the middle of the capture, where they are not allowed, and that has
+    aborted the process.  torch.cuda.graph no longer collects before capture by itself."""
+    was = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 
 def stream_ptr(stream=None):

@@ -166,7 +166,7 @@ class WireBatch:
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(s):
+        with _lib.gc_paused(), torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
                 self._launch(plan)
         torch.cuda.current_stream(self.device).wait_stream(s)

@@ -1138,7 +1138,8 @@ template <class TF>
 __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double* __restrict__ partial, int nblk,
                                                        const TF* __restrict__ dFx, const TF* __restrict__ dFy,
                                                        const TF* __restrict__ dFz, const int32_t* __restrict__ idx,
-                                                       const double* __restrict__ w, double* out) {
+                                                       const double* __restrict__ w, double* out,
+                                                       double* clock = nullptr, double tick = 0.0) {
   __shared__ double red[6][4];
   __shared__ double dfw[3][8];  // w_c dF_comp(x_c), 0 for unused corners
   const double invn = 1.0 / ((double)g.Nx * g.Ny * g.Nz);
@@ -1166,6 +1167,7 @@ __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double*
       for (int c = 0; c < 8; ++c) r += dfw[q >> 1][c];
     out[q] = r;
   }
+  if (clock && threadIdx.x == 0) *clock += tick;  // device clock of the soft source, if given
 }
 
 // Uniform wire line: Exx[f][x] = sample[f] for all x (QuantumWire, PSTD3D/rhoPJ.py:1160-1164).
@@ -1255,7 +1257,7 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                                      const double* Jx, const double* Jy, const double* Jz, const double* jline,
                                      const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
                                      const int32_t* sample_idx, const double* sample_w, double* partial,
-                                     double* sample_out, void* stream) {
+                                     double* sample_out, double* clock, double tick, void* stream) {
   if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d_sample: axis lengths must be powers of two");
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
@@ -1265,7 +1267,7 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                      (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy, (const cplx*)Jz, (const cplx*)jline,
                      (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial);
   hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
-                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out);
+                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out, clock, tick);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

@@ -124,7 +124,7 @@ class Solve:
         s = torch.cuda.Stream(device=prop.device)
         s.wait_stream(torch.cuda.current_stream(prop.device))
         self.g_e, self.g_b = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-        with torch.cuda.stream(s):
+        with _lib.gc_paused(), torch.cuda.stream(s):
             with torch.cuda.graph(self.g_e, stream=s):
                 self._launch_e()
             with torch.cuda.graph(self.g_b, stream=s):

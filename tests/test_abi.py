@@ -76,3 +76,23 @@ def test_cpml_half_rejects_pml_that_does_not_fit(shape, pml):
                                 None, None, None, None, None, None, None)
     assert rc != 0
     assert b"npml" in _lib.lib.pss_last_error()
+
+
+def test_gc_paused_holds_the_collector_off_and_restores_it():
+    """Graph captures run under gc_paused: the collector is off inside (no destructor of
+    a dead graph or event can run mid-capture) and back to its prior state afterwards."""
+    import gc
+
+    from pulsesuite_amd import _lib
+
+    assert gc.isenabled()
+    with _lib.gc_paused():
+        assert not gc.isenabled()
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with _lib.gc_paused():
+            pass
+        assert not gc.isenabled()
+    finally:
+        gc.enable()

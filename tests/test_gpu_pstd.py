@@ -112,6 +112,7 @@ def test_update_e_sample_and_ksample(shape):
     t_idx = torch.tensor(idx, dtype=torch.int32, device=dev)
     t_w = torch.tensor(wts, dtype=torch.float64, device=dev)
     part = torch.zeros(6144, dtype=torch.float64, device=dev)
+    clock = torch.full((1,), 2.0, dtype=torch.float64, device=dev)
     out1 = torch.zeros(3, dtype=torch.complex128, device=dev)
     out2 = torch.zeros(3, dtype=torch.complex128, device=dev)
     dB, djl, dgy, dgz = [cu(a) for a in B], cu(jl), cu(gy), cu(gz)
@@ -120,12 +121,14 @@ def test_update_e_sample_and_ksample(shape):
     _lib.check(_lib.lib.pss_update_e3d(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
                                        *(P(t) for t in E1), st), "e")
     _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
-                                              *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), st), "es")
+                                              *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), P(clock), 0.25,
+                                              st), "es")
     _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E1), P(t_idx), P(t_w), P(part), P(out2), st), "ks")
     torch.cuda.synchronize()
     for a, b in zip(E1, E2):
         assert rel(b, a.cpu().numpy()) < 1e-15
     assert rel(out1, out2.cpu().numpy()) < 1e-14
+    assert float(clock.item()) == 2.25  # the clock advanced once by the given tick
     real = [np.fft.ifftn(t.cpu().numpy()) for t in E1]
     ref = [sum(w * r.reshape(-1)[o] for o, w in zip(idx, wts) if o >= 0) for r in real]
     assert rel(out1, np.array(ref)) < 1e-12
