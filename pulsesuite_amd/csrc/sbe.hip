@@ -191,10 +191,50 @@ __device__ inline double spline_eval(const double* y, int ystride, const double*
   return yi + dx * (bi + dx * (ci + dx * di));
 }
 
+// ---------------------------------- K2: populations, screening (CalcScreenedArrays)
+__device__ void prep_body(const pss_wire_desc& d, double* sh) {
+  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const int Nk = d.Nk;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < Nk; k += 256) {
+    const cplx c = B.C2[(int64_t)k * Nk + k], e = B.D2[(int64_t)k * Nk + k];
+    ws[L.ne2 + 2 * k] = c.x; ws[L.ne2 + 2 * k + 1] = c.y;
+    ws[L.nh2 + 2 * k] = e.x; ws[L.nh2 + 2 * k + 1] = e.y;
+    s += c.x + e.x;
+  }
+  s = block_sum<256>(s, sh);
+  // PSTD3D/coulomb.py:1218-1230
+  double n1 = s / 2.0 / d.L;
+  n1 = fmin(n1, d.density_max);
+  const bool scr = d.flags & PSS_SCREENED;
+  double vmax[3] = {0.0, 0.0, 0.0};
+  for (int q = threadIdx.x; q < Nk; q += 256) {
+    double e1 = 1.0;
+    if (scr) {
+      e1 = 1.0 - d.chie[q] * 2 * log(fabs((d.qe[q] - kPi * n1) / (d.qe[q] + n1)))
+               - d.chih[q] * 2 * log(fabs((d.qh[q] - kPi * n1) / (d.qh[q] + n1)));
+    }
+    ws[L.e1d + q] = e1;
+    for (int n = 0; n < 3; ++n) {
+      const double v = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
+      ws[L.vc + n * Nk + q] = v;
+      if (q > 0) vmax[n] = fmax(vmax[n], fabs(v));
+    }
+  }
+  for (int n = 0; n < 3; ++n) vmax[n] = block_max<256>(vmax[n], sh);
+  if (threadIdx.x == 0) ws[L.red + 0] = n1;
+  if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
+  if (threadIdx.x < 3) ws[L.red + 16 + threadIdx.x] = vmax[threadIdx.x];  // max V(|q|), q != 0
+}
+
 // -------------------------------------------------- K1: Prop2QW (rr -> r)
-// Gate + three spline solves on rr of Re(E) (Prop2QW takes the real part after
-// windowing; the spline is real-linear so Re commutes with it).
-__global__ void __launch_bounds__(256) k_prop_solve(const pss_wire_desc* __restrict__ descs) {
+// Per wire, one workgroup: the wireoff gate, then K2 (populations, screening), the
+// spline solves on rr of the three propagation lines and their evaluation on the wire
+// grid r_j (window applied, real part kept: Prop2QW takes the real part after
+// windowing, and the spline is real-linear, so Re commutes with it).
+__global__ void __launch_bounds__(256) k_prop_front(const pss_wire_desc* __restrict__ descs) {
   const pss_wire_desc& d = descs[blockIdx.x];
   __shared__ double sh[16];
   double* ws = d.ws;
@@ -213,29 +253,23 @@ __global__ void __launch_bounds__(256) k_prop_solve(const pss_wire_desc* __restr
   }
   __syncthreads();
   if (!wire_on(d)) return;
+  prep_body(d, sh);
   const int wv = threadIdx.x >> 6;
   if (wv < 3) {
     const double* y = (wv == 0 ? d.Exx : wv == 1 ? d.Eyy : d.Ezz);
     spline_solve_wave(y, 2, d.nrr, d.sa_h, d.sa_t, d.sa_b, d.sa_end, ws + L.spa + wv * d.nrr);
   }
-}
-
-// Evaluate at r_j, window, keep real part.
-__global__ void __launch_bounds__(256) k_prop_eval(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.y];
-  if (!wire_on(d)) return;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= d.Nr) return;
-  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
-  double* ws = d.ws;
-  const int i = d.sa_idx[j];
-  for (int f = 0; f < 3; ++f) {
-    double v = 0.0;
-    if (i >= 0) {
-      const double* y = (f == 0 ? d.Exx : f == 1 ? d.Eyy : d.Ezz);
-      v = spline_eval(y, 2, ws + L.spa + f * d.nrr, d.sa_h, d.nrr, i, d.sa_dx[j]);
+  __syncthreads();
+  for (int j = threadIdx.x; j < d.Nr; j += 256) {
+    const int i = d.sa_idx[j];
+    for (int f = 0; f < 3; ++f) {
+      double v = 0.0;
+      if (i >= 0) {
+        const double* y = (f == 0 ? d.Exx : f == 1 ? d.Eyy : d.Ezz);
+        v = spline_eval(y, 2, ws + L.spa + f * d.nrr, d.sa_h, d.nrr, i, d.sa_dx[j]);
+      }
+      ws[L.er + f * d.Nr + j] = v * d.win[j];
     }
-    ws[L.er + f * d.Nr + j] = v * d.win[j];
   }
 }
 
@@ -271,47 +305,6 @@ __global__ void __launch_bounds__(256) k_prop_dft(const pss_wire_desc* __restric
       ws[L.eq + (f * nd + od) * 2 + 1] = -(sm * acc[2 * f + 1]) / Nr;
     }
   }
-}
-
-// ---------------------------------- K2: populations, screening (CalcScreenedArrays)
-__global__ void __launch_bounds__(256) k_prep(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.x];
-  if (!wire_on(d)) return;
-  __shared__ double sh[16];
-  const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
-  double* ws = d.ws;
-  const Bufs B = bufs(d);
-  const int Nk = d.Nk;
-  double s = 0.0;
-  for (int k = threadIdx.x; k < Nk; k += 256) {
-    const cplx c = B.C2[(int64_t)k * Nk + k], e = B.D2[(int64_t)k * Nk + k];
-    ws[L.ne2 + 2 * k] = c.x; ws[L.ne2 + 2 * k + 1] = c.y;
-    ws[L.nh2 + 2 * k] = e.x; ws[L.nh2 + 2 * k + 1] = e.y;
-    s += c.x + e.x;
-  }
-  s = block_sum<256>(s, sh);
-  // PSTD3D/coulomb.py:1218-1230
-  double n1 = s / 2.0 / d.L;
-  n1 = fmin(n1, d.density_max);
-  const bool scr = d.flags & PSS_SCREENED;
-  double vmax[3] = {0.0, 0.0, 0.0};
-  for (int q = threadIdx.x; q < Nk; q += 256) {
-    double e1 = 1.0;
-    if (scr) {
-      e1 = 1.0 - d.chie[q] * 2 * log(fabs((d.qe[q] - kPi * n1) / (d.qe[q] + n1)))
-               - d.chih[q] * 2 * log(fabs((d.qh[q] - kPi * n1) / (d.qh[q] + n1)));
-    }
-    ws[L.e1d + q] = e1;
-    for (int n = 0; n < 3; ++n) {
-      const double v = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
-      ws[L.vc + n * Nk + q] = v;
-      if (q > 0) vmax[n] = fmax(vmax[n], fabs(v));
-    }
-  }
-  for (int n = 0; n < 3; ++n) vmax[n] = block_max<256>(vmax[n], sh);
-  if (threadIdx.x == 0) ws[L.red + 0] = n1;
-  if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
-  if (threadIdx.x < 3) ws[L.red + 16 + threadIdx.x] = vmax[threadIdx.x];  // max V(|q|), q != 0
 }
 
 // ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H
@@ -1754,10 +1747,7 @@ __global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ 
 
 // --------------------- K8: exponential relaxation + renormalisation of n3
 // PSTD3D/SBEs.py:5321-5333 and 5704-5718 (diagonals of C3, D3 and of the averaged prev).
-__global__ void __launch_bounds__(256) k_pops(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.x];
-  if (!wire_on(d)) return;
-  __shared__ double sh[16];
+__device__ void pops_body(const pss_wire_desc& d, double* sh) {
   const int Nk = d.Nk;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
@@ -1802,7 +1792,8 @@ __global__ void __launch_bounds__(256) k_pops(const pss_wire_desc* __restrict__ 
   }
 }
 
-// --------------------------- K9: diagonal sums of p3, C3, conj(D3)
+// --------------------------- K9: diagonal sums of p3, C3, conj(D3); the workgroup of
+// the main diagonal runs K8 before its sums
 // QWPolarization3 / QWRho5 depend on k_e - k_h (resp. k2 - k1) only through
 // exp(i (k_e - k_h) r) with k_e - k_h = dd * dkr, so the O(Nr Nk^2) sums of
 // PSTD3D/qwoptics.py:56-82, 367-380 reduce to sums along diagonals dd.
@@ -1816,6 +1807,10 @@ __global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restri
   double* ws = d.ws;
   const Bufs B = bufs(d);
   const int dd = (int)blockIdx.x - (Nk - 1);
+  if (dd == 0) {  // K8 on the main diagonal first: its sums read the new populations
+    pops_body(d, sh);
+    __syncthreads();
+  }
   const int a0 = max(0, -dd), a1 = min(Nk - 1, Nk - 1 - dd);
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = a0 + threadIdx.x; a <= a1; a += 256) {
@@ -1880,10 +1875,7 @@ __global__ void __launch_bounds__(256) k_pol_rho(const pss_wire_desc* __restrict
 // QWRho5 edge removal + normalisation (qwoptics.py:1065-1077), the 2 e0 ehint/area
 // scale (SBEs.py:5764-5765), QW2Prop plasmonic normalisation (qwoptics.py:784-789)
 // and the five spline solves on r for QW2Prop.
-__global__ void __launch_bounds__(320) k_rho_spline(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.x];
-  if (!wire_on(d)) return;
-  __shared__ double sh[4 * 5 + 4];
+__device__ void rho_spline_body(const pss_wire_desc& d, double* sh) {
   const int Nk = d.Nk, Nr = d.Nr;
   const WS L = ws_layout(Nk, Nr, d.nrr);
   double* ws = d.ws;
@@ -1951,12 +1943,10 @@ __global__ void __launch_bounds__(320) k_rho_spline(const pss_wire_desc* __restr
 
 // Evaluate on rr, |.| and final normalisation of rho, Rho = RhoH - RhoE, zeros when
 // gated off or ignorewire; advance the step counter.
-__global__ void __launch_bounds__(256) k_qw2prop(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.x];
+__device__ void qw2prop_body(const pss_wire_desc& d, const bool on, double* sh) {
+  constexpr int NT = 320;
   const int n = d.nrr;
-  const bool on = wire_on(d);
   const bool zero = !on || (d.flags & PSS_IGNOREWIRE);
-  __shared__ double sh[16];
   const int Nr = d.Nr;
   const WS L = ws_layout(d.Nk, Nr, n);
   double* ws = d.ws;
@@ -1965,7 +1955,7 @@ __global__ void __launch_bounds__(256) k_qw2prop(const pss_wire_desc* __restrict
   if (on) {
     const double* y5 = ws + L.pr;
     const cplx* rho = (const cplx*)(ws + L.rho);
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = threadIdx.x; i < n; i += NT) {
       const int ix = d.sb_idx[i];
       for (int f = 0; f < 5; ++f) {
         double v = 0.0;
@@ -1984,12 +1974,12 @@ __global__ void __launch_bounds__(256) k_qw2prop(const pss_wire_desc* __restrict
       }
     }
   }
-  se = block_sum<256>(se, sh);
-  sh_ = block_sum<256>(sh_, sh);
+  se = block_sum<NT>(se, sh);
+  sh_ = block_sum<NT>(sh_, sh);
   const double total = ws[L.red + 1];
   const double fe = lf ? total / (se * d.drr + 1e-100) : 0.0;
   const double fh = lf ? total / (sh_ * d.drr + 1e-100) : 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += NT) {
     if (zero) {
       d.Pxx[2 * i] = 0; d.Pxx[2 * i + 1] = 0;
       d.Pyy[2 * i] = 0; d.Pyy[2 * i + 1] = 0;
@@ -2006,6 +1996,17 @@ __global__ void __launch_bounds__(256) k_qw2prop(const pss_wire_desc* __restrict
     d.Rho[2 * i + 1] = 0;
   }
   if (threadIdx.x == 0 && on) d.ctl[0] += 1;
+}
+
+// The rho spline (K10) and the QW2Prop evaluation on the propagation grid in one
+// workgroup per wire; a wire that is off (or ignorewire) writes zeros.
+__global__ void __launch_bounds__(320) k_rho_qw2prop(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  __shared__ double sh[4 * 5 + 4];
+  const bool on = wire_on(d);
+  if (on) rho_spline_body(d, sh);
+  __syncthreads();
+  qw2prop_body(d, on, sh);
 }
 
 // ------------------------------------------------------------ Vint (init)
@@ -2120,11 +2121,8 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
     PSS_CHECK(hipFuncSetAttribute((const void*)k_coll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
     lds_granted = lds_rel;
   }
-  const int nrb = (max_Nr + 255) / 256;
-  hipLaunchKernelGGL(k_prop_solve, dim3(nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_prop_eval, dim3(nrb, nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_prop_front, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_prep, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
                      (size_t)(256 + max_Nk) * sizeof(double), s, d_descs);
@@ -2174,11 +2172,9 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
   }
   hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
   hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_pops, dim3(nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_rho_spline, dim3(nwires), dim3(320), 0, s, d_descs);
-  hipLaunchKernelGGL(k_qw2prop, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_rho_qw2prop, dim3(nwires), dim3(320), 0, s, d_descs);
   PSS_CHECK(hipGetLastError());
   return 0;
 }
