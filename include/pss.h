@@ -148,7 +148,9 @@ int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const 
 /* pss_update_e3d followed by the point sample of the new E (as pss_ksample), in one
  * pass over E: partial needs 6144 doubles of device scratch, sample_out[6] = (Re, Im)
  * of each component at the wire point.  clock (may be null): a device scalar advanced by
- * tick after the sample (the soft source's device time, as pss_add_scalar would). */
+ * tick after the sample (the soft source's device time, as pss_add_scalar would).
+ * Bx, By, Bz all null: the current term only (E -= mu0 v2 dt J), for a step whose curl
+ * term was applied before by pss_cpml_e_curl. */
 int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
                           const double* Jx, const double* Jy, const double* Jz, const double* jline,
                           const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
@@ -192,6 +194,16 @@ int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, doub
 int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz, const double* Ex, const double* Ey,
                const double* Ez, double* psi, double* scratch, int32_t src_comp, const double* src_par,
                const double* srcx, const double* rx, const double* ry, const double* rz, void* stream);
+
+/* The CPML-E correction (PSTD3D/cpml.py:640-702) together with the curl term of the E
+ * update, E += i v2 dt (k x B) (PSTD3D/PSTD3D.py:29-76 without its current term): both
+ * read only B, so the curl term is added in the last forward pass of the correction when
+ * that axis is 64 ... 1024 long, else the two updates run one after the other.  The
+ * current term then follows with pss_update_e3d_sample and null B.  Arguments as for
+ * pss_update_e3d and pss_cpml_half (which = 0, no sample). */
+int pss_cpml_e_curl(const pss_grid* g, double* Ex, double* Ey, double* Ez, const double* Bx, const double* By,
+                    const double* Bz, double* psi, double* scratch, int32_t src_comp, const double* src_par,
+                    const double* srcx, const double* rx, const double* ry, const double* rz, void* stream);
 
 /* The trilinear point sample of pss_cpml_half taken on its own, from k-space fields that
  * already hold the whole half step (EAtXYZ of the IFFT, PSTD3D/usefulsubs.py:2016-2085):

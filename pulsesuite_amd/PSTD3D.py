@@ -118,6 +118,7 @@ class PSTD3DPropagator:
         # mixed: the CPML's spectral derivatives (real parts, PML cells only) in FP32;
         # the fields, their transforms and the curl updates stay FP64
         self.precision = precision
+        self._curl_early = precision == "mixed"  # pipelined runs: E curl term with the correction
         self.ctx.g.precision = _lib.MIXED if precision == "mixed" else _lib.FP64
         # device_time: the source reads the time from a device scalar advanced by a kernel
         # after each half step, so a half step can be captured once and replayed
@@ -257,15 +258,30 @@ class PSTD3DPropagator:
         self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
                               src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
 
+    def _cpml_e_next(self, stream):
+        """The next step's CPML correction of E in a pipelined run, soft source on Ey at the
+        current (host or device) time; with the curl term of its E update (both read only
+        B) in mixed precision.  In FP64 the field branch is the longer one and the curl term
+        stays in the E update: on the correction's FP64 last pass it lengthened that branch
+        by more than it took off the E update (128^3: 0.813 -> 0.851 ms per step), while in
+        mixed precision the step got shorter (0.590 -> 0.586 ms)."""
+        tm = self.time
+        if self._curl_early:
+            self.cpml.CurlE_ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, tm.dt, self.v2,
+                                        src=self._src(1, self.pulse.Amp, tm.t), stream=stream)
+        else:
+            self._cpml_e(stream)
+
     # ---- pipelined steps (graph replays of several steps)
-    # The CPML correction of an E half step reads only B, psi_E and the clock, not J, and it
-    # adds to E like the curl/current update does, so the two commute.  Applied right after
-    # the B half step of the step before, it runs while the wire step (calc_j) still works on
-    # the side stream; what waits for the wire's current is only the curl/current update and
-    # the point sample.  Same operations on the same operands in another order: results agree
-    # with eager steps to rounding.  The state between replays holds the next step's
-    # correction, so the sequence is closed: a prologue applies the first correction and the
-    # last step applies none.
+    # The CPML correction of an E half step reads only B, psi_E and the clock, not J, and so
+    # does the curl term of the E update; both add to E like the current term does, so the
+    # three commute.  Applied right after the B half step of the step before (in mixed
+    # precision with the curl term in the correction's last forward pass), they run while
+    # the wire step (calc_j) still works on the side stream; what waits for the wire's
+    # current is the rest of the E update and the point sample.  Same operations on the same operands in another order: results
+    # agree with eager steps to rounding.  The state between replays holds the next step's
+    # correction and curl term, so the sequence is closed: a prologue applies the first ones
+    # and the last step applies none.
     def _pipelined_step(self, pre_next, stream=None):
         tm = self.time
         P = _lib.ptr
@@ -275,9 +291,11 @@ class PSTD3DPropagator:
         g.v2 = self.v2
         jl, gy, gz = self.calc_j.jsep()
         sidx, sw, sout = self.calc_j.sample_args()
-        # E update and the point sample of the new E in one pass over E; the sample's last
+        # the E update (only its current term when the curl term came with the correction,
+        # before) and the point sample of the new E in one pass over E; the sample's last
         # kernel also advances the device clock by dt/2
-        _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, P(self.Bx), P(self.By), P(self.Bz), None, None, None,
+        Bp = (None, None, None) if self._curl_early else (P(self.Bx), P(self.By), P(self.Bz))
+        _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, *Bp, None, None, None,
                                                   P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), P(sidx),
                                                   P(sw), P(self.cpml.scratch), P(sout), P(self._tdev), tm.dt / 2.0,
                                                   st), "pss_update_e3d_sample")
@@ -289,7 +307,7 @@ class PSTD3DPropagator:
         # branch's queue starts later); it is the longer one here
         self.step_b_half(stream)
         if pre_next:
-            self._cpml_e(stream)
+            self._cpml_e_next(stream)
         self._side.wait_event(self._ev_e)
         with torch.cuda.stream(self._side):
             self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
@@ -335,7 +353,7 @@ class PSTD3DPropagator:
         if self._pgraphs is None:
             self.capture_pipelined()
         gp, gl = self._pgraphs
-        self._cpml_e(None)
+        self._cpml_e_next(None)
         tm = self.time
         for k in range(nsteps):
             (gp if k < nsteps - 1 else gl).replay()

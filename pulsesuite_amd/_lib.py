@@ -79,6 +79,8 @@ def _load():
         "pss_ksample": (C.c_int, [C.POINTER(Grid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_b_half": (C.c_int, [C.POINTER(Grid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_d), _vp,
                                  _vp, _vp, _vp, _vp]),
+        "pss_cpml_e_curl": (C.c_int, [C.POINTER(Grid), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_d),
+                                      _vp, _vp, _vp, _vp, _vp]),
         "pss_cpml_half": (C.c_int, [C.POINTER(Grid), _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                     C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
@@ -99,7 +101,7 @@ def _load():
 lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_sbe_step_p", "pss_vint", "pss_update_e3d", "pss_update_e3d_sample", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_ksample", "pss_b_half", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
+           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_ksample", "pss_b_half", "pss_cpml_e_curl", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 
 

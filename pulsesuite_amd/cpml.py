@@ -57,6 +57,19 @@ class CPML:
                                        P(self.scratch), comp, par, P(srcx), P(rx), P(ry), P(rz),
                                        _lib.stream_ptr(stream)), "pss_b_half")
 
+    def CurlE_ApplyCPML_E(self, Ex, Ey, Ez, Bx, By, Bz, dt, v2, src=None, stream=None):
+        """E += i v2 dt (k x B) and ApplyCPML_E in one call (pss_cpml_e_curl: the curl term
+        rides on the correction's last forward pass); the current term of the E update is
+        left to the caller."""
+        g = self.ctx.g
+        g.dt = dt
+        g.v2 = v2
+        comp, par, srcx, rx, ry, rz = src if src is not None else (-1, None, None, None, None, None)
+        P = _lib.ptr
+        _lib.check(_lib.lib.pss_cpml_e_curl(self.ctx.gp, P(Ex), P(Ey), P(Ez), P(Bx), P(By), P(Bz), P(self.psi_E),
+                                            P(self.scratch), comp, par, P(srcx), P(rx), P(ry), P(rz),
+                                            _lib.stream_ptr(stream)), "pss_cpml_e_curl")
+
     def ApplyCPML_E(self, Ex, Ey, Ez, Bx, By, Bz, kx=None, ky=None, kz=None, v2=None, dt=None, **kw):
         """PSTD3D/cpml.py:640-702 (kx, ky, kz are implied by the grid)."""
         self._half(0, (Ex, Ey, Ez), (Bx, By, Bz), self.ctx.dt if dt is None else dt, v2, **kw)

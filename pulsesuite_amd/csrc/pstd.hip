@@ -15,6 +15,8 @@ constexpr double kPi = 3.14159265358979323846;
 
 // ------------------------------------------------- spectral curl updates
 // PSTD3D/PSTD3D.py:29-76, at cell o = (ix, iy, iz); the new values are returned too
+// CURL = false leaves out the curl term (B is not read): E -= mu0 v2 dt J only.
+template <bool CURL = true>
 __device__ __forceinline__ void update_e_cell(const pss_grid& g, int64_t o, int ix, int iy, int iz,
                                               const cplx* __restrict__ Bx, const cplx* __restrict__ By,
                                               const cplx* __restrict__ Bz, const cplx* __restrict__ Jx,
@@ -24,7 +26,6 @@ __device__ __forceinline__ void update_e_cell(const pss_grid& g, int64_t o, int 
                                               cplx& ey, cplx& ez) {
   const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
   const double coeff = g.v2 * g.dt;
-  const cplx bx = Bx[o], by = By[o], bz = Bz[o];
   cplx jx = mk(0, 0), jy = mk(0, 0), jz = mk(0, 0);
   if (Jx) {
     jx = Jx[o]; jy = Jy[o]; jz = Jz[o];
@@ -34,11 +35,18 @@ __device__ __forceinline__ void update_e_cell(const pss_grid& g, int64_t o, int 
     jy = jl[g.Nx + ix] * gg;
     jz = jl[2 * g.Nx + ix] * gg;
   }
-  const cplx cx = qy * bz - qz * by, cy = qz * bx - qx * bz, cz = qx * by - qy * bx;
-  // i * c * coeff - mu0 * J * coeff
-  ex = Ex[o] + (mk(-cx.y, cx.x) * coeff - (g.mu0 * jx) * coeff);
-  ey = Ey[o] + (mk(-cy.y, cy.x) * coeff - (g.mu0 * jy) * coeff);
-  ez = Ez[o] + (mk(-cz.y, cz.x) * coeff - (g.mu0 * jz) * coeff);
+  if (CURL) {
+    const cplx bx = Bx[o], by = By[o], bz = Bz[o];
+    const cplx cx = qy * bz - qz * by, cy = qz * bx - qx * bz, cz = qx * by - qy * bx;
+    // i * c * coeff - mu0 * J * coeff
+    ex = Ex[o] + (mk(-cx.y, cx.x) * coeff - (g.mu0 * jx) * coeff);
+    ey = Ey[o] + (mk(-cy.y, cy.x) * coeff - (g.mu0 * jy) * coeff);
+    ez = Ez[o] + (mk(-cz.y, cz.x) * coeff - (g.mu0 * jz) * coeff);
+  } else {
+    ex = Ex[o] - (g.mu0 * jx) * coeff;
+    ey = Ey[o] - (g.mu0 * jy) * coeff;
+    ez = Ez[o] - (g.mu0 * jz) * coeff;
+  }
   Ex[o] = ex;
   Ey[o] = ey;
   Ez[o] = ez;
@@ -1105,7 +1113,9 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
 }
 
 // The E update with the point sample of the new E (partial sums as k_ksample): one pass
-// over E instead of two.
+// over E instead of two.  CURL = false: the current term only (the curl term applied
+// before, by pss_cpml_e_curl).
+template <bool CURL>
 __global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx* __restrict__ Bx,
                                                          const cplx* __restrict__ By, const cplx* __restrict__ Bz,
                                                          const cplx* __restrict__ Jx, const cplx* __restrict__ Jy,
@@ -1123,7 +1133,7 @@ __global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx*
   for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
     const int iz = (int)(o & (Nz - 1)), iy = (int)((o >> lz) & (Ny - 1)), ix = (int)(o >> (lz + ly));
     cplx e0, e1, e2;
-    update_e_cell(g, o, ix, iy, iz, Bx, By, Bz, Jx, Jy, Jz, jl, gy, gz, Ex, Ey, Ez, e0, e1, e2);
+    update_e_cell<CURL>(g, o, ix, iy, iz, Bx, By, Bz, Jx, Jy, Jz, jl, gy, gz, Ex, Ey, Ez, e0, e1, e2);
     const cplx phi = sample_phi(L, tws, sl, ix, iy, iz);
     a0 = a0 + e0 * phi;
     a1 = a1 + e1 * phi;
@@ -1259,11 +1269,15 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                                      const int32_t* sample_idx, const double* sample_w, double* partial,
                                      double* sample_out, double* clock, double tick, void* stream) {
   if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d_sample: axis lengths must be powers of two");
+  const bool curl = Bx != nullptr;
+  if ((By != nullptr) != curl || (Bz != nullptr) != curl)
+    return pss_set_msg(2, "pss_update_e3d_sample: B is given in full (curl term) or not at all (current only)");
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
   const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
-  hipLaunchKernelGGL(k_update_e_sample, dim3(nblk), dim3(256), lds, s, *g, (const cplx*)Bx, (const cplx*)By,
+  auto kern = curl ? k_update_e_sample<true> : k_update_e_sample<false>;
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), lds, s, *g, (const cplx*)Bx, (const cplx*)By,
                      (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy, (const cplx*)Jz, (const cplx*)jline,
                      (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial);
   hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
@@ -1449,6 +1463,30 @@ extern "C" int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz,
   }
   const CurlArgs cu{{(const cplx*)Ex, (const cplx*)Ey, (const cplx*)Ez}, -g->dt};
   return cpml_half_impl(g, 1, Bx, By, Bz, Ex, Ey, Ez, psi, scratch, src_comp, src_par, srcx, rx, ry, rz, nullptr,
+                        nullptr, nullptr, &cu, s);
+}
+
+extern "C" int pss_cpml_e_curl(const pss_grid* g, double* Ex, double* Ey, double* Ez, const double* Bx,
+                               const double* By, const double* Bz, double* psi, double* scratch, int32_t src_comp,
+                               const double* src_par, const double* srcx, const double* rx, const double* ry,
+                               const double* rz, void* stream) {
+  if (!Bx || !By || !Bz) return pss_set_msg(2, "pss_cpml_e_curl: null B");
+  hipStream_t s = (hipStream_t)stream;
+  int last = 0;
+  for (int a = 0; a < 3; ++a)
+    if ((a == 0 ? g->Nx : a == 1 ? g->Ny : g->Nz) > 1) last = a;
+  const int N = last == 0 ? g->Nx : last == 1 ? g->Ny : g->Nz;
+  const bool fuse = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  if (!fuse) {
+    // the curl term alone (no current), then the correction
+    const int rc = pss_update_e3d(g, Bx, By, Bz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, Ex, Ey, Ez,
+                                  stream);
+    if (rc) return rc;
+    return cpml_half_impl(g, 0, Ex, Ey, Ez, Bx, By, Bz, psi, scratch, src_comp, src_par, srcx, rx, ry, rz, nullptr,
+                          nullptr, nullptr, nullptr, s);
+  }
+  const CurlArgs cu{{(const cplx*)Bx, (const cplx*)By, (const cplx*)Bz}, g->v2 * g->dt};
+  return cpml_half_impl(g, 0, Ex, Ey, Ez, Bx, By, Bz, psi, scratch, src_comp, src_par, srcx, rx, ry, rz, nullptr,
                         nullptr, nullptr, &cu, s);
 }
 

@@ -116,9 +116,11 @@ class QWArray:
     def __call__(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
         """calc_j hook: wires step on the sampled field, J line -> k-space factors."""
         self.drive(space, time)
-        # the hook runs beside the field half steps: a split-K of at most 4 keeps the
-        # contractions' workgroups from crowding the field passes (measured at Nk = 256)
-        self.wires.step(ksplit_cap=4)
+        # the hook runs beside the field half steps: a split-K of at most 4 (mixed) or 8
+        # (FP64) keeps the contractions' workgroups from crowding the field passes (measured
+        # at Nk = 256, 128^3: caps 1/2/4/8 gave 0.634/0.598/0.591/0.596 ms mixed and
+        # 1.137/0.865/0.813/0.807 ms FP64 per pipelined step)
+        self.wires.step(ksplit_cap=4 if self.precision == "mixed" else 8)
         self.deposit(space, time)
 
     def drive(self, space, time):

@@ -132,6 +132,22 @@ def test_update_e_sample_and_ksample(shape):
     real = [np.fft.ifftn(t.cpu().numpy()) for t in E1]
     ref = [sum(w * r.reshape(-1)[o] for o, w in zip(idx, wts) if o >= 0) for r in real]
     assert rel(out1, np.array(ref)) < 1e-12
+    # null B: the current term only, the same as a full update with B = 0
+    zB = [torch.zeros_like(t) for t in dB]
+    E3, E4 = [cu(a) for a in E], [cu(a) for a in E]
+    _lib.check(_lib.lib.pss_update_e3d(ctx.gp, *(P(t) for t in zB), None, None, None, P(djl), P(dgy), P(dgz),
+                                       *(P(t) for t in E3), st), "e0")
+    _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, None, None, None, None, None, None, P(djl), P(dgy), P(dgz),
+                                              *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0.0,
+                                              st), "es0")
+    _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E3), P(t_idx), P(t_w), P(part), P(out2), st), "ks0")
+    torch.cuda.synchronize()
+    for a, b in zip(E3, E4):
+        assert rel(b, a.cpu().numpy()) < 1e-15
+    assert rel(out1, out2.cpu().numpy()) < 1e-14
+    assert _lib.lib.pss_update_e3d_sample(ctx.gp, P(dB[0]), None, None, None, None, None, P(djl), P(dgy), P(dgz),
+                                          *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0.0,
+                                          st) != 0  # B in part is refused
 
 
 def test_update_e_b_match_oracle():
@@ -354,8 +370,8 @@ def test_captured_step_matches_eager(precision):
                                                                 ("fp64", True, 8, 1e-12, (64, 64, 64)),
                                                                 ("mixed", False, 6, 1e-6, (128, 64, 64))])
 def test_pipelined_run_matches_eager(precision, feedback, steps, tol, shape):
-    """run_pipelined (the next step's CPML-E correction beside the wire step, graph
-    replays) against as many eager steps, in chunks of 1, 2 and the rest (one grid whose
+    """run_pipelined (the next step's CPML-E correction beside the wire step, in mixed
+    precision with the E update's curl term, graph replays) against as many eager steps, in chunks of 1, 2 and the rest (one grid whose
     axes differ: the slab-derivative stages then take one launch per axis).  The same
     updates are summed in another order, so agreement is to rounding, measured on each
     field as a vector (its three components together: with the current fed back, Ey
@@ -465,6 +481,56 @@ def test_cpml_anisotropic_matches_oracle(shape):
         cp.ApplyCPML_B(*dB, *dE, None, None, None, dt)
     for a, b in zip(dE + dB, E + B):
         assert rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("shape,fused", [((64, 64, 128), True), ((128, 64, 1), True), ((1024, 1, 1), True),
+                                         ((64, 64, 32), False), ((2048, 1, 1), False)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cpml_e_curl_matches_update_then_cpml(shape, fused, precision):
+    """pss_cpml_e_curl (E += i v2 dt k x B added in the correction's last forward pass
+    when that axis is 64 ... 1024 long) against pss_update_e3d without current followed by
+    pss_cpml_half (which = 0), soft source on Ey, two half steps (psi carried): the same
+    sums in another order, FP64 rounding (1e-13 on E as a vector; mixed 1e-7, whose FP32
+    passes round differently in the two kernels).  Other lengths take the two calls."""
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd.cpml import CPML
+    from pulsesuite_amd.PSTD3D import InitializeTFSF
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import GetXArray, GetYArray, GetZArray, ss
+
+    Nx, Ny, Nz = shape
+    dx, epsr = 20e-9, 9.1
+    v2 = (2.99792458e8 / np.sqrt(epsr)) ** 2
+    dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    rng = np.random.default_rng(12)
+    E = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    B = [1e-8 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)]
+    sp = ss(3, Nx, Ny, Nz, dx, dx, dx, epsr)
+    pulse = ps(800e-9, 3e8, 5e-15, 4e-15, 0.0)
+    dev = torch.device("cuda")
+    srcx = torch.as_tensor(InitializeTFSF(sp, pulse), device=dev)
+    r = [torch.as_tensor(f(sp), device=dev) for f in (GetXArray, GetYArray, GetZArray)]
+    par = (_lib.C.c_double * 9)(2e-15, 3e8 / 1e8, pulse.CalcOmega0(), pulse.CalcTau(), 0.0, 1e8, pulse.Tp, 0.0, dx)
+    out = []
+    for mode in ("split", "fused"):
+        cp = CPML(Nx, Ny, Nz, dx, dx, dx, dt, epsr)
+        cp.ctx.g.precision = _lib.MIXED if precision == "mixed" else _lib.FP64
+        dE, dB = [cu(a) for a in E], [cu(a) for a in B]
+        src = (1, par, srcx, *r)
+        for _ in range(2):
+            if mode == "split":
+                cp.ctx.g.dt, cp.ctx.g.v2 = dt, v2
+                _lib.check(_lib.lib.pss_update_e3d(cp.ctx.gp, *(_lib.ptr(t) for t in dB), None, None, None, None,
+                                                   None, None, *(_lib.ptr(t) for t in dE), _lib.stream_ptr()), "e")
+                cp.ApplyCPML_E(*dE, *dB, dt=dt, v2=v2, src=src)
+            else:
+                cp.CurlE_ApplyCPML_E(*dE, *dB, dt, v2, src=src)
+        torch.cuda.synchronize()
+        out.append(np.stack([t.cpu().numpy() for t in dE]))
+    assert np.isfinite(out[1]).all()
+    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 1e-7)
 
 
 @pytest.mark.parametrize("shape,fused", [((64, 64, 128), True), ((128, 64, 1), True), ((1024, 1, 1), True),
