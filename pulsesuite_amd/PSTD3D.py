@@ -295,10 +295,14 @@ class PSTD3DPropagator:
         # before) and the point sample of the new E in one pass over E; the sample's last
         # kernel also advances the device clock by dt/2
         Bp = (None, None, None) if self._curl_early else (P(self.Bx), P(self.By), P(self.Bz))
+        # the wires' input lines filled with the sample by the same kernels, when the hook
+        # keeps them in one block
+        la = self.calc_j.line_args() if hasattr(self.calc_j, "line_args") else None
+        lines, line_n, nlines = (P(la[0]), la[1], la[2]) if la is not None and la[0] is not None else (None, 0, 0)
         _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, *Bp, None, None, None,
                                                   P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), P(sidx),
-                                                  P(sw), P(self.cpml.scratch), P(sout), P(self._tdev), tm.dt / 2.0,
-                                                  st), "pss_update_e3d_sample")
+                                                  P(sw), P(self.cpml.scratch), P(sout), lines, line_n, nlines,
+                                                  P(self._tdev), tm.dt / 2.0, st), "pss_update_e3d_sample")
         tm.t += tm.dt / 2.0
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ensure_side()
@@ -310,7 +314,10 @@ class PSTD3DPropagator:
             self._cpml_e_next(stream)
         self._side.wait_event(self._ev_e)
         with torch.cuda.stream(self._side):
-            self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
+            if lines is not None:
+                self.calc_j.step_filled(self.space, self.time)
+            else:
+                self.calc_j(self.space, self.time, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
         self._ev_j.record(self._side)
         cur.wait_event(self._ev_j)
 

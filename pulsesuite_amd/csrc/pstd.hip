@@ -1149,8 +1149,10 @@ __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double*
                                                        const TF* __restrict__ dFx, const TF* __restrict__ dFy,
                                                        const TF* __restrict__ dFz, const int32_t* __restrict__ idx,
                                                        const double* __restrict__ w, double* out,
-                                                       double* clock = nullptr, double tick = 0.0) {
+                                                       double* clock = nullptr, double tick = 0.0,
+                                                       double* lines = nullptr, int line_n = 0, int nlines = 0) {
   __shared__ double red[6][4];
+  __shared__ double sv[6];
   __shared__ double dfw[3][8];  // w_c dF_comp(x_c), 0 for unused corners
   const double invn = 1.0 / ((double)g.Nx * g.Ny * g.Nz);
   if (threadIdx.x < 24) {
@@ -1176,8 +1178,18 @@ __global__ void __launch_bounds__(256) k_ksample_final(pss_grid g, const double*
     if ((q & 1) == 0)  // dF is real: the sampled correction adds to the real parts
       for (int c = 0; c < 8; ++c) r += dfw[q >> 1][c];
     out[q] = r;
+    sv[q] = r;
   }
   if (clock && threadIdx.x == 0) *clock += tick;  // device clock of the soft source, if given
+  if (lines) {  // uniform wire lines [nlines][3][line_n] = the sample (as k_fill_line)
+    __syncthreads();
+    const int64_t m = (int64_t)nlines * 3 * line_n;
+    for (int64_t i = threadIdx.x; i < m; i += 256) {
+      const int f = (int)((i / line_n) % 3);
+      lines[2 * i] = sv[2 * f];
+      lines[2 * i + 1] = sv[2 * f + 1];
+    }
+  }
 }
 
 // Uniform wire line: Exx[f][x] = sample[f] for all x (QuantumWire, PSTD3D/rhoPJ.py:1160-1164).
@@ -1267,8 +1279,10 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                                      const double* Jx, const double* Jy, const double* Jz, const double* jline,
                                      const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
                                      const int32_t* sample_idx, const double* sample_w, double* partial,
-                                     double* sample_out, double* clock, double tick, void* stream) {
+                                     double* sample_out, double* lines, int32_t line_n, int32_t nlines,
+                                     double* clock, double tick, void* stream) {
   if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d_sample: axis lengths must be powers of two");
+  if (lines && (line_n < 1 || nlines < 1)) return pss_set_msg(2, "pss_update_e3d_sample: line_n, nlines >= 1");
   const bool curl = Bx != nullptr;
   if ((By != nullptr) != curl || (Bz != nullptr) != curl)
     return pss_set_msg(2, "pss_update_e3d_sample: B is given in full (curl term) or not at all (current only)");
@@ -1281,7 +1295,8 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                      (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy, (const cplx*)Jz, (const cplx*)jline,
                      (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial);
   hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
-                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out, clock, tick);
+                     (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out, clock, tick,
+                     lines, line_n, nlines);
   PSS_CHECK(hipGetLastError());
   return 0;
 }

@@ -113,15 +113,30 @@ class QWArray:
     def jsep(self):
         return (self.jl, self.gy, self.gz)
 
+    def line_args(self):
+        """The wires' input lines as one block [Nw][3][Nx] (for a sample kernel that fills
+        them itself), with Nx and Nw."""
+        return (self.wires.E_all, self.wires.nrr[0], self.Nw)
+
+    def step_filled(self, space, time):
+        """calc_j hook when the input lines already hold the sample (filled by the sample
+        kernel): the wires step and the J line is deposited."""
+        self.wires.set_dt(time.dt)
+        self._step_wires()
+        self.deposit(space, time)
+
     def __call__(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
         """calc_j hook: wires step on the sampled field, J line -> k-space factors."""
         self.drive(space, time)
+        self._step_wires()
+        self.deposit(space, time)
+
+    def _step_wires(self):
         # the hook runs beside the field half steps: a split-K of at most 4 (mixed) or 8
         # (FP64) keeps the contractions' workgroups from crowding the field passes (measured
         # at Nk = 256, 128^3: caps 1/2/4/8 gave 0.634/0.598/0.591/0.596 ms mixed and
         # 1.137/0.865/0.813/0.807 ms FP64 per pipelined step)
         self.wires.step(ksplit_cap=4 if self.precision == "mixed" else 8)
-        self.deposit(space, time)
 
     def drive(self, space, time):
         """Wire lines <- the point sample taken during the E half step."""

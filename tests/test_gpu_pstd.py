@@ -113,6 +113,7 @@ def test_update_e_sample_and_ksample(shape):
     t_w = torch.tensor(wts, dtype=torch.float64, device=dev)
     part = torch.zeros(6144, dtype=torch.float64, device=dev)
     clock = torch.full((1,), 2.0, dtype=torch.float64, device=dev)
+    lines = torch.zeros((2, 3, Nx), dtype=torch.complex128, device=dev)
     out1 = torch.zeros(3, dtype=torch.complex128, device=dev)
     out2 = torch.zeros(3, dtype=torch.complex128, device=dev)
     dB, djl, dgy, dgz = [cu(a) for a in B], cu(jl), cu(gy), cu(gz)
@@ -121,14 +122,17 @@ def test_update_e_sample_and_ksample(shape):
     _lib.check(_lib.lib.pss_update_e3d(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
                                        *(P(t) for t in E1), st), "e")
     _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
-                                              *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), P(clock), 0.25,
-                                              st), "es")
+                                              *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), P(lines), Nx, 2,
+                                              P(clock), 0.25, st), "es")
     _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E1), P(t_idx), P(t_w), P(part), P(out2), st), "ks")
     torch.cuda.synchronize()
     for a, b in zip(E1, E2):
         assert rel(b, a.cpu().numpy()) < 1e-15
     assert rel(out1, out2.cpu().numpy()) < 1e-14
     assert float(clock.item()) == 2.25  # the clock advanced once by the given tick
+    # both wire lines hold the sample along their whole length
+    ln = lines.cpu().numpy()
+    assert np.array_equal(ln, np.broadcast_to(out1.cpu().numpy()[None, :, None], ln.shape))
     real = [np.fft.ifftn(t.cpu().numpy()) for t in E1]
     ref = [sum(w * r.reshape(-1)[o] for o, w in zip(idx, wts) if o >= 0) for r in real]
     assert rel(out1, np.array(ref)) < 1e-12
@@ -138,16 +142,16 @@ def test_update_e_sample_and_ksample(shape):
     _lib.check(_lib.lib.pss_update_e3d(ctx.gp, *(P(t) for t in zB), None, None, None, P(djl), P(dgy), P(dgz),
                                        *(P(t) for t in E3), st), "e0")
     _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, None, None, None, None, None, None, P(djl), P(dgy), P(dgz),
-                                              *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0.0,
-                                              st), "es0")
+                                              *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0, 0,
+                                              None, 0.0, st), "es0")
     _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E3), P(t_idx), P(t_w), P(part), P(out2), st), "ks0")
     torch.cuda.synchronize()
     for a, b in zip(E3, E4):
         assert rel(b, a.cpu().numpy()) < 1e-15
     assert rel(out1, out2.cpu().numpy()) < 1e-14
     assert _lib.lib.pss_update_e3d_sample(ctx.gp, P(dB[0]), None, None, None, None, None, P(djl), P(dgy), P(dgz),
-                                          *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0.0,
-                                          st) != 0  # B in part is refused
+                                          *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0, 0,
+                                          None, 0.0, st) != 0  # B in part is refused
 
 
 def test_update_e_b_match_oracle():
