@@ -81,6 +81,30 @@ PSS_INLINE double block_max(double v, double* sh) {
   return r;
 }
 
+// K maxima at once (max is exact in any order), one set of barriers; sh holds
+// K * NT / 64 + K doubles.
+template <int NT, int K>
+PSS_INLINE void block_max_n(double* v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] = fmax(v[q], __shfl_down(v[q], o, 64));
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < K; ++q) sh[q * (NT / 64) + wid] = v[q];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double r = sh[threadIdx.x * (NT / 64)];
+    for (int i = 1; i < NT / 64; ++i) r = fmax(r, sh[threadIdx.x * (NT / 64) + i]);
+    sh[K * (NT / 64) + threadIdx.x] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = sh[K * (NT / 64) + q];
+  __syncthreads();
+}
+
 #define PSS_CHECK(expr)                                      \
   do {                                                       \
     hipError_t _e = (expr);                                  \

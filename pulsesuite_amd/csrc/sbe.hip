@@ -223,19 +223,20 @@ __device__ void prep_body(const pss_wire_desc& d, double* sh) {
       if (q > 0) vmax[n] = fmax(vmax[n], fabs(v));
     }
   }
-  for (int n = 0; n < 3; ++n) vmax[n] = block_max<256>(vmax[n], sh);
+  block_max_n<256, 3>(vmax, sh);
   if (threadIdx.x == 0) ws[L.red + 0] = n1;
   if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
   if (threadIdx.x < 3) ws[L.red + 16 + threadIdx.x] = vmax[threadIdx.x];  // max V(|q|), q != 0
 }
 
 // -------------------------------------------------- K1: Prop2QW (rr -> r)
-// Per wire, one workgroup: the wireoff gate, then K2 (populations, screening), the
-// spline solves on rr of the three propagation lines and their evaluation on the wire
-// grid r_j (window applied, real part kept: Prop2QW takes the real part after
-// windowing, and the spline is real-linear, so Re commutes with it).
+// Two workgroups per wire (blockIdx.y), both taking the wireoff gate (the same sum of the
+// same lines, so the latch gets one value): x = 0 solves the splines on rr of the three
+// propagation lines and evaluates them on the wire grid r_j (window applied, real part
+// kept: Prop2QW takes the real part after windowing, and the spline is real-linear, so Re
+// commutes with it); x = 1 runs K2 (populations, screening) beside it.
 __global__ void __launch_bounds__(256) k_prop_front(const pss_wire_desc* __restrict__ descs) {
-  const pss_wire_desc& d = descs[blockIdx.x];
+  const pss_wire_desc& d = descs[blockIdx.y];
   __shared__ double sh[16];
   double* ws = d.ws;
   const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
@@ -253,7 +254,10 @@ __global__ void __launch_bounds__(256) k_prop_front(const pss_wire_desc* __restr
   }
   __syncthreads();
   if (!wire_on(d)) return;
-  prep_body(d, sh);
+  if (blockIdx.x == 1) {
+    prep_body(d, sh);
+    return;
+  }
   const int wv = threadIdx.x >> 6;
   if (wv < 3) {
     const double* y = (wv == 0 ? d.Exx : wv == 1 ? d.Eyy : d.Ezz);
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(256) k_prop_dft(const pss_wire_desc* __restric
   const int nd = 2 * d.Nk - 1;
   const int od = blockIdx.x;
   if (od >= nd) return;
-  __shared__ double sh[16];
+  __shared__ double sh[6 * 4 + 6];
   const WS L = ws_layout(d.Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const int Nr = d.Nr;
@@ -297,7 +301,7 @@ __global__ void __launch_bounds__(256) k_prop_dft(const pss_wire_desc* __restric
       acc[2 * f + 1] += v * wi;
     }
   }
-  for (int q = 0; q < 6; ++q) acc[q] = block_sum<256>(acc[q], sh);
+  block_sum_n<256, 6>(acc, sh);
   if (threadIdx.x == 0) {
     const double sm = (m & 1) ? -1.0 : 1.0;
     for (int f = 0; f < 3; ++f) {
@@ -313,7 +317,7 @@ __global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __res
   if (!wire_on(d)) return;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
   if ((int)blockIdx.x >= nd + Nk) return;
-  __shared__ double sh[16];
+  __shared__ double sh[8 * 4 + 8];
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __res
       const cplx g = B.C2[(int64_t)(k + qd) * Nk + k]; // C[k+qd, k]
       a[0] += c.x; a[1] += c.y; a[2] += e.x; a[3] += e.y; a[4] += f.x; a[5] += f.y; a[6] += g.x; a[7] += g.y;
     }
-    for (int q = 0; q < 8; ++q) a[q] = block_sum<256>(a[q], sh);
+    block_sum_n<256, 8>(a, sh);
     if (threadIdx.x == 0)
       for (int q = 0; q < 4; ++q) {
         ws[L.tr + (q * nd + blockIdx.x) * 2] = a[2 * q];
@@ -377,8 +381,10 @@ __global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __res
         gh += f * Veh2 * (nea * (1.0 - neb) * nhc + (1.0 - nea) * neb * (1.0 - nhc)) * axe;
       }
     }
-    ge = block_sum<256>(ge, sh);
-    gh = block_sum<256>(gh, sh);
+    double gg[2] = {ge, gh};
+    block_sum_n<256, 2>(gg, sh);
+    ge = gg[0];
+    gh = gg[1];
   } else {
     ge = d.gam_e;
     gh = d.gam_h;
@@ -1735,8 +1741,10 @@ __global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ 
       wout += V * (1.0 - ni) * Pik;
     }
   }
-  win = block_sum<256>(win, sh);
-  wout = block_sum<256>(wout, sh);
+  double ww[2] = {win, wout};
+  block_sum_n<256, 2>(ww, sh);
+  win = ww[0];
+  wout = ww[1];
   if (threadIdx.x == 0) {
     double* W = ws + L.W;
     const int iw = (holes ? 2 : 0) * Nk + kk, iwo = (holes ? 3 : 1) * Nk + kk;
@@ -1775,8 +1783,10 @@ __device__ void pops_body(const pss_wire_desc& d, double* sh) {
     se += ae;
     sh_ += ah;
   }
-  se = block_sum<256>(se, sh);
-  sh_ = block_sum<256>(sh_, sh);
+  double ss2[2] = {se, sh_};
+  block_sum_n<256, 2>(ss2, sh);
+  se = ss2[0];
+  sh_ = ss2[1];
   const double total = (se + sh_) / 2.0;
   for (int k = threadIdx.x; k < Nk; k += 256) {
     const int64_t o = (int64_t)k * Nk + k;
@@ -1802,7 +1812,7 @@ __global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restri
   if (!wire_on(d)) return;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
   if ((int)blockIdx.x >= nd) return;
-  __shared__ double sh[16];
+  __shared__ double sh[8 * 4 + 8];
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
@@ -1823,7 +1833,7 @@ __global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restri
     s[4] += cv.x; s[5] += cv.y;
     s[6] += dv.x; s[7] -= dv.y;
   }
-  for (int q = 0; q < 8; ++q) s[q] = block_sum<256>(s[q], sh);
+  block_sum_n<256, 8>(s, sh);
   if (threadIdx.x == 0)
     for (int q = 0; q < 4; ++q) {
       ws[L.ds + (q * nd + blockIdx.x) * 2] = s[2 * q];
@@ -1974,8 +1984,10 @@ __device__ void qw2prop_body(const pss_wire_desc& d, const bool on, double* sh) 
       }
     }
   }
-  se = block_sum<NT>(se, sh);
-  sh_ = block_sum<NT>(sh_, sh);
+  double ss2[2] = {se, sh_};
+  block_sum_n<NT, 2>(ss2, sh);
+  se = ss2[0];
+  sh_ = ss2[1];
   const double total = ws[L.red + 1];
   const double fe = lf ? total / (se * d.drr + 1e-100) : 0.0;
   const double fh = lf ? total / (sh_ * d.drr + 1e-100) : 0.0;
@@ -2121,7 +2133,7 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
     PSS_CHECK(hipFuncSetAttribute((const void*)k_coll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
     lds_granted = lds_rel;
   }
-  hipLaunchKernelGGL(k_prop_front, dim3(nwires), dim3(256), 0, s, d_descs);
+  hipLaunchKernelGGL(k_prop_front, dim3(2, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
