@@ -232,9 +232,11 @@ int pss_fill_line(const double* sample, double* lines, int32_t n, void* stream);
 
 /* Current line of the wires (PSTD3D/rhoPJ.py:1183-1194 with CalcJfx :561-592):
  * J_c = sum_w (P_cw - Pold_cw)/dt/Nw, J_x += CalcJfx(sum Rho, sum RhoOld, dt, dx)/Nw;
- * then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx], jline: [3][Nx]. */
+ * then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx], jline: [3][Nx].
+ * fft != 0: each line then takes its forward FFT along x (numpy convention, as
+ * pss_fft_line_batch with dir = -1) in the same kernel; Nx a power of two. */
 int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
-              double dt, double dx, double* jline, void* stream);
+              double dt, double dx, double* jline, int32_t fft, void* stream);
 
 /*
  * Fused stage combine of the classic RK4 step (libpulsesuite/integrator.py:143-157,

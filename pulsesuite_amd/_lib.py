@@ -84,7 +84,7 @@ def _load():
         "pss_cpml_half": (C.c_int, [C.POINTER(Grid), _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                     C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
-        "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
+        "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _vp]),
         "pss_drive_uniform": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_add_scalar": (C.c_int, [_vp, _d, _vp]),
         "pss_rk4_stage": (C.c_int, [_i, C.c_int64, _d, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -1206,7 +1206,7 @@ __global__ void k_fill_line(const double* sample, double* lines, int n) {
 // J_c(x) = sum_w (P_cw - Pold_cw)/dt / Nw, J_x += CalcJfx(sum_w Rho_w, sum_w RhoOld_w)/Nw;
 // then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx] complex.  One block.
 __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, cplx* Pold, cplx* RhoOld, int Nw,
-                                               int Nx, double dt, double dx, cplx* jl) {
+                                               int Nx, double dt, double dx, cplx* jl, int fft) {
   __shared__ cplx drho[2048];
   for (int x = threadIdx.x; x < Nx; x += 256) {
     for (int c = 0; c < 3; ++c) {
@@ -1250,6 +1250,28 @@ __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, c
   for (int x = max(x0, 1); x < x1; ++x) {
     pre = pre + drho[x];
     jl[x] = jl[x] + mk(-(pre.x * dx) / Nw, -(pre.y * dx) / Nw);
+  }
+  if (!fft || Nx == 1) return;
+  // forward FFT of each line (numpy convention, unnormalised; Nx a power of two) in LDS:
+  // bit-reversed load, then radix-2 stages, twiddles exp(-2 pi i k / len) = cospi/sinpi
+  // of -k / (len / 2) (exact arguments)
+  const int lg = __builtin_ctz(Nx);
+  for (int c = 0; c < 3; ++c) {
+    __syncthreads();  // the line is written and the LDS buffer is free
+    for (int x = threadIdx.x; x < Nx; x += 256) drho[__brev((unsigned)x) >> (32 - lg)] = jl[c * Nx + x];
+    __syncthreads();
+    for (int h = 1; h < Nx; h <<= 1) {
+      for (int t = threadIdx.x; t < Nx / 2; t += 256) {
+        const int k = t & (h - 1), i = 2 * (t - k) + k;  // butterfly k of group t / h
+        double sn, cs;
+        sincospi(-(double)k / h, &sn, &cs);
+        const cplx u = drho[i], v = drho[i + h] * mk(cs, sn);
+        drho[i] = u + v;
+        drho[i + h] = u - v;
+      }
+      __syncthreads();
+    }
+    for (int x = threadIdx.x; x < Nx; x += 256) jl[c * Nx + x] = drho[x];
   }
 }
 
@@ -1528,10 +1550,11 @@ extern "C" int pss_fill_line(const double* sample, double* lines, int32_t n, voi
 }
 
 extern "C" int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
-                         double dt, double dx, double* jline, void* stream) {
-  if (Nx > 2048) return pss_set_msg(2, "pss_jline: Nx > 2048");
+                         double dt, double dx, double* jline, int32_t fft, void* stream) {
+  if (Nx < 1 || Nx > 2048) return pss_set_msg(2, "pss_jline: 1 <= Nx <= 2048");
+  if (fft && (Nx & (Nx - 1))) return pss_set_msg(2, "pss_jline: the FFT of the line needs a power-of-two Nx");
   hipLaunchKernelGGL(k_jline, dim3(1), dim3(256), 0, (hipStream_t)stream, (const cplx*)P, (const cplx*)Rho,
-                     (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline);
+                     (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline, (int)(fft != 0));
   PSS_CHECK(hipGetLastError());
   return 0;
 }

@@ -311,13 +311,22 @@ __global__ void __launch_bounds__(256) k_prop_dft(const pss_wire_desc* __restric
   }
 }
 
-// ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H
-__global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __restrict__ descs) {
+__device__ void maxabs_body(const pss_wire_desc& d, const int m, const int bx, const int nbx, double* sh);
+
+// ---------------------- K3: diagonal traces (direct Hartree terms) + CalcGammaE/H; the
+// workgroups from mx_base on take max |p|, |C|, |D| of mixed-precision wires (64 each)
+__global__ void __launch_bounds__(256) k_traces_gamma(const pss_wire_desc* __restrict__ descs, const int mx_base) {
   const pss_wire_desc& d = descs[blockIdx.y];
   if (!wire_on(d)) return;
+  __shared__ double sh[8 * 4 + 8];
+  if ((int)blockIdx.x >= mx_base) {
+    if (d.precision != PSS_MIXED) return;
+    const int r = (int)blockIdx.x - mx_base;
+    maxabs_body(d, r >> 6, r & 63, 64, sh);
+    return;
+  }
   const int Nk = d.Nk, nd = 2 * Nk - 1;
   if ((int)blockIdx.x >= nd + Nk) return;
-  __shared__ double sh[8 * 4 + 8];
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
@@ -931,32 +940,36 @@ __device__ inline void atomic_max_pos(double* addr, double v) {
   atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-__global__ void __launch_bounds__(256) k_maxabs(const pss_wire_desc* __restrict__ descs, int m0) {
-  const pss_wire_desc& d = descs[blockIdx.z];
-  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+// max |x| of matrix m over the elements bx, bx + nbx*256, ... of one wire (workgroup bx of nbx)
+__device__ void maxabs_body(const pss_wire_desc& d, const int m, const int bx, const int nbx, double* sh) {
   const int Nk = d.Nk;
-  const int m = m0 + (int)blockIdx.y;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
   const int64_t NN = (int64_t)Nk * Nk;
   const cplx* H = (const cplx*)(ws + L.H);
-  __shared__ double sh[16];
   double v = 0.0;
   if (m < 6) {
     const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : m == 2 ? B.D2 : H + (m - 3) * NN;
-    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < NN; o += (int64_t)gridDim.x * 256) {
+    for (int64_t o = (int64_t)bx * 256 + threadIdx.x; o < NN; o += (int64_t)nbx * 256) {
       if ((m == 4 || m == 5) && (o / Nk == o % Nk)) continue;  // diagonal handled in FP64
       v = fmax(v, fmax(fabs(X[o].x), fabs(X[o].y)));
     }
   } else if (d.flags & PSS_OFFDIAGDPH) {
     const double* D = ws + (m == 6 ? L.Dh : L.De);
     const int64_t n = (2 * (int64_t)Nk + 1) * Nk;
-    for (int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256)
+    for (int64_t o = (int64_t)bx * 256 + threadIdx.x; o < n; o += (int64_t)nbx * 256)
       v = fmax(v, fabs(kHbar * D[o]));
   }
   v = block_max<256>(v, sh);
   if (threadIdx.x == 0 && v > 0.0) atomic_max_pos(ws + L.red + 8 + m, v);
+}
+
+__global__ void __launch_bounds__(256) k_maxabs(const pss_wire_desc* __restrict__ descs, int m0) {
+  const pss_wire_desc& d = descs[blockIdx.z];
+  if (!wire_on(d) || d.precision != PSS_MIXED) return;
+  __shared__ double sh[16];
+  maxabs_body(d, m0 + (int)blockIdx.y, (int)blockIdx.x, (int)gridDim.x, sh);
 }
 
 __device__ inline double pow2_scale(double mx) {
@@ -2135,7 +2148,9 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
   }
   hipLaunchKernelGGL(k_prop_front, dim3(2, nwires), dim3(256), 0, s, d_descs);
   hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk, nwires), dim3(256), 0, s, d_descs);
+  // max |p|, |C|, |D| of the mixed wires in 3 x 64 extra workgroups (after K2 cleared the slots)
+  hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk + (anymx ? 3 * 64 : 0), nwires), dim3(256), 0, s, d_descs,
+                     nd + max_Nk);
   hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
                      (size_t)(256 + max_Nk) * sizeof(double), s, d_descs);
   const int t16 = (max_Nk + 15) / 16;
@@ -2152,10 +2167,9 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
         hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires),
                            dim3(256), lds, s, d_descs);
     }
-    // mixed wires: max |p|, |C|, |D| -> exchange planes -> exchange GEMM
+    // mixed wires: max |p|, |C|, |D| (taken with K3) -> exchange planes -> exchange GEMM
     const int np = np64(max_Nk);
     if (anymx) {
-      hipLaunchKernelGGL(k_maxabs, dim3(64, 3, nwires), dim3(256), 0, s, d_descs, 0);
       hipLaunchKernelGGL(k_xsplit, dim3(np / 16, np / 64 + 1, 3 * nwires), dim3(256), 0, s, d_descs);
       hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
     }

@@ -96,3 +96,13 @@ def test_gc_paused_holds_the_collector_off_and_restores_it():
         assert not gc.isenabled()
     finally:
         gc.enable()
+
+
+def test_jline_fft_needs_power_of_two():
+    """The in-kernel FFT of the current line is radix-2: other lengths are refused before
+    any device work (null buffers are never touched)."""
+    from pulsesuite_amd import _lib
+
+    assert _lib.lib.pss_jline(None, None, None, None, 1, 96, 1e-17, 2e-8, None, 1, None) != 0
+    assert b"power-of-two" in _lib.lib.pss_last_error()
+    assert _lib.lib.pss_jline(None, None, None, None, 1, 4096, 1e-17, 2e-8, None, 0, None) != 0

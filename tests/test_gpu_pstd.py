@@ -631,3 +631,30 @@ def test_propagator_golden_mixed(gold):
     prop.run(int(gold["run_nsteps"][0]))
     for n in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
         assert rel(getattr(prop, n), gold["run_" + n]) < 1e-6, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("Nx,Nw", [(1, 1), (64, 2), (128, 1), (2048, 3)])
+def test_jline_fft_in_kernel(Nx, Nw):
+    """pss_jline with fft = 1: the current line and then its forward FFT along x in the
+    same kernel (radix-2 in LDS), against the line without FFT and numpy's FFT of it
+    (FP64 rounding of an N log N transform: 1e-14), and the same Pold/RhoOld update."""
+    import torch
+
+    from pulsesuite_amd import _lib
+
+    rng = np.random.default_rng(Nx + Nw)
+    c = lambda *sh: rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    P, Rho, Pold, RhoOld = c(Nw, 3, Nx), c(Nw, Nx), c(Nw, 3, Nx), c(Nw, Nx)
+    out = []
+    for fft in (0, 1):
+        t = [cu(a) for a in (P, Rho, Pold, RhoOld)]
+        jl = torch.zeros((3, Nx), dtype=torch.complex128, device="cuda")
+        _lib.check(_lib.lib.pss_jline(*(_lib.ptr(a) for a in t), Nw, Nx, 1e-17, 2e-8, _lib.ptr(jl), fft,
+                                      _lib.stream_ptr()), "jline")
+        torch.cuda.synchronize()
+        out.append((jl.cpu().numpy(), t[2].cpu().numpy(), t[3].cpu().numpy()))
+    (j0, po0, ro0), (j1, po1, ro1) = out
+    assert rel(j1, np.fft.fft(j0, axis=1)) < 1e-14
+    assert np.array_equal(po0, P) and np.array_equal(po1, P)
+    assert np.array_equal(ro0, Rho) and np.array_equal(ro1, Rho)
