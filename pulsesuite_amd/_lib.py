@@ -120,14 +120,15 @@ def ptr(t):
 
 @contextlib.contextmanager
 def gc_paused():
-    """Run the cyclic garbage collector now and keep it off until the block ends.
+    """Keep the cyclic garbage collector off until the block ends.
 
     Wraps every HIP graph capture: a collection that starts inside a capture runs the
     destructors of dead graphs, events and pooled tensors (graph-exec destroy, pool
     release) in the middle of the capture, where they are not allowed, and that has
-    aborted the process.  torch.cuda.graph no longer collects before capture by itself."""
+    aborted the process.  Garbage is left for the next collection after the capture (a
+    full collection before each capture cost the serving loop, which captures per solve,
+    about a second per 48 requests)."""
     was = gc.isenabled()
-    gc.collect()
     gc.disable()
     try:
         yield

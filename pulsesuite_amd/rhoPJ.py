@@ -100,6 +100,11 @@ class QWArray:
         # FFT of the transverse gates, once: passes along y and z of (1, Ny, 1) / (1, 1, Nz) views
         self._gline_y = _line_ctx(1, sp.Ny, 1, dev)
         self._gline_z = _line_ctx(1, 1, sp.Nz, dev)
+        # the current line's FFT along x: in the current-line kernel (one workgroup) up to
+        # 256 cells, where that is faster than a separate launch (Nx = 128: 10.6 us for both
+        # against 6.3 + 8.1 us); longer lines take the multi-workgroup line FFT
+        self._fft_in_jline = sp.Nx <= 256
+        self._gline_x = None if self._fft_in_jline else _line_ctx(Nx, 1, 1, dev)
         if sp.Ny > 1:
             _lib.check(_lib.lib.pss_fft_line(self._gline_y.gp, _lib.ptr(self.gy), _lib.ptr(self.gy), 1, -1, st), "fft gy")
         if sp.Nz > 1:
@@ -146,10 +151,15 @@ class QWArray:
         W.set_dt(time.dt)
 
     def deposit(self, space, time):
-        """J line from the wires' new P and rho, then its FFT along x (in the same kernel)."""
+        """J line from the wires' new P and rho, then its FFT along x."""
         st = _lib.stream_ptr()
+        ink = self._fft_in_jline and space.Nx > 1
         _lib.check(_lib.lib.pss_jline(_lib.ptr(self.P), _lib.ptr(self.Rho), _lib.ptr(self.Pold), _lib.ptr(self.RhoOld),
-                                      self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), 1, st), "jline")
+                                      self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), 1 if ink else 0, st),
+                   "jline")
+        if space.Nx > 1 and not ink:
+            lines = (_lib.C.c_void_p * 3)(*(_lib.ptr(self.jl[c]) for c in range(3)))
+            _lib.check(_lib.lib.pss_fft_line_batch(self._gline_x.gp, lines, lines, 3, 0, -1, st), "fft jline")
 
 
 _line_cache = {}
