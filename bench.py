@@ -421,8 +421,9 @@ def main():
         "config": {"workload": f"3-D PSTD {a.grid}^3 + {a.npml}-cell CPML + soft source, 1 wire Nk={a.nk}, "
                                f"{a.steps} leapfrog steps", "grid": a.grid, "nk": a.nk, "npml": a.npml,
                    "wires": 1, "precision": main_prec,
-                   "launch": "eager" if a.eager else ("HIP graph replays of the pipelined step (wire step on a second "
-                                                        "stream beside the B half and the next CPML-E correction)")},
+                   "launch": "eager" if a.eager else ("HIP graph replays of pipelined steps, 4 per graph (wire step on "
+                                                        "a second stream beside the B half and the next CPML-E "
+                                                        "correction)")},
     }
     if errs is not None:
         line["fp64_value"] = results["fp64"]["value"]

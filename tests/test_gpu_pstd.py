@@ -369,14 +369,15 @@ def test_captured_step_matches_eager(precision):
     assert np.linalg.norm(a[1]) > 0 and np.linalg.norm(a[6]) > 0
 
 
-@pytest.mark.parametrize("precision,feedback,steps,tol,shape", [("fp64", False, 12, 1e-12, (64, 64, 64)),
-                                                                ("mixed", False, 12, 1e-6, (64, 64, 64)),
-                                                                ("fp64", True, 8, 1e-12, (64, 64, 64)),
-                                                                ("mixed", False, 6, 1e-6, (128, 64, 64))])
-def test_pipelined_run_matches_eager(precision, feedback, steps, tol, shape):
+@pytest.mark.parametrize("precision,feedback,steps,tol,shape,spg", [("fp64", False, 12, 1e-12, (64, 64, 64), 1),
+                                                                    ("mixed", False, 12, 1e-6, (64, 64, 64), 3),
+                                                                    ("fp64", True, 8, 1e-12, (64, 64, 64), 2),
+                                                                    ("mixed", False, 6, 1e-6, (128, 64, 64), 1)])
+def test_pipelined_run_matches_eager(precision, feedback, steps, tol, shape, spg):
     """run_pipelined (the next step's CPML-E correction beside the wire step, in mixed
-    precision with the E update's curl term, graph replays) against as many eager steps, in chunks of 1, 2 and the rest (one grid whose
-    axes differ: the slab-derivative stages then take one launch per axis).  The same
+    precision with the E update's curl term, graph replays of spg steps, of one step and
+    of the closing step) against as many eager steps, in chunks of 1, 2 and the rest (one
+    grid whose axes differ: the slab-derivative stages then take one launch per axis).  The same
     updates are summed in another order, so agreement is to rounding, measured on each
     field as a vector (its three components together: with the current fed back, Ey
     grows to ~1e12 while Ex stays ~1e4, and a rounding difference of the order of Ey's
@@ -413,6 +414,7 @@ def test_pipelined_run_matches_eager(precision, feedback, steps, tol, shape):
 
     pe, qe = make()
     pp, qp = make()
+    pp.steps_per_graph = spg
     assert pp.pipelined()
     for _ in range(2):
         pe.step()
