@@ -188,9 +188,10 @@ int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, double* Fy, doub
 /* The B half step of a leapfrog step, UpdateB3D followed by ApplyCPML_B
  * (PSTD3D/PSTD3D.py:79-113 and PSTD3D/cpml.py:575-638, as PSTD3D/PSTD3D.py:523-627 calls
  * them): B -= i dt (k x E), then the CPML-B correction with the soft source on src_comp.
- * Both updates read only E, so the curl term is added in the last pass of the
- * correction's forward transform when that axis is 64 ... 1024 long (one read-modify-
- * write of B instead of two); other lengths run the two updates one after the other.
+ * Both updates read only E, so in mixed precision the curl term is added in the last
+ * pass of the correction's forward transform when that axis is 64 ... 1024 long (one
+ * read-modify-write of B instead of two); FP64 grids and other lengths run the two
+ * updates one after the other (with the FP64 correction the fused pass was slower).
  * Arguments as for pss_update_b3d and pss_cpml_half (which = 1, no sample). */
 int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz, const double* Ex, const double* Ey,
                const double* Ez, double* psi, double* scratch, int32_t src_comp, const double* src_par,

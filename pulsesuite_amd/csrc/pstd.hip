@@ -1491,7 +1491,10 @@ extern "C" int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz,
   for (int a = 0; a < 3; ++a)
     if ((a == 0 ? g->Nx : a == 1 ? g->Ny : g->Nz) > 1) last = a;
   const int N = last == 0 ? g->Nx : last == 1 ? g->Ny : g->Nz;
-  const bool fuse = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  // fused in mixed precision only: with the FP64 correction (complex128 dF) the fused last
+  // pass cost more than the separate B update (128^3 pipelined step: FP64 0.806 ms fused,
+  // 0.792 ms separate; mixed 0.578 ms fused, 0.587 ms separate)
+  const bool fuse = g->precision == PSS_MIXED && (N == 64 || N == 128 || N == 256 || N == 512 || N == 1024);
   if (!fuse) {
     const int rc = pss_update_b3d(g, Ex, Ey, Ez, Bx, By, Bz, stream);
     if (rc) return rc;

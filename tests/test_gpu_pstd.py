@@ -544,7 +544,7 @@ def test_cpml_e_curl_matches_update_then_cpml(shape, fused, precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_b_half_matches_update_then_cpml(shape, fused, precision):
     """pss_b_half (B -= i dt k x E added in the correction's last forward pass when that
-    axis is 64 ... 1024 long) against pss_update_b3d followed by pss_cpml_half, with the
+    axis is 64 ... 1024 long, in mixed precision; FP64 runs the two updates) against pss_update_b3d followed by pss_cpml_half, with the
     soft source on Bz, two half steps (psi carried): the same sums in another order,
     FP64 rounding (1e-13 on B as a vector).  Mixed: the correction's last pass is an FP32
     transform in both, but in two different kernels (plain and curl-adding), whose FP32
