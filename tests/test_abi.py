@@ -106,3 +106,20 @@ def test_jline_fft_needs_power_of_two():
     assert _lib.lib.pss_jline(None, None, None, None, 1, 96, 1e-17, 2e-8, None, 1, None) != 0
     assert b"power-of-two" in _lib.lib.pss_last_error()
     assert _lib.lib.pss_jline(None, None, None, None, 1, 4096, 1e-17, 2e-8, None, 0, None) != 0
+
+
+def test_pipelined_bookkeeping_over_multi_step_replays():
+    """run_pipelined replays graphs of several steps: the host clock, the step counters and
+    the periodic finite check must advance as over single steps (a check whenever a
+    multiple of check_every is crossed, once per replay).  Host logic only."""
+    import types
+
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+
+    checks = []
+    fake = types.SimpleNamespace(time=types.SimpleNamespace(t=0.0, dt=0.5, n=0), _nstep=0, check_every=3,
+                                 check_finite=lambda: checks.append(fake._nstep))
+    for k in (4, 4, 1, 1, 4, 1):
+        PSTD3DPropagator._book_steps(fake, k)
+    assert fake._nstep == 15 and fake.time.n == 15 and fake.time.t == 7.5
+    assert checks == [4, 8, 9, 14, 15]  # replays that crossed 3, 6, 9, 12, 15
