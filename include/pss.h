@@ -151,12 +151,15 @@ int pss_update_e3d(const pss_grid* g, const double* Bx, const double* By, const 
  * tick after the sample (the soft source's device time, as pss_add_scalar would).
  * Bx, By, Bz all null: the current term only (E -= mu0 v2 dt J), for a step whose curl
  * term was applied before by pss_cpml_e_curl.  lines (may be null): nlines wire lines
- * [nlines][3][line_n] complex, each filled with the sample as pss_fill_line would. */
+ * [nlines][3][line_n] complex, each filled with the sample as pss_fill_line would.
+ * update_b != 0 (B given): also B -= i dt (k x E) from the new E in the same pass (the
+ * curl term of UpdateB3D, PSTD3D/PSTD3D.py:79-113), so B is written too. */
 int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const double* By, const double* Bz,
                           const double* Jx, const double* Jy, const double* Jz, const double* jline,
                           const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
                           const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
-                          double* lines, int32_t line_n, int32_t nlines, double* clock, double tick, void* stream);
+                          double* lines, int32_t line_n, int32_t nlines, double* clock, double tick,
+                          int32_t update_b, void* stream);
 /* B -= i dt (k x E)  (PSTD3D/PSTD3D.py:79-113). */
 int pss_update_b3d(const pss_grid* g, const double* Ex, const double* Ey, const double* Ez,
                    double* Bx, double* By, double* Bz, void* stream);

@@ -307,14 +307,16 @@ class PSTD3DPropagator:
         _lib.check(_lib.lib.pss_update_e3d_sample(self.ctx.gp, *Bp, None, None, None,
                                                   P(jl), P(gy), P(gz), P(self.Ex), P(self.Ey), P(self.Ez), P(sidx),
                                                   P(sw), P(self.cpml.scratch), P(sout), lines, line_n, nlines,
-                                                  P(self._tdev), tm.dt / 2.0, st), "pss_update_e3d_sample")
+                                                  P(self._tdev), tm.dt / 2.0, 0 if self._curl_early else 1, st),
+                   "pss_update_e3d_sample")
         tm.t += tm.dt / 2.0
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ensure_side()
         self._ev_e.record(cur)
         # recorded first, the field branch is the one the graph starts at once (the other
         # branch's queue starts later); it is the longer one here
-        self.step_b_half(stream)
+        # FP64: the B curl term came with the E update (same pass over the cells)
+        self.step_b_half(stream, curl=self._curl_early)
         if pre_next:
             self._cpml_e_next(stream)
         self._side.wait_event(self._ev_e)
@@ -393,13 +395,19 @@ class PSTD3DPropagator:
         if self.check_every and self._nstep // self.check_every > before // self.check_every:
             self.check_finite()
 
-    def step_b_half(self, stream=None):
+    def step_b_half(self, stream=None, curl=True):
         """B update + CPML-B half step (soft source on Bz), then t += dt/2: the part of a
-        step after the calc_j hook."""
+        step after the calc_j hook.  curl=False: the CPML-B correction only (the curl term
+        of the B update was applied already)."""
         tm = self.time
         st = _lib.stream_ptr(stream)
-        self.cpml.UpdateB_ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, tm.dt,
-                                      src=self._src(2, self.pulse.Amp / self.v, tm.t), stream=stream)
+        src = self._src(2, self.pulse.Amp / self.v, tm.t)
+        if curl:
+            self.cpml.UpdateB_ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, tm.dt, src=src,
+                                          stream=stream)
+        else:
+            self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, dt=tm.dt, src=src,
+                                  stream=stream)
         self._advance(tm.dt / 2.0, st)
         tm.t += tm.dt / 2.0
         tm.n += 1

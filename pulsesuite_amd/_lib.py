@@ -71,7 +71,8 @@ def _load():
         "pss_sbe_step_p": (C.c_int, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
         "pss_vint": (C.c_int, [_vp, _i, _vp, _i, _d, _d, _d, _d, _vp, _vp]),
         "pss_update_e3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp]),
-        "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp, _i, _i, _vp, _d, _vp]),
+        "pss_update_e3d_sample": (C.c_int, [C.POINTER(Grid)] + [_vp] * 12 + [_vp] * 4 + [_vp, _i, _i, _vp, _d, _i,
+                                                                                      _vp]),
         "pss_update_b3d": (C.c_int, [C.POINTER(Grid)] + [_vp] * 6 + [_vp]),
         "pss_fft3d": (C.c_int, [C.POINTER(Grid), _vp, _i, _vp]),
         "pss_fft_line": (C.c_int, [C.POINTER(Grid), _vp, _vp, _i, _i, _vp]),

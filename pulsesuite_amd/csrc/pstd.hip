@@ -1115,14 +1115,14 @@ __global__ void __launch_bounds__(256) k_ksample(pss_grid g, const cplx* __restr
 // The E update with the point sample of the new E (partial sums as k_ksample): one pass
 // over E instead of two.  CURL = false: the current term only (the curl term applied
 // before, by pss_cpml_e_curl).
-template <bool CURL>
-__global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx* __restrict__ Bx,
-                                                         const cplx* __restrict__ By, const cplx* __restrict__ Bz,
+template <bool CURL, bool BUPD>
+__global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx* Bx, const cplx* By, const cplx* Bz,
                                                          const cplx* __restrict__ Jx, const cplx* __restrict__ Jy,
                                                          const cplx* __restrict__ Jz, const cplx* __restrict__ jl,
                                                          const cplx* __restrict__ gy, const cplx* __restrict__ gz,
                                                          cplx* Ex, cplx* Ey, cplx* Ez, const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ w, double* partial) {
+                                                         const double* __restrict__ w, double* partial, cplx* Bwx,
+                                                         cplx* Bwy, cplx* Bwz) {
   extern __shared__ cplx tws[];
   __shared__ SampleLDS sl;
   const int L = sample_prep(g, idx, w, tws, sl);
@@ -1134,6 +1134,13 @@ __global__ void __launch_bounds__(256) k_update_e_sample(pss_grid g, const cplx*
     const int iz = (int)(o & (Nz - 1)), iy = (int)((o >> lz) & (Ny - 1)), ix = (int)(o >> (lz + ly));
     cplx e0, e1, e2;
     update_e_cell<CURL>(g, o, ix, iy, iz, Bx, By, Bz, Jx, Jy, Jz, jl, gy, gz, Ex, Ey, Ez, e0, e1, e2);
+    if (BUPD) {  // B -= i dt (k x E) from the new E (as k_update_b)
+      const double qx = g.kx[ix], qy = g.ky[iy], qz = g.kz[iz];
+      const cplx cx = qy * e2 - qz * e1, cy = qz * e0 - qx * e2, cz = qx * e1 - qy * e0;
+      Bwx[o] = Bwx[o] - mk(-cx.y, cx.x) * g.dt;
+      Bwy[o] = Bwy[o] - mk(-cy.y, cy.x) * g.dt;
+      Bwz[o] = Bwz[o] - mk(-cz.y, cz.x) * g.dt;
+    }
     const cplx phi = sample_phi(L, tws, sl, ix, iy, iz);
     a0 = a0 + e0 * phi;
     a1 = a1 + e1 * phi;
@@ -1302,7 +1309,7 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
                                      const double* gy, const double* gz, double* Ex, double* Ey, double* Ez,
                                      const int32_t* sample_idx, const double* sample_w, double* partial,
                                      double* sample_out, double* lines, int32_t line_n, int32_t nlines,
-                                     double* clock, double tick, void* stream) {
+                                     double* clock, double tick, int32_t update_b, void* stream) {
   if (!pow2_axes(g)) return pss_set_msg(2, "pss_update_e3d_sample: axis lengths must be powers of two");
   if (lines && (line_n < 1 || nlines < 1)) return pss_set_msg(2, "pss_update_e3d_sample: line_n, nlines >= 1");
   const bool curl = Bx != nullptr;
@@ -1312,10 +1319,17 @@ extern "C" int pss_update_e3d_sample(const pss_grid* g, const double* Bx, const 
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
   const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
-  auto kern = curl ? k_update_e_sample<true> : k_update_e_sample<false>;
+  if (update_b && !curl) return pss_set_msg(2, "pss_update_e3d_sample: update_b needs B");
+  auto kern = update_b ? k_update_e_sample<true, true> : curl ? k_update_e_sample<true, false>
+                                                             : k_update_e_sample<false, false>;
+  cplx* bw[3] = {nullptr, nullptr, nullptr};
+  if (update_b) {
+    bw[0] = (cplx*)Bx; bw[1] = (cplx*)By; bw[2] = (cplx*)Bz;
+  }
   hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), lds, s, *g, (const cplx*)Bx, (const cplx*)By,
                      (const cplx*)Bz, (const cplx*)Jx, (const cplx*)Jy, (const cplx*)Jz, (const cplx*)jline,
-                     (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial);
+                     (const cplx*)gy, (const cplx*)gz, (cplx*)Ex, (cplx*)Ey, (cplx*)Ez, sample_idx, sample_w, partial,
+                     bw[0], bw[1], bw[2]);
   hipLaunchKernelGGL((k_ksample_final<cplx>), dim3(1), dim3(256), 0, s, *g, partial, nblk, (const cplx*)nullptr,
                      (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out, clock, tick,
                      lines, line_n, nlines);

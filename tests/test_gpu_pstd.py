@@ -123,7 +123,7 @@ def test_update_e_sample_and_ksample(shape):
                                        *(P(t) for t in E1), st), "e")
     _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, *(P(t) for t in dB), None, None, None, P(djl), P(dgy), P(dgz),
                                               *(P(t) for t in E2), P(t_idx), P(t_w), P(part), P(out1), P(lines), Nx, 2,
-                                              P(clock), 0.25, st), "es")
+                                              P(clock), 0.25, 0, st), "es")
     _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E1), P(t_idx), P(t_w), P(part), P(out2), st), "ks")
     torch.cuda.synchronize()
     for a, b in zip(E1, E2):
@@ -136,6 +136,18 @@ def test_update_e_sample_and_ksample(shape):
     real = [np.fft.ifftn(t.cpu().numpy()) for t in E1]
     ref = [sum(w * r.reshape(-1)[o] for o, w in zip(idx, wts) if o >= 0) for r in real]
     assert rel(out1, np.array(ref)) < 1e-12
+    # update_b: B -= i dt k x E_new in the same pass, as pss_update_b3d after the E update
+    E7, B7, B8 = [cu(a) for a in E], [cu(a) for a in B], [cu(a) for a in B]
+    _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, *(P(t) for t in B7), None, None, None, P(djl), P(dgy), P(dgz),
+                                              *(P(t) for t in E7), P(t_idx), P(t_w), P(part), P(out2), None, 0, 0,
+                                              None, 0.0, 1, st), "es7")
+    _lib.check(_lib.lib.pss_update_b3d(ctx.gp, *(P(t) for t in E1), *(P(t) for t in B8), st), "b8")
+    torch.cuda.synchronize()
+    for a, b in zip(E7, E1):
+        assert rel(a, b.cpu().numpy()) < 1e-15
+    for a, b in zip(B7, B8):
+        assert rel(a, b.cpu().numpy()) < 1e-15
+    assert rel(out2, out1.cpu().numpy()) < 1e-14
     # null B: the current term only, the same as a full update with B = 0
     zB = [torch.zeros_like(t) for t in dB]
     E3, E4 = [cu(a) for a in E], [cu(a) for a in E]
@@ -143,7 +155,7 @@ def test_update_e_sample_and_ksample(shape):
                                        *(P(t) for t in E3), st), "e0")
     _lib.check(_lib.lib.pss_update_e3d_sample(ctx.gp, None, None, None, None, None, None, P(djl), P(dgy), P(dgz),
                                               *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0, 0,
-                                              None, 0.0, st), "es0")
+                                              None, 0.0, 0, st), "es0")
     _lib.check(_lib.lib.pss_ksample(ctx.gp, *(P(t) for t in E3), P(t_idx), P(t_w), P(part), P(out2), st), "ks0")
     torch.cuda.synchronize()
     for a, b in zip(E3, E4):
@@ -151,7 +163,7 @@ def test_update_e_sample_and_ksample(shape):
     assert rel(out1, out2.cpu().numpy()) < 1e-14
     assert _lib.lib.pss_update_e3d_sample(ctx.gp, P(dB[0]), None, None, None, None, None, P(djl), P(dgy), P(dgz),
                                           *(P(t) for t in E4), P(t_idx), P(t_w), P(part), P(out1), None, 0, 0,
-                                          None, 0.0, st) != 0  # B in part is refused
+                                          None, 0.0, 0, st) != 0  # B in part is refused
 
 
 def test_update_e_b_match_oracle():
