@@ -212,7 +212,8 @@ int pss_cpml_e_curl(const pss_grid* g, double* Ex, double* Ey, double* Ez, const
 
 /* The trilinear point sample of pss_cpml_half taken on its own, from k-space fields that
  * already hold the whole half step (EAtXYZ of the IFFT, PSTD3D/usefulsubs.py:2016-2085):
- * sample_out[6] = (Re, Im) of each component.  partial: 6144 doubles of device scratch. */
+ * sample_out[6] = (Re, Im) of each component.  partial: 6144 doubles of device scratch.
+ * Axis lengths: powers of two <= 2048. */
 int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz, const int32_t* sample_idx,
                 const double* sample_w, double* partial, double* sample_out, void* stream);
 
@@ -237,10 +238,14 @@ int pss_fill_line(const double* sample, double* lines, int32_t n, void* stream);
 /* Current line of the wires (PSTD3D/rhoPJ.py:1183-1194 with CalcJfx :561-592):
  * J_c = sum_w (P_cw - Pold_cw)/dt/Nw, J_x += CalcJfx(sum Rho, sum RhoOld, dt, dx)/Nw;
  * then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx], jline: [3][Nx].
+ * keep_old = 0 is the reference's behaviour: QWArray.QuantumWire re-creates PxxOld and
+ * RhoOld as zeros on every call (PSTD3D/rhoPJ.py:1138-1147), so J_c = sum_w P_cw/dt/Nw
+ * and CalcJfx sees RhoOld = 0; keep_old = 1 keeps the previous call's values
+ * (J = dP/dt).  Pold and RhoOld are written in both cases.
  * fft != 0: each line then takes its forward FFT along x (numpy convention, as
  * pss_fft_line_batch with dir = -1) in the same kernel; Nx a power of two. */
 int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
-              double dt, double dx, double* jline, int32_t fft, void* stream);
+              double dt, double dx, double* jline, int32_t fft, int32_t keep_old, void* stream);
 
 /*
  * Fused stage combine of the classic RK4 step (libpulsesuite/integrator.py:143-157,
@@ -259,7 +264,7 @@ int pss_add_scalar(double* x, double v, void* stream);
 
 /* Message of the last failing call (host string, valid until the next call). */
 const char* pss_last_error(void);
-/* Library version string (host). */
+/* Library version string (host); starts with "pss 0.2" for this ABI. */
 const char* pss_version(void);
 
 #ifdef __cplusplus

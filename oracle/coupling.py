@@ -3,15 +3,16 @@
 Test infrastructure only (see ``oracle/__init__.py``).  Follows
 PSTD3D/rhoPJ.py:1060-1329 (QWArray.QuantumWire) for the current-density path:
 point-sample E at the wire centre (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085),
-run every sub-band wire's QWCalculator on the x line, J = sum_w dP_w/dt / Nw
-+ CalcJfx (PSTD3D/rhoPJ.py:561-592), deposit J(x) g_y(y) g_z(z)
+run every sub-band wire's QWCalculator on the x line, J = sum_w (P_w - P_old_w)/dt / Nw
++ CalcJfx (PSTD3D/rhoPJ.py:561-592, 1183-1194), deposit J(x) g_y(y) g_z(z)
 (QWPlacement, PSTD3D/rhoPJ.py:736-781) and FFT it to k-space.
 
-Deviation kept on purpose and documented: the reference zeroes PxxOld on every
-call (rhoPJ.py:1138-1147), which turns J into P/dt; here the previous
-polarisation is kept so J = (P - P_old)/dt as the reference's comment states.
-The longitudinal-field diagnostics (ElongfromRho, RhoB*) are not on the J path
-and are omitted.
+Current semantics (``current=``): "reference" follows the reference's code, which
+re-creates PxxOld/PyyOld/PzzOld/RhoOld as zeros on every call (rhoPJ.py:1138-1147)
+before forming J, so J = sum_w P_w/dt/Nw; "delta" keeps the previous call's P and
+rho (J = dP/dt, as the reference's comment at rhoPJ.py:1182 describes).  The
+longitudinal-field diagnostics (ElongfromRho, RhoB*) are not on the J path and are
+omitted.
 
 Pinning: EAtXYZ, CalcJfx, QWPlacement pinned (closed forms); the composition is
 build-defined (no reference driver ships) and parity unpinned.
@@ -60,7 +61,9 @@ def CalcJfx(RhoNew, RhoPrev, dt, dx):
 class OracleCoupling:
     """Nw sub-band wires along x through (x0, y0, z0) feeding J into the grid."""
 
-    def __init__(self, space, lam, Emax0, Nw=1, qw=None, mb=None, y0=0.0, z0=0.0, ay=5e-9, az=5e-9, x0=0.0):
+    def __init__(self, space, lam, Emax0, Nw=1, qw=None, mb=None, y0=0.0, z0=0.0, ay=5e-9, az=5e-9, x0=0.0,
+                 current="reference"):
+        self.current = current
         self.space = space
         self.rx = GetXArray(space)
         self.ry = GetYArray(space)
@@ -78,6 +81,9 @@ class OracleCoupling:
 
     def calc_j(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
         dt = time.dt
+        if self.current == "reference":  # rhoPJ.py:1138-1147: the old arrays start from zero each call
+            self.Pold = np.zeros_like(self.Pold)
+            self.RhoOld = np.zeros_like(self.RhoOld)
         Er = []
         for F in (Ex, Ey, Ez):
             G = F.copy()

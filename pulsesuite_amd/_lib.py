@@ -28,6 +28,9 @@ LORENTZDELTA = 1 << 14
 FP64 = 0
 MIXED = 1
 
+# ABI version the signatures below describe (pss_version() of include/pss.h)
+ABI_VERSION = "pss 0.2"
+
 _vp = C.c_void_p
 _d = C.c_double
 _i = C.c_int32
@@ -85,7 +88,7 @@ def _load():
         "pss_cpml_half": (C.c_int, [C.POINTER(Grid), _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                     C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
-        "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _vp]),
+        "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _i, _vp]),
         "pss_drive_uniform": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_add_scalar": (C.c_int, [_vp, _d, _vp]),
         "pss_rk4_stage": (C.c_int, [_i, C.c_int64, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -96,6 +99,9 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    ver = lib.pss_version().decode(errors="replace")
+    if not ver.startswith(ABI_VERSION):
+        raise RuntimeError(f"{LIB_PATH} reports {ver!r}; this binding needs {ABI_VERSION!r}: rebuild the library")
     return lib
 
 

@@ -1212,15 +1212,18 @@ __global__ void k_fill_line(const double* sample, double* lines, int n) {
 // J line from the wires' polarisation and charge (PSTD3D/rhoPJ.py:1183-1194, CalcJfx :561-592):
 // J_c(x) = sum_w (P_cw - Pold_cw)/dt / Nw, J_x += CalcJfx(sum_w Rho_w, sum_w RhoOld_w)/Nw;
 // then Pold <- P, RhoOld <- Rho.  P: [Nw][3][Nx], Rho: [Nw][Nx] complex.  One block.
+// keep_old = 0: Pold and RhoOld enter as zero (the reference re-creates them as zero
+// arrays on every call, PSTD3D/rhoPJ.py:1138-1147, so its J is sum_w P_cw/dt/Nw); they
+// are still written.  Subtracting an exact zero leaves the arithmetic of each term as is.
 __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, cplx* Pold, cplx* RhoOld, int Nw,
-                                               int Nx, double dt, double dx, cplx* jl, int fft) {
+                                               int Nx, double dt, double dx, cplx* jl, int fft, int keep_old) {
   __shared__ cplx drho[2048];
   for (int x = threadIdx.x; x < Nx; x += 256) {
     for (int c = 0; c < 3; ++c) {
       cplx s = mk(0, 0);
       for (int w = 0; w < Nw; ++w) {
         const int64_t o = ((int64_t)w * 3 + c) * Nx + x;
-        s = s + (P[o] - Pold[o]);
+        s = s + (P[o] - (keep_old ? Pold[o] : mk(0, 0)));
         Pold[o] = P[o];
       }
       jl[c * Nx + x] = mk(s.x / dt / Nw, s.y / dt / Nw);
@@ -1229,7 +1232,7 @@ __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, c
     for (int w = 0; w < Nw; ++w) {
       const int64_t o = (int64_t)w * Nx + x;
       rn = rn + Rho[o];
-      ro = ro + RhoOld[o];
+      if (keep_old) ro = ro + RhoOld[o];
       RhoOld[o] = Rho[o];
     }
     const cplx dr = rn - ro;
@@ -1362,10 +1365,9 @@ extern "C" int pss_fft_line(const pss_grid* g, const double* in, double* out, in
   return 0;
 }
 
-extern "C" int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
-                           const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
-                           void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+static int ksample_launch(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
+                          const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
+                          hipStream_t s) {
   const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
   const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
   const size_t lds = (size_t)(std::max(g->Nx, std::max(g->Ny, g->Nz)) / 2 + 1) * sizeof(cplx);
@@ -1375,6 +1377,14 @@ extern "C" int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy
                      (const cplx*)nullptr, (const cplx*)nullptr, sample_idx, sample_w, sample_out);
   PSS_CHECK(hipGetLastError());
   return 0;
+}
+
+extern "C" int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
+                           const int32_t* sample_idx, const double* sample_w, double* partial, double* sample_out,
+                           void* stream) {
+  if (!pow2_axes(g) || g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
+    return pss_set_msg(2, "pss_ksample: axis lengths must be powers of two <= 2048");
+  return ksample_launch(g, Fx, Fy, Fz, sample_idx, sample_w, partial, sample_out, (hipStream_t)stream);
 }
 
 extern "C" int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out, int32_t nb,
@@ -1567,11 +1577,12 @@ extern "C" int pss_fill_line(const double* sample, double* lines, int32_t n, voi
 }
 
 extern "C" int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, int32_t Nw, int32_t Nx,
-                         double dt, double dx, double* jline, int32_t fft, void* stream) {
+                         double dt, double dx, double* jline, int32_t fft, int32_t keep_old, void* stream) {
   if (Nx < 1 || Nx > 2048) return pss_set_msg(2, "pss_jline: 1 <= Nx <= 2048");
+  if (Nw < 1) return pss_set_msg(2, "pss_jline: Nw >= 1");
   if (fft && (Nx & (Nx - 1))) return pss_set_msg(2, "pss_jline: the FFT of the line needs a power-of-two Nx");
   hipLaunchKernelGGL(k_jline, dim3(1), dim3(256), 0, (hipStream_t)stream, (const cplx*)P, (const cplx*)Rho,
-                     (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline, (int)(fft != 0));
+                     (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline, (int)(fft != 0), (int)(keep_old != 0));
   PSS_CHECK(hipGetLastError());
   return 0;
 }

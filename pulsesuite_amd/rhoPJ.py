@@ -9,10 +9,15 @@ in k-space.  Because the deposit is separable, its 3-D FFT is the product of
 three 1-D FFTs: J_hat(kx,ky,kz) = FFT_x(J)(kx) FFT_y(g_y)(ky) FFT_z(g_z)(kz); the
 E-update kernel forms it on the fly and no J grid is stored or transformed.
 
-The previous polarisation is kept between calls, so J = (P - P_old)/dt as the
-reference's comment describes (its code resets P_old to zero every call,
-rhoPJ.py:1138-1147, which would make J = P/dt).  The longitudinal-field
-diagnostics (ElongfromRho, RhoB*) are not on the J path and are not computed.
+Two current semantics (``current=``):
+  * "reference" (default): what the reference's code does.  QuantumWire re-creates
+    PxxOld/PyyOld/PzzOld/RhoOld as zero arrays on every call (rhoPJ.py:1138-1147)
+    before it forms J = sum_w (P_w - P_old_w)/dt/Nw (:1183-1194), so J = sum_w P_w/dt/Nw
+    and CalcJfx sees RhoOld = 0.
+  * "delta": the previous call's P and rho are kept, so J = (P - P_old)/dt, the time
+    derivative that the reference's comment (:1182) describes.
+The longitudinal-field diagnostics (ElongfromRho, RhoB*) are not on the J path and
+are not computed.
 """
 
 import numpy as np
@@ -59,13 +64,17 @@ class QWArray:
     """Nw sub-band wires along x through (x0, y0, z0), coupled to a PSTD3DPropagator.
 
     qw_params / mb_flags configure the wires (see SBEs.SBESolver); Emax0 sets
-    the wireoff gate (1e-3 Emax0).
+    the wireoff gate (1e-3 Emax0); current selects the current semantics, "reference"
+    (J = P/dt, the reference's code) or "delta" (J = dP/dt).
     """
 
     separable = True
 
     def __init__(self, qw_params, mb_flags, Nw=1, Emax0=1.0, y0=0.0, z0=0.0, x0=0.0, ay=5e-9, az=5e-9,
-                 precision="fp64"):
+                 precision="fp64", current="reference"):
+        if current not in ("reference", "delta"):
+            raise ValueError(f"current must be 'reference' or 'delta', not {current!r}")
+        self.current = current
         self.qw, self.mb = qw_params, mb_flags
         self.Nw = Nw
         self.Emax0 = Emax0
@@ -155,7 +164,8 @@ class QWArray:
         st = _lib.stream_ptr()
         ink = self._fft_in_jline and space.Nx > 1
         _lib.check(_lib.lib.pss_jline(_lib.ptr(self.P), _lib.ptr(self.Rho), _lib.ptr(self.Pold), _lib.ptr(self.RhoOld),
-                                      self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), 1 if ink else 0, st),
+                                      self.Nw, space.Nx, time.dt, self.dx, _lib.ptr(self.jl), 1 if ink else 0,
+                                      1 if self.current == "delta" else 0, st),
                    "jline")
         if space.Nx > 1 and not ink:
             lines = (_lib.C.c_void_p * 3)(*(_lib.ptr(self.jl[c]) for c in range(3)))

@@ -103,9 +103,9 @@ def test_jline_fft_needs_power_of_two():
     any device work (null buffers are never touched)."""
     from pulsesuite_amd import _lib
 
-    assert _lib.lib.pss_jline(None, None, None, None, 1, 96, 1e-17, 2e-8, None, 1, None) != 0
+    assert _lib.lib.pss_jline(None, None, None, None, 1, 96, 1e-17, 2e-8, None, 1, 0, None) != 0
     assert b"power-of-two" in _lib.lib.pss_last_error()
-    assert _lib.lib.pss_jline(None, None, None, None, 1, 4096, 1e-17, 2e-8, None, 0, None) != 0
+    assert _lib.lib.pss_jline(None, None, None, None, 1, 4096, 1e-17, 2e-8, None, 0, 1, None) != 0
 
 
 def test_pipelined_bookkeeping_over_multi_step_replays():

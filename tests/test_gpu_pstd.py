@@ -245,10 +245,12 @@ def test_ic_seed_golden(gold):
     assert rel(Ey, gold["ic_Ey"]) < 1e-13 and rel(Bz, gold["ic_Bz"]) < 1e-13
 
 
+@pytest.mark.parametrize("current", ["reference", "delta"])
 @pytest.mark.parametrize("grid,nk,steps", [((32, 32, 32), 14, 6), ((64, 1, 1), 12, 8)])
-def test_coupled_step_matches_oracle(grid, nk, steps):
+def test_coupled_step_matches_oracle(grid, nk, steps, current):
     """Soft-source PSTD + CPML with one wire (Nw = 2 sub-bands) driven by the
-    point field, current fed back through the separable deposit."""
+    point field, current fed back through the separable deposit; both current
+    semantics (the reference's J = P/dt and J = dP/dt) against the oracle's."""
     from oracle.coupling import OracleCoupling
     from oracle.sbe import MBFlags as OMB
     from oracle.sbe import QWParams as OQW
@@ -268,12 +270,12 @@ def test_coupled_step_matches_oracle(grid, nk, steps):
     npml = 6 if Nx >= 32 else None
     # oracle
     osp = OP.Space(3, Nx, Ny, Nz, dx, dx, dx, epsr)
-    oc = OracleCoupling(osp, 800e-9, pulse["Amp"], Nw=2, qw=OQW(L=L), mb=OMB())
+    oc = OracleCoupling(osp, 800e-9, pulse["Amp"], Nw=2, qw=OQW(L=L), mb=OMB(), current=current)
     oprop = OP.Propagator(osp, OP.Time(0.0, 1.0, dt, 0), OP.Pulse(**pulse), "soft", "cpml", calc_j=oc.calc_j,
                           npml=npml)
     # device
     sp = ss(3, Nx, Ny, Nz, dx, dx, dx, epsr)
-    qa = QWArray(QWParams(L=L), MBFlags(), Nw=2, Emax0=pulse["Amp"])
+    qa = QWArray(QWParams(L=L), MBFlags(), Nw=2, Emax0=pulse["Amp"], current=current)
     prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), ps(**pulse), source_type="soft", boundary_type="cpml",
                             calc_j=qa, npml=npml)
     for _ in range(steps):
@@ -648,23 +650,29 @@ def test_propagator_golden_mixed(gold):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("keep_old", [0, 1])
 @pytest.mark.parametrize("Nx,Nw", [(1, 1), (64, 2), (128, 1), (2048, 3)])
-def test_jline_fft_in_kernel(Nx, Nw):
+def test_jline_fft_in_kernel(Nx, Nw, keep_old):
     """pss_jline with fft = 1: the current line and then its forward FFT along x in the
     same kernel (radix-2 in LDS), against the line without FFT and numpy's FFT of it
-    (FP64 rounding of an N log N transform: 1e-14), and the same Pold/RhoOld update."""
+    (FP64 rounding of an N log N transform: 1e-14), and the same Pold/RhoOld update.
+    The line itself against the oracle's CalcJfx and the sums of PSTD3D/rhoPJ.py:1183-1194,
+    with the old arrays kept (keep_old = 1) or zero (keep_old = 0, the reference's
+    re-created PxxOld/RhoOld, PSTD3D/rhoPJ.py:1138-1147)."""
     import torch
 
+    from oracle.coupling import CalcJfx
     from pulsesuite_amd import _lib
 
     rng = np.random.default_rng(Nx + Nw)
     c = lambda *sh: rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
     P, Rho, Pold, RhoOld = c(Nw, 3, Nx), c(Nw, Nx), c(Nw, 3, Nx), c(Nw, Nx)
+    dt, dx = 1e-17, 2e-8
     out = []
     for fft in (0, 1):
         t = [cu(a) for a in (P, Rho, Pold, RhoOld)]
         jl = torch.zeros((3, Nx), dtype=torch.complex128, device="cuda")
-        _lib.check(_lib.lib.pss_jline(*(_lib.ptr(a) for a in t), Nw, Nx, 1e-17, 2e-8, _lib.ptr(jl), fft,
+        _lib.check(_lib.lib.pss_jline(*(_lib.ptr(a) for a in t), Nw, Nx, dt, dx, _lib.ptr(jl), fft, keep_old,
                                       _lib.stream_ptr()), "jline")
         torch.cuda.synchronize()
         out.append((jl.cpu().numpy(), t[2].cpu().numpy(), t[3].cpu().numpy()))
@@ -672,3 +680,7 @@ def test_jline_fft_in_kernel(Nx, Nw):
     assert rel(j1, np.fft.fft(j0, axis=1)) < 1e-14
     assert np.array_equal(po0, P) and np.array_equal(po1, P)
     assert np.array_equal(ro0, Rho) and np.array_equal(ro1, Rho)
+    po, ro = (Pold, RhoOld) if keep_old else (0 * Pold, 0 * RhoOld)
+    want = np.sum(P - po, axis=0) / dt / Nw
+    want[0] += CalcJfx(np.sum(Rho, axis=0), np.sum(ro, axis=0), dt, dx) / Nw
+    assert rel(j0, want) < 1e-13
