@@ -217,6 +217,17 @@ int pss_cpml_e_curl(const pss_grid* g, double* Ex, double* Ey, double* Ez, const
 int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz, const int32_t* sample_idx,
                 const double* sample_w, double* partial, double* sample_out, void* stream);
 
+/* pss_ksample without the partial buffer: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc): the per-block partial sums go to a 6144-double (48 KiB) array in the
+ * library's own device memory, one per GPU, held until the process ends.  The call
+ * allocates nothing.  Same kernels and arithmetic as pss_ksample, so the same result bit
+ * for bit.  Grids: axis lengths powers of two <= 2048 (the partial sums never exceed 6144
+ * doubles).  Calls that overlap on one GPU (two streams, two threads) share the array and
+ * race; serialise them. */
+int pss_ksample_noalloc(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
+                        const int32_t* sample_idx, const double* sample_w, double* sample_out, void* stream);
+
 /* Length in doubles of the psi buffer of pss_cpml_half for this grid (sizing helper of
  * the CPML state of PSTD3D/cpml.py:340-450, which keeps full-grid psi arrays). */
 int64_t pss_cpml_psi_len(const pss_grid* g);

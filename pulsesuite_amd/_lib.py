@@ -89,6 +89,7 @@ def _load():
                                     C.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _i, _vp]),
+        "pss_ksample_noalloc": (C.c_int, [C.POINTER(Grid), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pss_drive_uniform": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_add_scalar": (C.c_int, [_vp, _d, _vp]),
         "pss_rk4_stage": (C.c_int, [_i, C.c_int64, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -108,7 +109,7 @@ def _load():
 lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_sbe_step_p", "pss_vint", "pss_update_e3d", "pss_update_e3d_sample", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_ksample", "pss_b_half", "pss_cpml_e_curl", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
+           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_ksample", "pss_ksample_noalloc", "pss_b_half", "pss_cpml_e_curl", "pss_cpml_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 
 

@@ -5,6 +5,7 @@
 // Fields are complex128, C order (x slowest, z fastest), k-space between
 // half steps exactly as PSTD3D/PSTD3D.py keeps them.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "../../include/pss.h"
@@ -1385,6 +1386,50 @@ extern "C" int pss_ksample(const pss_grid* g, const double* Fx, const double* Fy
   if (!pow2_axes(g) || g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
     return pss_set_msg(2, "pss_ksample: axis lengths must be powers of two <= 2048");
   return ksample_launch(g, Fx, Fy, Fz, sample_idx, sample_w, partial, sample_out, (hipStream_t)stream);
+}
+
+// ---- the point sample without a caller-provided partial-sum buffer
+// The per-block partial sums of k_ksample (at most 1024 blocks x 6 doubles, whatever the
+// grid) live in this array: one copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in symbol); it uses the
+// device address that hipGetSymbolAddress returns for the current GPU, looked up once per GPU
+// on that GPU's first call.  Every call on a GPU shares that GPU's copy: calls that overlap
+// on one GPU (two streams, two threads) race on it.
+#define PSS_KSAMPLE_SCRATCH_DOUBLES 6144
+static __device__ __align__(256) double scratch[PSS_KSAMPLE_SCRATCH_DOUBLES];
+static double* g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_on_current_device(double** out) {
+  int dev = -1;
+  PSS_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return pss_set_msg(2, "pss_ksample_noalloc: device ordinal >= 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return pss_set_error(e, "pss_ksample_noalloc: hipGetSymbolAddress(scratch)");
+    if (!p) return pss_set_msg(2, "pss_ksample_noalloc: null scratch address");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int pss_ksample_noalloc(const pss_grid* g, const double* Fx, const double* Fy, const double* Fz,
+                                   const int32_t* sample_idx, const double* sample_w, double* sample_out,
+                                   void* stream) {
+  if (!pow2_axes(g) || g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
+    return pss_set_msg(2, "pss_ksample_noalloc: axis lengths must be powers of two <= 2048");
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  const int64_t need = std::min<int64_t>(1024, (n + 255) / 256) * 6;
+  if (need > PSS_KSAMPLE_SCRATCH_DOUBLES)
+    return pss_set_msg(2, "pss_ksample_noalloc: partial sums exceed the 6144-double device scratch");
+  double* part = nullptr;
+  const int rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  // every partial slot the final reduction reads is written by k_ksample first: nothing to clear
+  return ksample_launch(g, Fx, Fy, Fz, sample_idx, sample_w, part, sample_out, (hipStream_t)stream);
 }
 
 extern "C" int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out, int32_t nb,
