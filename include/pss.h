@@ -81,28 +81,73 @@ typedef struct pss_wire_desc {
 /* Bytes of workspace one wire needs. */
 int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr, int32_t precision);
 
-/* Split-K factor of the mixed-precision contraction launch (power of two <= 16;
- * 0 = chosen per launch from the tile count).  Process-wide; a tuning knob with no
- * reference counterpart, set by the batch autotuner. */
+/* Split-K factor of the contractions (power of two <= 16; 0 = each wire's own choice from
+ * its Nk, see pss_sbe_step_p).  Process-wide; a tuning knob with no reference counterpart.
+ * A wire's summation order follows it, so results under different settings differ by
+ * rounding. */
 int pss_sbe_tune(int32_t ksplit);
 
 /*
  * One SBE time step of `nwires` wires whose descriptors sit in device memory
  * at `d_descs`.  Replaces, per wire, SBESolver.QWCalculator minus file output
  * (PSTD3D/SBEs.py:5838-5997): Prop2QW, _sbe_calculator, QW2Prop.
- * max_* bound the per-wire sizes in the batch (grid sizing).
+ * min_Nk / max_* bound the per-wire sizes in the batch (grid sizing); min_Nk must not
+ * exceed the smallest wire's Nk.  Every wire's arithmetic depends on its own sizes only
+ * (its split-K factor is chosen from its own Nk), so a wire gives the same result bit for
+ * bit whichever wires share the launch.
  */
-int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
                  int32_t max_nrr, void* stream);
 /* The same step for a batch whose wires hold only the precisions in the mask
  * `precisions` (1: PSS_FP64 wires, 2: PSS_MIXED wires, 3: both): the kernels of an absent
  * precision are not launched (pss_sbe_step launches all of them; each returns at once for
- * wires of the other precision).  ksplit_cap > 0 (a power of two <= 16) caps the split-K
- * factor of the contractions, which is otherwise chosen to fill the chip: a step that
- * runs beside other work (the wire step of the coupled step) does better with fewer
- * workgroups.  pss_sbe_step = precisions 3, no cap. */
-int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
+ * wires of the other precision).  ksplit_cap > 0 (a power of two <= 16) caps every wire's
+ * split-K factor, which is otherwise chosen to fill the chip as if the wire ran alone: a
+ * step that runs beside other work (the wire step of the coupled step) does better with
+ * fewer workgroups.  pss_sbe_step = precisions 3, no cap. */
+int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
                    int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, void* stream);
+
+/*
+ * Kernel-level entries: parts of pss_sbe_step on the wires' own buffers, with the same kernels.
+ * The state buffers rotate with the wire's step counter ctl[0]: "cur" is buffer 1 + (ctl[0] & 1),
+ * "next" buffer 2 - (ctl[0] & 1), "prev" buffer 0.
+ *
+ * pss_calc_h: CalcH (PSTD3D/SBEs.py:820-935, _calc_Meh :5165-5175) of the cur state and the
+ *   propagation lines Exx/Eyy/Ezz, with everything it needs before it (Prop2QW
+ *   PSTD3D/qwoptics.py:673-721, populations and CalcScreenedArrays PSTD3D/coulomb.py:1195-1230,
+ *   the diagonal traces of the direct terms; CalcGammaE/H PSTD3D/dephasing.py:540-624 as a
+ *   by-product).  Results in the workspace: H = (Heh, Hee, Hhh) complex [3][Nk][Nk], the
+ *   screened V(|q|) rows [3][Nk], E1D [Nk], GamE/GamH [2][Nk].  Does not touch the state.
+ * pss_sbe_rhs: dpdt, dCdt, dDdt (PSTD3D/SBEs.py:1953-1976, 2320-2338, 2618-2636) of the cur
+ *   state with the Hamiltonian, dephasing rates and off-diagonal dephasing
+ *   (PSTD3D/dephasing.py:713-742) the step would use, written to the next buffers (dp/dt,
+ *   dC/dt, dD/dt); prev and cur untouched.
+ * pss_coll_rates: MBCE/MBCH (PSTD3D/coulomb.py:957-1071, which & 1) and MBPE/MBPH
+ *   (PSTD3D/phonons.py:140-268, which & 2) of the populations on the next buffers' diagonals,
+ *   with the screened V(|q|) rows and E1D in the workspace: Win_e, Wout_e, Win_h, Wout_h
+ *   [4][Nk] in the workspace (the wire's flags must include the terms asked for).
+ * pss_qwpol: QWPolarization3 and the k sums of QWRho5 (PSTD3D/qwoptics.py:815-946, 948-1077
+ *   before their centred FFTs) of the next buffers p, C, D: P(r) [3][Nr] and rho_e, rho_h
+ *   complex [2][Nr] in the workspace.  Populations are not relaxed or renormalised.
+ * Offsets of the workspace segments: ws_layout in csrc/sbe.hip (mirrored by
+ * pulsesuite_amd/SBEs.py ws_layout).
+ */
+int pss_calc_h(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
+               int32_t max_nrr, int32_t precisions, void* stream);
+int pss_sbe_rhs(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
+                int32_t max_nrr, int32_t precisions, void* stream);
+int pss_coll_rates(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
+                   int32_t max_nrr, int32_t which, void* stream);
+int pss_qwpol(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
+              int32_t max_nrr, void* stream);
+
+/* State of one wire (descriptor at d_desc, device) to / from a packed device buffer
+ * [6][Nk][Nk] complex = (p1, p2, C1, C2, D1, D2) (the reference's YY1/YY2/CC1/CC2/DD1/DD2,
+ * PSTD3D/SBEs.py:4782-4790, as its checkpoint writes them, :5536-5581).  The prev / cur
+ * buffers are resolved on the device from ctl[0], so the copies can be captured. */
+int pss_sbe_get_state(const pss_wire_desc* d_desc, int32_t Nk, double* packed, void* stream);
+int pss_sbe_set_state(const pss_wire_desc* d_desc, int32_t Nk, const double* packed, void* stream);
 
 /*
  * Uniform drive of batched wires: Exx/Eyy/Ezz of every wire are set along the wire to
@@ -241,6 +286,16 @@ int pss_fft_line(const pss_grid* g, const double* in, double* out, int32_t axis,
  * gates is the 3-D FFT of the deposited current (QWPlacement, PSTD3D/rhoPJ.py:736-781). */
 int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out, int32_t nb, int32_t axis,
                        int32_t dir, void* stream);
+
+/* One accumulating pass of a 3-D transform along `axis`, as the last forward pass of the
+ * CPML correction's dF transform runs it (PSTD3D/cpml.py:575-702 add FFT(dF) to the
+ * k-space field): out_b += pass(in_b) for b < nb, in read as complex64 (in32 = 1, axes of
+ * 64 ... 1024) or complex128, out complex128; in is not modified.  With Gx, Gy, Gz non-null
+ * (nb = 3) the curl term coef i (k x G)_b of the E or B update (PSTD3D/PSTD3D.py:29-113) is
+ * added in the same pass (coef = v2 dt for E from B, -dt for B from E).  A kernel-level entry
+ * for tests and for timing that pass alone. */
+int pss_fft_pass_acc(const pss_grid* g, double* const* in, double* const* out, int32_t nb, int32_t axis, int32_t dir,
+                     int32_t in32, const double* Gx, const double* Gy, const double* Gz, double coef, void* stream);
 
 /* Fill three uniform wire lines [3][n] with the point sample (complex [3]): the
  * wire sees E at one point (QWArray.QuantumWire, PSTD3D/rhoPJ.py:1160-1164). */

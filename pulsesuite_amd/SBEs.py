@@ -207,9 +207,48 @@ class DeviceWires:
         sz = C.sizeof(_lib.WireDesc)
         ptr = C.c_void_p(self.d_descs.data_ptr() + lo * sz)
         sub = self.models[lo:hi]
-        _lib.check(_lib.lib.pss_sbe_step_p(ptr, hi - lo, max(m.Nk for m in sub), max(m.Nr for m in sub),
+        _lib.check(_lib.lib.pss_sbe_step_p(ptr, hi - lo, min(m.Nk for m in sub), max(m.Nk for m in sub),
+                                           max(m.Nr for m in sub),
                                            max(self.nrr[lo:hi]), precision_mask(self.prec_code), ksplit_cap,
                                            _lib.stream_ptr(stream)), "pss_sbe_step_p")
+
+    def kernel(self, name, *extra, wires=None, stream=None):
+        """Run a kernel-level entry (pss_calc_h, pss_sbe_rhs, pss_coll_rates, pss_qwpol) on the
+        wires in the contiguous range ``wires`` (default: all); ``extra`` are the entry's
+        arguments after the size bounds (precision mask or rate selection)."""
+        lo, hi = (0, self.Nw) if wires is None else wires
+        sz = C.sizeof(_lib.WireDesc)
+        ptr = C.c_void_p(self.d_descs.data_ptr() + lo * sz)
+        sub = self.models[lo:hi]
+        fn = getattr(_lib.lib, name)
+        _lib.check(fn(ptr, hi - lo, min(m.Nk for m in sub), max(m.Nk for m in sub), max(m.Nr for m in sub),
+                      max(self.nrr[lo:hi]), *extra, _lib.stream_ptr(stream)), name)
+
+    def buffer(self, i, name, which):
+        """Device view of wire i's state buffer ``which`` in ("prev", "cur", "next") of p, C or D
+        (the rotation follows the wire's step counter)."""
+        s = int(self.ctl[i, 0].item()) & 1
+        b = {"prev": 0, "cur": 1 + s, "next": 2 - s}[which]
+        return self.state[i][name][b]
+
+    def packed_state(self, i, stream=None):
+        """(p1, p2, C1, C2, D1, D2) of wire i as one device tensor [6][Nk][Nk] (pss_sbe_get_state)."""
+        Nk = self.models[i].Nk
+        out = torch.empty((6, Nk, Nk), dtype=_CDT, device=self.device)
+        sz = C.sizeof(_lib.WireDesc)
+        _lib.check(_lib.lib.pss_sbe_get_state(C.c_void_p(self.d_descs.data_ptr() + i * sz), Nk, _lib.ptr(out),
+                                              _lib.stream_ptr(stream)), "pss_sbe_get_state")
+        return out
+
+    def load_packed_state(self, i, packed, stream=None):
+        """Write wire i's (p1, p2, C1, C2, D1, D2) from a device tensor [6][Nk][Nk] (pss_sbe_set_state)."""
+        Nk = self.models[i].Nk
+        packed = packed.to(self.device, _CDT).contiguous()
+        if tuple(packed.shape) != (6, Nk, Nk):
+            raise ValueError(f"packed state must be [6][{Nk}][{Nk}]")
+        sz = C.sizeof(_lib.WireDesc)
+        _lib.check(_lib.lib.pss_sbe_set_state(C.c_void_p(self.d_descs.data_ptr() + i * sz), Nk, _lib.ptr(packed),
+                                              _lib.stream_ptr(stream)), "pss_sbe_set_state")
 
     # ------------------------------------------------------------ state access
     def _cur(self, i):

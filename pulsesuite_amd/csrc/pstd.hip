@@ -1444,6 +1444,28 @@ extern "C" int pss_fft_line_batch(const pss_grid* g, double* const* in, double* 
   return 0;
 }
 
+extern "C" int pss_fft_pass_acc(const pss_grid* g, double* const* in, double* const* out, int32_t nb, int32_t axis,
+                                int32_t dir, int32_t in32, const double* Gx, const double* Gy, const double* Gz,
+                                double coef, void* stream) {
+  if (nb < 1 || nb > 3 || axis < 0 || axis > 2 || (dir != 1 && dir != -1) || (in32 != 0 && in32 != 1))
+    return pss_set_msg(2, "pss_fft_pass_acc: 1 <= nb <= 3, axis 0..2, dir +-1, in32 0/1");
+  const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
+  const bool two_level = N == 64 || N == 128 || N == 256 || N == 512 || N == 1024;
+  const bool curl = Gx != nullptr;
+  if (curl && (!Gy || !Gz || nb != 3))
+    return pss_set_msg(2, "pss_fft_pass_acc: the curl term needs the three components of G and nb = 3");
+  if ((curl || in32) && !two_level)
+    return pss_set_msg(2, "pss_fft_pass_acc: complex64 input and the curl term need an axis of 64 ... 1024");
+  const cplx* ins[3] = {(const cplx*)in[0], nb > 1 ? (const cplx*)in[1] : nullptr, nb > 2 ? (const cplx*)in[2] : nullptr};
+  cplx* outs[3] = {(cplx*)out[0], nb > 1 ? (cplx*)out[1] : nullptr, nb > 2 ? (cplx*)out[2] : nullptr};
+  const CurlArgs cu{{(const cplx*)Gx, (const cplx*)Gy, (const cplx*)Gz}, coef};
+  const int rc = launch_pass_b(g, ins, outs, nb, axis, dir, nullptr, -1, (hipStream_t)stream, curl ? 3 : 1, in32, 0, 0,
+                               curl ? &cu : nullptr);
+  if (rc) return pss_set_msg(2, "pss_fft_pass_acc: bad axis length");
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
 // cu: the curl update of F from G, fused into the last forward pass (null: none)
 static int cpml_half_impl(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
                    const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,

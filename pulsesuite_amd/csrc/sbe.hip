@@ -11,6 +11,7 @@
 // and next (X3) swap between buffers 1 and 2 by the parity of the wire's step
 // counter ctl[0]; *on is the solver-wide latch of the reference's wireoff gate.
 #include <algorithm>
+#include <stdio.h>
 
 #include "common.h"
 #include "../../include/pss.h"
@@ -652,6 +653,31 @@ __global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict_
   H[2 * NN + o] = hhh;
 }
 
+__host__ __device__ inline int np64(int Nk) { return (Nk + 63) / 64 * 64; }
+
+__host__ __device__ inline int ksplit_max(int Nk) {
+  const int np = np64(Nk);
+  return np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
+}
+
+// Split-K factor of one wire's contractions, a function of that wire alone (so a wire's
+// summation order, and its result, do not depend on which wires share the launch):
+// knob > 0 forces it (pss_sbe_tune), else enough workgroups to fill the chip were the wire
+// alone (3 t64^2 ks >= 1024); capped by its partial buffer (ksplit_max), by cap > 0 (a step
+// that runs beside other work) and by the launch's slice count KS.  Non-increasing in Nk.
+__host__ __device__ inline int wire_ks(int Nk, int knob, int cap, int KS) {
+  int ks = knob;
+  if (ks <= 0) {
+    const int t64 = np64(Nk) / 64;
+    const int64_t tiles = (int64_t)t64 * t64 * 3;
+    ks = 1;
+    while (ks < ksplit_max(Nk) && tiles * ks < 1024) ks *= 2;
+  }
+  ks = ks < ksplit_max(Nk) ? ks : ksplit_max(Nk);
+  if (cap > 0 && ks > cap) ks = cap;
+  return ks < KS ? ks : KS;
+}
+
 // ------------------------------- K5b: exchange sums as Toeplitz MFMA products
 // For a matrix X (p, C or D) and potential row V, the exchange term of CalcH
 // (PSTD3D/SBEs.py:820-907) is  E[k1][k2] = sum_{q != 0} V(|q|) X[k2+q][k1+q].
@@ -810,7 +836,7 @@ __device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, cons
 // FP64 leapfrog epilogue of one element with the products' sum S (the Hee / Hhh
 // diagonals are inside the products): + Heh^T - i hbar gamma p for dp.
 __device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, const cplx* H, const double* gam,
-                                     int out, int i, int j, cplx S) {
+                                     int out, int i, int j, cplx S, int deriv) {
   const int Nk = d.Nk;
   const int64_t o = (int64_t)i * Nk + j;
   const cplx* X1 = out == 0 ? B.p1 : out == 1 ? B.C1 : B.D1;
@@ -824,6 +850,10 @@ __device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, cons
     S = S + hT - mk(-g * pv.y, g * pv.x);
   }
   const cplx R = div_ih(S, kHbar);
+  if (deriv) {  // kernel-level entry: dX/dt into the next buffer, nothing integrated
+    Xn[o] = R;
+    return;
+  }
   const cplx x1 = X1[o];
   const cplx x3 = x1 + (R * d.dt) * 2.0;
   Xn[o] = x3;
@@ -835,12 +865,15 @@ __device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, cons
 // k chunk staged in LDS, shared by the waves; coalesced loads along the contiguous
 // index of op(X).  The (term, chunk) list is cut into `ks` slices (blockIdx.z); with
 // ks == 1 the epilogue runs here, else FP64 partial tiles go to the pk buffer.
-__global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs, int ks) {
+__global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restrict__ descs, int KS, int knob,
+                                                   int cap, int deriv) {
   const int zz = blockIdx.z;
-  const int slice = zz % ks, out = (zz / ks) % 3, wire = zz / (3 * ks);
+  const int slice = zz % KS, out = (zz / KS) % 3, wire = zz / (3 * KS);
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || d.precision == PSS_MIXED) return;
   const int Nk = d.Nk;
+  const int ks = wire_ks(Nk, knob, cap, KS);  // this wire's own split-K factor
+  if (slice >= ks) return;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   if (m0 >= Nk || n0 >= Nk) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
@@ -919,7 +952,7 @@ __global__ void __launch_bounds__(256) k_rhs_gemm(const pss_wire_desc* __restric
         const int j = n0 + wc * 32 + y * 16 + (lane & 15);
         if (i >= Nk || j >= Nk) continue;
         const cplx S = mk(accr[x][y][v], acci[x][y][v]);
-        if (ks == 1) fp64_epilogue(d, B, H, gam, out, i, j, S);
+        if (ks == 1) fp64_epilogue(d, B, H, gam, out, i, j, S, deriv);
         else pk[(int64_t)i * Nk + j] = S;
       }
 }
@@ -1004,12 +1037,6 @@ __device__ inline void split16(double x, _Float16& hi, _Float16& lo) {
 // im lo) in two orientations: o = 0 row-major X[i][k], o = 1 transposed X[k][i].
 // Planes are NP x NP (NP = Nk rounded up to 64) with zeros outside Nk, so the
 // contraction kernel loads whole 16-wide k segments without bounds checks.
-__host__ __device__ inline int np64(int Nk) { return (Nk + 63) / 64 * 64; }
-
-__host__ __device__ inline int ksplit_max(int Nk) {
-  const int np = np64(Nk);
-  return np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
-}
 
 __device__ inline _Float16* plane(double* ws, const WS& L, int NP, int m, int o, int q) {
   return (_Float16*)(ws + L.sp) + ((int64_t)((m * 2 + o) * 4 + q)) * NP * NP;
@@ -1084,7 +1111,7 @@ __device__ inline void b_side(int src, int op, int& m, int& o, bool& cj) {
 // diagonal parts Hhh[kh,kh] p + p Hee[ke,ke] (+ Heh^T - i hbar gamma p) for dp,
 // X (Hd[j,j] - Hd[i,i]) for dC / dD; R = S / (i hbar); X3 = X1 + 2 dt R.
 __device__ inline void mixed_epilogue(const pss_wire_desc& d, const Bufs& B, const cplx* H, const double* gam,
-                                      int out, int i, int j, cplx S) {
+                                      int out, int i, int j, cplx S, int deriv) {
   const int Nk = d.Nk;
   const int64_t NN = (int64_t)Nk * Nk;
   const int64_t o = (int64_t)i * Nk + j;
@@ -1104,6 +1131,10 @@ __device__ inline void mixed_epilogue(const pss_wire_desc& d, const Bufs& B, con
     S = S + x2 * (Hd[(int64_t)j * Nk + j] - Hd[(int64_t)i * Nk + i]);
   }
   const cplx R = div_ih(S, kHbar);
+  if (deriv) {  // kernel-level entry: dX/dt into the next buffer, nothing integrated
+    Xn[o] = R;
+    return;
+  }
   const cplx x1 = X1[o];
   const cplx x3 = x1 + (R * d.dt) * 2.0;
   Xn[o] = x3;
@@ -1126,12 +1157,15 @@ constexpr int kFold = 4;       // stages per FP32 partial
 
 __device__ inline f4 mfma32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
-__global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict__ descs, int ks) {
+__global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict__ descs, int KS, int knob, int cap,
+                                                 int deriv) {
   const int zz = blockIdx.z;
-  const int slice = zz % ks, out = (zz / ks) % 3, wire = zz / (3 * ks);
+  const int slice = zz % KS, out = (zz / KS) % 3, wire = zz / (3 * KS);
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || d.precision != PSS_MIXED) return;
   const int Nk = d.Nk, NP = np64(Nk);
+  const int ks = wire_ks(Nk, knob, cap, KS);  // this wire's own split-K factor
+  if (slice >= ks) return;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   if (m0 >= NP || n0 >= NP) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
@@ -1285,7 +1319,7 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
         const int j = n0 + wc * 32 + y * 16 + (lane & 15);
         if (i >= Nk || j >= Nk) continue;
         const cplx S = mk(accr[x][y][v], acci[x][y][v]);
-        if (ks == 1) mixed_epilogue(d, B, H, gam, out, i, j, S);
+        if (ks == 1) mixed_epilogue(d, B, H, gam, out, i, j, S, deriv);
         else pk[(int64_t)i * Nk + j] = S;
       }
 }
@@ -1502,11 +1536,15 @@ __global__ void __launch_bounds__(256) k_exch_mx(const pss_wire_desc* __restrict
       }
 }
 
-// sum of the ks partial tiles (fixed order) + epilogue; grid (ceil(NN/256), 3, nwires)
-__global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict__ descs, int ks) {
+// sum of the wire's ks partial tiles (fixed order) + epilogue; grid (ceil(NN/256), 3, nwires);
+// a wire with ks = 1 had its epilogue in the contraction kernel
+__global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict__ descs, int KS, int knob, int cap,
+                                                  int deriv) {
   const pss_wire_desc& d = descs[blockIdx.z];
   if (!wire_on(d)) return;
   const int Nk = d.Nk, out = blockIdx.y;
+  const int ks = wire_ks(Nk, knob, cap, KS);
+  if (ks == 1) return;
   const int64_t NN = (int64_t)Nk * Nk;
   const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (o >= NN) return;
@@ -1516,9 +1554,9 @@ __global__ void __launch_bounds__(256) k_rhs_epi(const pss_wire_desc* __restrict
   cplx S = mk(0, 0);
   for (int s = 0; s < ks; ++s) S = S + pk[((int64_t)s * 3 + out) * NN + o];
   if (d.precision == PSS_MIXED)
-    mixed_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
+    mixed_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S, deriv);
   else
-    fp64_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S);
+    fp64_epilogue(d, bufs(d), (const cplx*)(ws + L.H), ws + L.gam, out, (int)(o / Nk), (int)(o % Nk), S, deriv);
 }
 
 // --------------------------- K7: carrier-carrier + phonon collision rates
@@ -1661,9 +1699,9 @@ __device__ inline void coll_pair(const CollCtx& x, bool holes, int Nk, int o, in
   }
 }
 
-__global__ void __launch_bounds__(256) k_coll(const pss_wire_desc* __restrict__ descs) {
+__global__ void __launch_bounds__(256) k_coll(const pss_wire_desc* __restrict__ descs, int fmask) {
   const pss_wire_desc& d = descs[blockIdx.z];
-  if (!wire_on(d) || !(d.flags & PSS_EHS)) return;
+  if (!wire_on(d) || !(d.flags & fmask & PSS_EHS)) return;
   const int Nk = d.Nk;
   const int o = (int)blockIdx.x + 1;  // 1-based output momentum
   if (o > Nk) return;
@@ -1719,9 +1757,10 @@ __global__ void __launch_bounds__(256) k_coll(const pss_wire_desc* __restrict__ 
 //   MBPE: Win[k] += sum_i Vep[k,i] n[i] EP[k,i];  Wout[k] += sum_i Vep[k,i] (1-n[i]) EP[i,k]
 //   MBPH: Win[k] += sum_i Vhp[i,k] n[i] HP[k,i];  Wout[k] += sum_i Vhp[i,k] (1-n[i]) HP[i,k]
 //   EP[a,b] = 2 Gph (N0/((E[a]-E[b]-hO)^2+hG^2) + (N0+1)/((E[a]-E[b]+hO)^2+hG^2))
-__global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ descs) {
+__global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ descs, int fmask) {
   const pss_wire_desc& d = descs[blockIdx.z];
-  if (!wire_on(d) || !(d.flags & (PSS_EHS | PSS_PHONON))) return;
+  const int flags = d.flags & fmask;
+  if (!wire_on(d) || !(flags & (PSS_EHS | PSS_PHONON))) return;
   const int Nk = d.Nk;
   const int kk = blockIdx.x;
   if (kk >= Nk) return;
@@ -1731,11 +1770,11 @@ __global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ 
   double* ws = d.ws;
   const Bufs B = bufs(d);
   double win = 0.0, wout = 0.0;
-  if (d.flags & PSS_EHS) {
+  if (flags & PSS_EHS) {
     const double* pout = ws + L.cp + (holes ? (int64_t)Nk * Nk : 0) + (int64_t)kk * Nk;
     for (int i = threadIdx.x; i < Nk; i += 256) wout += pout[i];
   }
-  if (d.flags & PSS_PHONON) {
+  if (flags & PSS_PHONON) {
     const double* E = holes ? d.Eh : d.Ee;
     const double* e1d = ws + L.e1d;
     const double* vrow = ws + L.vc + (holes ? 2 : 1) * Nk;
@@ -1761,7 +1800,7 @@ __global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ 
   if (threadIdx.x == 0) {
     double* W = ws + L.W;
     const int iw = (holes ? 2 : 0) * Nk + kk, iwo = (holes ? 3 : 1) * Nk + kk;
-    W[iw] = ((d.flags & PSS_EHS) ? W[iw] : 0.0) + win;
+    W[iw] = ((flags & PSS_EHS) ? W[iw] : 0.0) + win;
     W[iwo] = wout;
   }
 }
@@ -1820,7 +1859,7 @@ __device__ void pops_body(const pss_wire_desc& d, double* sh) {
 // QWPolarization3 / QWRho5 depend on k_e - k_h (resp. k2 - k1) only through
 // exp(i (k_e - k_h) r) with k_e - k_h = dd * dkr, so the O(Nr Nk^2) sums of
 // PSTD3D/qwoptics.py:56-82, 367-380 reduce to sums along diagonals dd.
-__global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restrict__ descs) {
+__global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restrict__ descs, int pops) {
   const pss_wire_desc& d = descs[blockIdx.y];
   if (!wire_on(d)) return;
   const int Nk = d.Nk, nd = 2 * Nk - 1;
@@ -1830,7 +1869,7 @@ __global__ void __launch_bounds__(256) k_diag_sums(const pss_wire_desc* __restri
   double* ws = d.ws;
   const Bufs B = bufs(d);
   const int dd = (int)blockIdx.x - (Nk - 1);
-  if (dd == 0) {  // K8 on the main diagonal first: its sums read the new populations
+  if (dd == 0 && pops) {  // K8 on the main diagonal first: its sums read the new populations
     pops_body(d, sh);
     __syncthreads();
   }
@@ -2116,7 +2155,7 @@ extern "C" int64_t pss_sbe_workspace_bytes(int32_t Nk, int32_t Nr, int32_t nrr, 
 }
 
 namespace {
-int g_ksplit = 0;  // 0: choose per launch
+int g_ksplit = 0;  // 0: each wire chooses from its own Nk (wire_ks)
 }
 
 extern "C" int pss_sbe_tune(int32_t ksplit) {
@@ -2126,36 +2165,65 @@ extern "C" int pss_sbe_tune(int32_t ksplit) {
   return 0;
 }
 
-extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
-                              int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, void* stream) {
+// Stages of the step, in order (bits of a mask):
+//   FRONT   K1: wireoff gate, Prop2QW splines onto r; K2: populations, screened V, E1D; K1b: E(q)
+//   TRACES  K3: diagonal traces (direct terms), CalcGammaE/H, max |p|, |C|, |D| of mixed wires
+//   OFFDIAG K4: off-diagonal dephasing tables De, Dh
+//   CALCH   K5: Meh, band energies, direct terms; K5b: exchange (Toeplitz MFMA products)
+//   RHS     K6: the dpdt/dCdt/dDdt contractions and their epilogue (leapfrog, or dX/dt only)
+//   COLL    K7: collision and phonon rates Win/Wout from the populations of the next state
+//   TAIL    K8 relaxation + renormalisation, K9 diagonal sums, P(r) and rho(r), QW2Prop
+enum { ST_FRONT = 1, ST_TRACES = 2, ST_OFFDIAG = 4, ST_CALCH = 8, ST_RHS = 16, ST_COLL = 32, ST_TAIL = 64,
+       ST_ALL = 127 };
+// opt: bit 0 = dX/dt only in the RHS epilogue (no leapfrog); bit 1 = the tail's diagonal sums
+// without K8 (populations left as they are); bits 8.. = mask on the descriptor flags of K7
+static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
+                      int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, int stages, int opt,
+                      hipStream_t s, const char* who) {
   if (nwires <= 0) return 0;
-  if (precisions < 1 || precisions > 3)
-    return pss_set_msg(2, "pss_sbe_step_p: precisions is a mask of 1 (FP64 wires) and 2 (mixed wires)");
-  if (ksplit_cap < 0 || ksplit_cap > 16 || (ksplit_cap & (ksplit_cap - 1)))
-    return pss_set_msg(2, "pss_sbe_step_p: ksplit_cap must be 0 (none) or a power of two <= 16");
+  char msg[160];
+  if (min_Nk < 4 || min_Nk > max_Nk) {
+    snprintf(msg, sizeof(msg), "%s: need 4 <= min_Nk <= max_Nk", who);
+    return pss_set_msg(2, msg);
+  }
+  if (precisions < 1 || precisions > 3) {
+    snprintf(msg, sizeof(msg), "%s: precisions is a mask of 1 (FP64 wires) and 2 (mixed wires)", who);
+    return pss_set_msg(2, msg);
+  }
+  if (ksplit_cap < 0 || ksplit_cap > 16 || (ksplit_cap & (ksplit_cap - 1))) {
+    snprintf(msg, sizeof(msg), "%s: ksplit_cap must be 0 (none) or a power of two <= 16", who);
+    return pss_set_msg(2, msg);
+  }
+  if (max_Nk > 4096 || max_Nr > 8194 || max_nrr < 4) {
+    snprintf(msg, sizeof(msg), "%s: sizes out of range (4 <= Nk <= 4096, nrr >= 4)", who);
+    return pss_set_msg(2, msg);
+  }
   const bool any64 = precisions & 1, anymx = precisions & 2;
-  if (max_Nk < 4 || max_Nk > 4096 || max_Nr > 8194 || max_nrr < 4)
-    return pss_set_msg(2, "pss_sbe_step: sizes out of range (4 <= Nk <= 4096, nrr >= 4)");
-  hipStream_t s = (hipStream_t)stream;
+  const int deriv = opt & 1, pops = (opt & 2) ? 0 : 1;
+  const int fmask = (opt >> 8) ? (opt >> 8) : ~0;
   const int nd = 2 * max_Nk - 1;
   const size_t lds_rel = 2 * ((size_t)max_Nk + 1) * sizeof(double2);
   // gfx950 gives a workgroup up to 160 KB of LDS on request; raised once per process
   // for the largest size seen (outside any stream work, so a captured step is unaffected)
   static size_t lds_granted = 65536;
-  if (lds_rel > lds_granted) {
+  if ((stages & ST_COLL) && lds_rel > lds_granted) {
     PSS_CHECK(hipFuncSetAttribute((const void*)k_coll, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rel));
     lds_granted = lds_rel;
   }
-  hipLaunchKernelGGL(k_prop_front, dim3(2, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
+  if (stages & ST_FRONT) {
+    hipLaunchKernelGGL(k_prop_front, dim3(2, nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_prop_dft, dim3(nd, nwires), dim3(256), 0, s, d_descs);
+  }
   // max |p|, |C|, |D| of the mixed wires in 3 x 64 extra workgroups (after K2 cleared the slots)
-  hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk + (anymx ? 3 * 64 : 0), nwires), dim3(256), 0, s, d_descs,
-                     nd + max_Nk);
-  hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
-                     (size_t)(256 + max_Nk) * sizeof(double), s, d_descs);
-  const int t16 = (max_Nk + 15) / 16;
-  hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
-  {
+  if (stages & ST_TRACES)
+    hipLaunchKernelGGL(k_traces_gamma, dim3(nd + max_Nk + (anymx ? 3 * 64 : 0), nwires), dim3(256), 0, s, d_descs,
+                       nd + max_Nk);
+  if (stages & ST_OFFDIAG)
+    hipLaunchKernelGGL(k_offdiag, dim3((max_Nk + 255) / 256, 2 * max_Nk + 1, 2 * nwires), dim3(256),
+                       (size_t)(256 + max_Nk) * sizeof(double), s, d_descs);
+  if (stages & ST_CALCH) {
+    const int t16 = (max_Nk + 15) / 16;
+    hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
     const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
     const size_t lds = (size_t)max_Nk * sizeof(double);
     // each kernel below returns at once for wires of the other precision; launched only
@@ -2174,40 +2242,117 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
       hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
     }
   }
-  {
+  if (stages & ST_RHS) {
     const int np = np64(max_Nk), t64 = np / 64;
-    // split-K: enough workgroups to fill the chip, within every wire's partial buffer
-    int ks = g_ksplit;
-    if (ks == 0) {
-      ks = 1;
-      const int64_t tiles = (int64_t)t64 * t64 * 3 * nwires;
-      while (ks < ksplit_max(max_Nk) && tiles * ks < 1024) ks *= 2;
-    }
-    ks = std::min(ks, ksplit_max(max_Nk));
-    if (ksplit_cap > 0) ks = std::min(ks, (int)ksplit_cap);
-    if (any64) hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+    // split-K: each wire's own factor (wire_ks), the launch sized for the largest of them,
+    // which the smallest wire has
+    const int KS = wire_ks(min_Nk, g_ksplit, ksplit_cap, 16);
+    if (any64)
+      hipLaunchKernelGGL(k_rhs_gemm, dim3(t64, t64, 3 * nwires * KS), dim3(256), 0, s, d_descs, KS, g_ksplit,
+                         (int)ksplit_cap, deriv);
     if (anymx) {
       hipLaunchKernelGGL(k_maxabs, dim3(64, 5, nwires), dim3(256), 0, s, d_descs, 3);  // Heh .. hbar De
       hipLaunchKernelGGL(k_split, dim3(np / 32, np / 32, 8 * nwires), dim3(256), 0, s, d_descs);
-      hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * ks), dim3(256), 0, s, d_descs, ks);
+      hipLaunchKernelGGL(k_rhs_mx, dim3(t64, t64, 3 * nwires * KS), dim3(256), 0, s, d_descs, KS, g_ksplit,
+                         (int)ksplit_cap, deriv);
     }
-    if (ks > 1) {
+    if (KS > 1) {
       const int64_t nn = (int64_t)max_Nk * max_Nk;
-      hipLaunchKernelGGL(k_rhs_epi, dim3((unsigned)((nn + 255) / 256), 3, nwires), dim3(256), 0, s, d_descs, ks);
+      hipLaunchKernelGGL(k_rhs_epi, dim3((unsigned)((nn + 255) / 256), 3, nwires), dim3(256), 0, s, d_descs, KS,
+                         g_ksplit, (int)ksplit_cap, deriv);
     }
   }
-  hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs);
-  hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
-  hipLaunchKernelGGL(k_rho_qw2prop, dim3(nwires), dim3(320), 0, s, d_descs);
+  if (stages & ST_COLL) {
+    hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs, fmask);
+    hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs, fmask);
+  }
+  if (stages & ST_TAIL) {
+    hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs, pops);
+    hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
+    if (pops) hipLaunchKernelGGL(k_rho_qw2prop, dim3(nwires), dim3(320), 0, s, d_descs);
+  }
   PSS_CHECK(hipGetLastError());
   return 0;
 }
 
-extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t max_Nk, int32_t max_Nr,
-                            int32_t max_nrr, void* stream) {
-  return pss_sbe_step_p(d_descs, nwires, max_Nk, max_Nr, max_nrr, 3, 0, stream);
+extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                              int32_t max_Nr, int32_t max_nrr, int32_t precisions, int32_t ksplit_cap,
+                              void* stream) {
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, precisions, ksplit_cap, ST_ALL, 0,
+                    (hipStream_t)stream, "pss_sbe_step_p");
+}
+
+extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                            int32_t max_Nr, int32_t max_nrr, void* stream) {
+  return pss_sbe_step_p(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, stream);
+}
+
+// ---- kernel-level entries: parts of the step on the wires' own buffers (for tests and for
+// callers of the reference's module functions); the same kernels as the step
+extern "C" int pss_calc_h(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                          int32_t max_Nr, int32_t max_nrr, int32_t precisions, void* stream) {
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, precisions, 0, ST_FRONT | ST_TRACES | ST_CALCH,
+                    0, (hipStream_t)stream, "pss_calc_h");
+}
+
+extern "C" int pss_sbe_rhs(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                           int32_t max_Nr, int32_t max_nrr, int32_t precisions, void* stream) {
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, precisions, 0,
+                    ST_FRONT | ST_TRACES | ST_OFFDIAG | ST_CALCH | ST_RHS, 1, (hipStream_t)stream, "pss_sbe_rhs");
+}
+
+extern "C" int pss_coll_rates(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                              int32_t max_Nr, int32_t max_nrr, int32_t which, void* stream) {
+  if (which < 1 || which > 3) return pss_set_msg(2, "pss_coll_rates: which is a mask of 1 (MBCE/MBCH), 2 (MBPE/MBPH)");
+  const int fm = ((which & 1) ? PSS_EHS : 0) | ((which & 2) ? PSS_PHONON : 0);
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, ST_COLL, fm << 8, (hipStream_t)stream,
+                    "pss_coll_rates");
+}
+
+extern "C" int pss_qwpol(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                         int32_t max_Nr, int32_t max_nrr, void* stream) {
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, ST_TAIL, 2, (hipStream_t)stream,
+                    "pss_qwpol");
+}
+
+// ---- state access: (p1, p2, C1, C2, D1, D2) of one wire <-> a packed [6][Nk][Nk] complex
+// buffer; the prev / cur buffers are resolved on the device from the wire's step counter,
+// so the copy can sit in a captured graph
+__global__ void __launch_bounds__(256) k_state_copy(const pss_wire_desc* __restrict__ descs, double* packed,
+                                                    int to_packed) {
+  const pss_wire_desc& d = descs[0];
+  const int Nk = d.Nk;
+  const int64_t NN = (int64_t)Nk * Nk;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int m = blockIdx.y;  // 0 p1, 1 p2, 2 C1, 3 C2, 4 D1, 5 D2
+  if (o >= NN) return;
+  const int cur = 1 + (d.ctl[0] & 1);
+  double* const* base = m < 2 ? d.p : m < 4 ? d.C : d.D;
+  cplx* X = (cplx*)base[(m & 1) ? cur : 0];
+  cplx* P = (cplx*)packed + (int64_t)m * NN;
+  if (to_packed) P[o] = X[o];
+  else X[o] = P[o];
+}
+
+static int state_copy(const pss_wire_desc* d_desc, int32_t Nk, double* packed, int to_packed, hipStream_t s,
+                      const char* who) {
+  if (!d_desc || !packed || Nk < 4 || Nk > 4096) {
+    char msg[128];
+    snprintf(msg, sizeof(msg), "%s: null pointer or Nk outside 4 ... 4096", who);
+    return pss_set_msg(2, msg);
+  }
+  const int64_t NN = (int64_t)Nk * Nk;
+  hipLaunchKernelGGL(k_state_copy, dim3((unsigned)((NN + 255) / 256), 6), dim3(256), 0, s, d_desc, packed, to_packed);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_sbe_get_state(const pss_wire_desc* d_desc, int32_t Nk, double* packed, void* stream) {
+  return state_copy(d_desc, Nk, packed, 1, (hipStream_t)stream, "pss_sbe_get_state");
+}
+
+extern "C" int pss_sbe_set_state(const pss_wire_desc* d_desc, int32_t Nk, const double* packed, void* stream) {
+  return state_copy(d_desc, Nk, (double*)packed, 0, (hipStream_t)stream, "pss_sbe_set_state");
 }
 
 extern "C" int pss_drive_uniform(const pss_wire_desc* d_descs, const double* d_pulses, int32_t nwires,

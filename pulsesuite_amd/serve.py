@@ -16,7 +16,8 @@ solve plus the batched SBE launches.
 
 Requests that arrive while others run are admitted at the next tick when a slot is
 free and their device memory fits in the HBM budget (first fit in arrival order);
-finished solves leave at once and their memory returns to the budget.  A wire's
+finished solves leave the batch at once, and their memory returns to the budget when
+the GPU has finished their last step (an event per solve).  A wire's
 arithmetic does not depend on which other wires share its launches, so a result
 is the same alone and in the mix.
 
@@ -51,11 +52,20 @@ DT = 0.9 * 2.0 * DX / (math.pi * 2.99792458e8)
 
 @dataclasses.dataclass
 class Request:
+    """One solve: wire of Nk points, steps, source amplitude E0 (V/m), arrival time (s after
+    the server starts) and the wire's sample point x0 on the line (m; 0 = line centre)."""
     rid: int
     Nk: int
     steps: int
     E0: float = 1e7
-    arrival: float = 0.0  # seconds after the server starts
+    arrival: float = 0.0
+    x0: float = 0.0
+
+
+def source_plane_x():
+    """x of the soft source's plane on the served line (cell round(0.25 NX), PSTD3D/tfsf.py:26-60)."""
+    x = np.linspace(-DX * (NX - 1) / 2.0, DX * (NX - 1) / 2.0, NX)
+    return float(x[int(round(0.25 * NX))])
 
 
 def poisson_requests(n, rate, seed=0, nk_lo=64, nk_hi=2048, st_lo=500, st_hi=20000, e_lo=1e6, e_hi=1e8):
@@ -90,7 +100,7 @@ class Solve:
         self.req = req
         sp = ss(3, NX, 1, 1, DX, DX, DX, EPSR)
         mbf = dataclasses.replace(mb, ignorewire=not feedback)
-        self.qa = QWArray(QWParams(L=L_of_nk(req.Nk)), mbf, Nw=1, Emax0=0.0, precision=precision)
+        self.qa = QWArray(QWParams(L=L_of_nk(req.Nk)), mbf, Nw=1, Emax0=0.0, precision=precision, x0=req.x0)
         self.prop = PSTD3DPropagator(sp, ts(0.0, 1.0, DT, 0), ps(800e-9, req.E0, 100e-15, 50e-15, 0.0),
                                      source_type="soft", boundary_type="cpml", calc_j=self.qa, npml=NPML,
                                      check_every=0, device=device, precision=precision, device_time=True)
@@ -230,8 +240,9 @@ class Server:
             sub = self.active[lo:hi]
             m = [s.qa.model for s in sub]
             st = _lib.stream_ptr(self._pool[b % ns] if ns > 1 else main)
-            _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(base + lo * sz), hi - lo, max(x.Nk for x in m),
-                                               max(x.Nr for x in m), NX, self._pmask, 0, st), "pss_sbe_step_p")
+            _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(base + lo * sz), hi - lo, min(x.Nk for x in m),
+                                               max(x.Nk for x in m), max(x.Nr for x in m), NX, self._pmask, 0, st),
+                       "pss_sbe_step_p")
         if ns > 1:
             for j in range(ns):
                 self._ev_join[j].record(self._pool[j])
@@ -315,7 +326,6 @@ class Server:
                         ev.record()
                         s.event = ev
                         draining.append(s)
-                        self.used -= s.nbytes
                         self._dirty = True
                     else:
                         left.append(s)
@@ -327,6 +337,9 @@ class Server:
             keep = []
             for s in draining:
                 if s.event.query():
+                    # its device memory goes back to the budget only now: until its last
+                    # step has run, the solve's buffers are still in use
+                    self.used -= s.nbytes
                     done_t = time.perf_counter() - t0
                     r = s.req
                     self.records.append(dict(rid=r.rid, Nk=r.Nk, steps=r.steps, arrival=r.arrival,

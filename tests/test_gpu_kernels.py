@@ -1,0 +1,201 @@
+"""Kernel-level C-ABI entries (pss_calc_h, pss_sbe_rhs, pss_coll_rates, pss_qwpol,
+pss_sbe_get_state / pss_sbe_set_state) against the FP64 oracle's functions on seeded inputs.
+
+They run the step's own kernels on a wire's buffers: CalcH with its screening and traces,
+dpdt / dCdt / dDdt with the Hamiltonian and dephasing the step uses, MBCE / MBCH and
+MBPE / MBPH, the k sums of QWPolarization3 / QWRho5.  Bars: FP64 1e-12 (relative L2);
+mixed precision (FP16 hi/lo contractions) 1e-6 of each output's norm.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import sbe as O
+from oracle.helpers import iFFTG
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def herm(n, rng, top=0.5):
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Q, _ = np.linalg.qr(X)
+    return (Q * rng.uniform(0, top, n)) @ Q.conj().T
+
+
+def make(nk, precision="fp64", mb_kw=None, seed=3):
+    """A one-wire solver and the oracle with the same random state and field lines."""
+    import torch
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import MBFlags, QWParams
+
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, 48)
+    mb_kw = mb_kw or {}
+    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, 1, O.QWParams(L=L), O.MBFlags(**mb_kw))
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, 1, True, qw_params=QWParams(L=L), mb_flags=MBFlags(**mb_kw),
+                    precision=precision)
+    rng = np.random.default_rng(seed)
+    st = dict(p1=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+              p2=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+              C1=herm(nk, rng), C2=herm(nk, rng), D1=herm(nk, rng), D2=herm(nk, rng))
+    gpu.set_state(1, **st)
+    E = (3e7 * np.exp(-((rr / (1.5 * L)) ** 2)) * np.cos(2.0 * rr / L)).astype(complex)
+    lines = (E, 0.5 * E, 0.3 * E)
+    dev = gpu.dev
+    for c in range(3):
+        dev.E_in[0][c].copy_(torch.as_tensor(lines[c], device=dev.device))
+    dev.force_on()
+    return ref, gpu, st, lines, rr
+
+
+def oracle_h(ref, st, lines, rr):
+    Ex, Ey, Ez = ref.Prop2QW(rr, *lines)
+    Meh = ref.calc_Meh(Ex, Ey, Ez, 1)
+    ne, nh = np.diag(st["C2"]).copy(), np.diag(st["D2"]).copy()
+    VC, E1D = ref.CalcScreenedArrays(ne, nh)
+    H = ref.CalcH(Meh, st["C2"], st["D2"], st["p2"], VC)
+    GE, GH = ref.CalcGammaE(ne, nh, VC), ref.CalcGammaH(ne, nh, VC)
+    return ne, nh, VC, E1D, H, GE, GH
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_calc_h_entry(precision, tol):
+    import torch
+
+    nk = 40
+    ref, gpu, st, lines, rr = make(nk, precision)
+    dev = gpu.dev
+    before = dev.packed_state(0).clone()
+    dev.kernel("pss_calc_h", 3)
+    torch.cuda.synchronize()
+    ne, nh, VC, E1D, H, GE, GH = oracle_h(ref, st, lines, rr)
+    vc = dev.workspace_view(0, "vc").cpu().numpy().reshape(3, nk)
+    for n in range(3):
+        assert rel(vc[n], VC[0, :, n]) < 1e-13
+    assert rel(dev.workspace_view(0, "e1d").cpu().numpy(), E1D[0]) < 1e-13
+    Hd = dev.workspace_view(0, "H").cpu().numpy().view(complex).reshape(3, nk, nk)
+    for name, a, b in zip(("Heh", "Hee", "Hhh"), Hd, H):
+        assert rel(a, b) < tol, (name, rel(a, b))
+    gam = dev.workspace_view(0, "gam").cpu().numpy()
+    assert rel(gam[:nk], GE) < 1e-12 and rel(gam[nk:], GH) < 1e-12
+    assert torch.equal(dev.packed_state(0), before)  # the state is not touched
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_sbe_rhs_entry(precision, tol):
+    """dp/dt, dC/dt, dD/dt of the current state into the next buffers: the oracle's dpdt,
+    dCdt, dDdt (PSTD3D/SBEs.py:1953-1976, 2320-2338, 2618-2636) with its CalcH, CalcGammaE/H
+    and OffDiagDephasing2 of the same state; prev and cur untouched."""
+    import torch
+
+    nk = 40
+    ref, gpu, st, lines, rr = make(nk, precision)
+    dev = gpu.dev
+    before = dev.packed_state(0).clone()
+    dev.kernel("pss_sbe_rhs", 3)
+    torch.cuda.synchronize()
+    ne, nh, VC, E1D, (Heh, Hee, Hhh), GE, GH = oracle_h(ref, st, lines, rr)
+    OffP, _, _ = ref.OffDiagDephasing2(ne, nh, st["p2"], VC)
+    want = {"p": ref.dpdt(st["C2"], st["D2"], st["p2"], Heh, Hee, Hhh, GE, GH, OffP),
+            "C": ref.dCdt(st["C2"], st["D2"], st["p2"], Heh, Hee, Hhh),
+            "D": ref.dDdt(st["C2"], st["D2"], st["p2"], Heh, Hee, Hhh)}
+    for k, w in want.items():
+        got = dev.buffer(0, k, "next").cpu().numpy()
+        assert rel(got, w) < tol, (k, rel(got, w))
+    assert torch.equal(dev.packed_state(0), before)
+
+
+@pytest.mark.parametrize("lorentz", [False, True])
+def test_coll_rates_entry(lorentz):
+    """MBCE/MBCH (which = 1) and MBPE/MBPH (which = 2) of the populations on the next buffers'
+    diagonals, with the screened V rows and E1D that pss_calc_h leaves in the workspace."""
+    import torch
+
+    nk = 36
+    ref, gpu, st, lines, rr = make(nk, "fp64", dict(LorentzDelta=lorentz))
+    dev = gpu.dev
+    dev.kernel("pss_calc_h", 3)
+    rng = np.random.default_rng(11)
+    ne3, nh3 = rng.uniform(0.0, 0.6, nk), rng.uniform(0.0, 0.4, nk)
+    dev.buffer(0, "C", "next").diagonal().copy_(torch.as_tensor(ne3.astype(complex)))
+    dev.buffer(0, "D", "next").diagonal().copy_(torch.as_tensor(nh3.astype(complex)))
+    _, _, VC, E1D, _, _, _ = oracle_h(ref, st, lines, rr)
+    for which in (1, 2):
+        dev.kernel("pss_coll_rates", which)
+        torch.cuda.synchronize()
+        W = dev.workspace_view(0, "W").cpu().numpy().reshape(4, nk)
+        We, Wh = [np.zeros(nk), np.zeros(nk)], [np.zeros(nk), np.zeros(nk)]
+        if which == 1:
+            ref.MBCE(ne3, nh3, VC, *We)
+            ref.MBCH(ne3, nh3, VC, *Wh)
+        else:
+            ref.MBPE(ne3, VC, E1D, *We)
+            ref.MBPH(nh3, VC, E1D, *Wh)
+        for name, a, b in zip(("WinE", "WoutE", "WinH", "WoutH"), W, We + Wh):
+            assert rel(a, b) < 1e-12, (which, name, rel(a, b))
+            assert np.abs(b).max() > 0
+
+
+def test_qwpol_entry():
+    """P(r) and the k sums of rho(r) of the next buffers' p, C, D: QWPolarization3
+    (PSTD3D/qwoptics.py:815-946) before its centred FFT, and the first line of QWRho5
+    (:948-1077, sum_k1k2 C[k1,k2] e^{-i k1 r} e^{i k2 r} W(r)/(2L))."""
+    import torch
+
+    nk = 30
+    ref, gpu, st, lines, rr = make(nk, "fp64")
+    dev = gpu.dev
+    rng = np.random.default_rng(5)
+    p3 = 0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk)))
+    C3, D3 = herm(nk, rng), herm(nk, rng)
+    for k, a in (("p", p3), ("C", C3), ("D", D3)):
+        dev.buffer(0, k, "next").copy_(torch.as_tensor(a))
+    dev.kernel("pss_qwpol")
+    torch.cuda.synchronize()
+    Nr = ref.Nr
+    pr = dev.workspace_view(0, "pr").cpu().numpy().reshape(3, Nr)
+    for c, P in enumerate(ref.QWPolarization3(p3, 1)):
+        P = P.copy()
+        iFFTG(P)
+        assert rel(pr[c], P.real) < 1e-12, (c, rel(pr[c], P.real))
+    rho = dev.workspace_view(0, "rho").cpu().numpy().view(complex).reshape(2, Nr)
+    E, Ec = ref.Expikr, ref.Expikrc
+    re = np.sum(Ec * (C3 @ E), axis=0) * ref.QWWindow / (2.0 * ref.L)
+    rh = np.sum(Ec * (np.conj(D3) @ E), axis=0) * ref.QWWindow / (2.0 * ref.L)
+    assert rel(rho[0], re) < 1e-12 and rel(rho[1], rh) < 1e-12
+    # populations are left as they were (no relaxation in this entry)
+    assert np.array_equal(np.diag(dev.buffer(0, "C", "next").cpu().numpy()), np.diag(C3))
+
+
+def test_state_get_set_round_trip_resumes_the_run():
+    """pss_sbe_get_state after some steps, pss_sbe_set_state into a fresh solver (its step
+    counter at another parity): stepping both on gives the same states bit for bit."""
+    import torch
+
+    nk = 20
+    ref, a, st, lines, rr = make(nk, "fp64")
+    outs = [np.zeros(len(rr), complex) for _ in range(4)]
+    E = lines[0]
+    for _ in range(3):
+        a.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, 1, *outs, [False], [False])
+    packed = a.dev.packed_state(0).clone()
+    host = a.get_state(1)
+    for j, k in enumerate(("p1", "p2", "C1", "C2", "D1", "D2")):
+        assert np.array_equal(packed[j].cpu().numpy(), host[k])
+    _, b, _, _, _ = make(nk, "fp64", seed=9)
+    for _ in range(2):  # a different step parity
+        b.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, 1, *outs, [False], [False])
+    b.dev.load_packed_state(0, packed)
+    b.t, b.xxx, b.jjj = a.t, a.xxx, a.jjj
+    for _ in range(2):
+        for s in (a, b):
+            s.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, 1, *outs, [False], [False])
+    torch.cuda.synchronize()
+    sa, sb = a.get_state(1), b.get_state(1)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k

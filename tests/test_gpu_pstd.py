@@ -684,3 +684,53 @@ def test_jline_fft_in_kernel(Nx, Nw, keep_old):
     want = np.sum(P - po, axis=0) / dt / Nw
     want[0] += CalcJfx(np.sum(Rho, axis=0), np.sum(ro, axis=0), dt, dx) / Nw
     assert rel(j0, want) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,axis,in32,curl", [((16, 8, 128), 2, 1, True), ((16, 8, 128), 2, 0, True),
+                                                  ((64, 4, 2), 0, 1, False), ((4, 256, 2), 1, 0, False),
+                                                  ((8, 2, 32), 2, 0, False)])
+def test_fft_pass_acc_matches_numpy(shape, axis, in32, curl):
+    """pss_fft_pass_acc: out_b += FFT_axis(in_b) (complex64 input computed in FP32: 1e-6 of the
+    transform; complex128: 1e-14), plus coef i (k x G)_b when G is given (the curl term of the
+    E update, PSTD3D/PSTD3D.py:29-76, on the accumulating last pass); in is left as is."""
+    import ctypes as C
+
+    import torch
+
+    from pulsesuite_amd import _lib
+    from pulsesuite_amd._grid import GridContext
+    from pulsesuite_amd.typespace import GetKxArray, GetKyArray, GetKzArray, ss
+
+    sp = ss(3, *shape, 20e-9, 20e-9, 20e-9, 9.1)
+    ctx = GridContext(sp, 1e-17, with_cpml=False)
+    rng = np.random.default_rng(sum(shape) + axis)
+    c = lambda: rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    A = [c() for _ in range(3)]
+    F = [c() * 1e3 for _ in range(3)]
+    G = [c() * 1e-5 for _ in range(3)]
+    dt = torch.complex64 if in32 else torch.complex128
+    tin = [torch.as_tensor(a, dtype=dt, device="cuda") for a in A]
+    tout = [cu(f) for f in F]
+    tG = [cu(g) for g in G] if curl else [None] * 3
+    coef = 1.3e-2
+    ins = (C.c_void_p * 3)(*(t.data_ptr() for t in tin))
+    outs = (C.c_void_p * 3)(*(t.data_ptr() for t in tout))
+    _lib.check(_lib.lib.pss_fft_pass_acc(ctx.gp, ins, outs, 3, axis, -1, in32, *(_lib.ptr(t) for t in tG), coef,
+                                         _lib.stream_ptr()), "pss_fft_pass_acc")
+    torch.cuda.synchronize()
+    Ain = [a.astype(np.complex64).astype(np.complex128) if in32 else a for a in A]
+    want = [f + np.fft.fft(a, axis=axis) for f, a in zip(F, Ain)]
+    if curl:
+        kx, ky, kz = GetKxArray(sp)[:, None, None], GetKyArray(sp)[None, :, None], GetKzArray(sp)[None, None, :]
+        cx = ky * G[2] - kz * G[1]
+        cy = kz * G[0] - kx * G[2]
+        cz = kx * G[1] - ky * G[0]
+        want = [w + coef * 1j * cc for w, cc in zip(want, (cx, cy, cz))]
+    tol = 1e-6 if in32 else 1e-14
+    for t, w, f, a in zip(tout, want, F, Ain):
+        got = t.cpu().numpy()
+        scale = np.linalg.norm(np.fft.fft(a, axis=axis))
+        assert np.linalg.norm(got - w) / scale < tol
+    for t, a in zip(tin, A):  # input untouched
+        assert np.array_equal(t.cpu().numpy(), a.astype(np.complex64) if in32 else a)
