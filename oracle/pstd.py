@@ -8,6 +8,8 @@ Pinning:
   UpdateE3D / UpdateB3D            pinned (single-mode known answers of the
                                    reference's own tests, k.B = 0)
   CPML profiles / coefficients     pinned (closed forms: b=1,c=0 outside the PML)
+  absorber mask                    pinned (closed forms: 1 - U0 at the outer left cell,
+                                   exp(-gmax rho^3 dt), 1 inside, separability)
   ApplyCPML_E/B, UpdateTFSC,
   SeedInitialCondition, run loop   golden (fixtures from the reference modules)
   fft_3D / ifft_3D                 pinned (explicit DFT at small N)
@@ -277,6 +279,50 @@ class CPML:
             fft_3D(A)
 
 
+# --- masking absorber  PSTD3D/absorber.py:39-191 ------------------------------
+def CalcNPML_mask(N):
+    """PSTD3D/absorber.py:39-59"""
+    if N <= 4:
+        return 0
+    npml = max(6, int(0.05 * N))
+    if npml > N // 10:
+        npml = N // 10
+    if 2 * npml >= N:
+        npml = max(1, N // 4)
+    return npml
+
+
+def build_1d_mask(N, npml, dt, U0=0.5, m=3):
+    """PSTD3D/absorber.py:62-122 (the right layer's grading ends at (npml-1)/npml)."""
+    mask = np.ones(N)
+    if npml <= 0:
+        return mask
+    gmax = -np.log(1.0 - U0) / dt
+    i = np.arange(N)
+    left = i < npml
+    right = i >= N - npml
+    rho_l = (npml - 1 - i[left]) / max(npml - 1, 1)
+    rho_r = (i[right] - (N - npml)) / max(npml, 1)
+    mask[left] = np.exp(-gmax * rho_l**m * dt)
+    mask[right] = np.exp(-gmax * rho_r**m * dt)
+    return mask
+
+
+class Absorber:
+    """InitAbsorber / ApplyAbsorber_E/B (PSTD3D/absorber.py:125-191)."""
+
+    def __init__(self, Nx, Ny, Nz, dt):
+        self.npml = (CalcNPML_mask(Nx), CalcNPML_mask(Ny), CalcNPML_mask(Nz))
+        mx, my, mz = (build_1d_mask(N, p, dt) for N, p in zip((Nx, Ny, Nz), self.npml))
+        self.mask = mx[:, None, None] * my[None, :, None] * mz[None, None, :]
+
+    def apply(self, *F):
+        for A in F:
+            ifft_3D(A)
+            A *= self.mask
+            fft_3D(A)
+
+
 # --- soft source  PSTD3D/tfsf.py:26-142 --------------------------------------
 def InitializeTFSF(space, pulse):
     Nx = space.Nx
@@ -347,8 +393,11 @@ class Propagator:
     (fields in k-space; J persistent and owned by the callback).
     """
 
-    def __init__(self, space, time, pulse, source_type="ic", boundary_type="cpml", calc_j=None, npml=None):
+    def __init__(self, space, time, pulse, source_type="ic", boundary_type="cpml", calc_j=None, npml=None,
+                 polarization="y"):
         self.space, self.time, self.pulse = space, time, pulse
+        # the source field: Ey with Bz = Ey/v (the reference), or Ez with By = -Ez/v
+        self.polarization = polarization
         self.source_type, self.boundary_type = source_type, boundary_type
         self.calc_j = calc_j
         shape = (space.Nx, space.Ny, space.Nz)
@@ -358,13 +407,21 @@ class Propagator:
         self.v2 = c0**2 / GetEpsr(space)
         self.kx, self.ky, self.kz = GetKxArray(space), GetKyArray(space), GetKzArray(space)
         self.cpml = None
+        self.absorber = None
         if boundary_type == "cpml":
             self.cpml = CPML(space.Nx, space.Ny, space.Nz, GetDx(space), GetDy(space), GetDz(space),
                              time.dt, GetEpsr(space), npml=npml)
+        elif boundary_type == "mask":
+            self.absorber = Absorber(space.Nx, space.Ny, space.Nz, time.dt)
         self.tfsf = InitializeTFSF(space, pulse) if source_type == "soft" else None
         if source_type == "ic":
-            npx = self.cpml.npml_x if self.cpml is not None else 0
-            SeedInitialCondition(space, time, pulse, self.Ey, self.Bz, npx)
+            npx = (self.cpml.npml_x if self.cpml is not None else
+                   self.absorber.npml[0] if self.absorber is not None else 0)
+            if polarization == "y":
+                SeedInitialCondition(space, time, pulse, self.Ey, self.Bz, npx)
+            else:
+                SeedInitialCondition(space, time, pulse, self.Ez, self.By, npx)
+                self.By *= -1.0
 
     def emax_real(self):
         E = [self.Ex.copy(), self.Ey.copy(), self.Ez.copy()]
@@ -380,8 +437,10 @@ class Propagator:
         if self.cpml is not None:
             self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz,
                                   self.kx, self.ky, self.kz, self.v2, dt)
+        elif self.absorber is not None:
+            self.absorber.apply(self.Ex, self.Ey, self.Ez)
         if self.source_type == "soft":
-            UpdateTFSC(self.Ey, self.tfsf, sp, tm, pu, Amp)
+            UpdateTFSC(self.Ey if self.polarization == "y" else self.Ez, self.tfsf, sp, tm, pu, Amp)
         tm.t += dt / 2.0
         if self.calc_j is not None:
             self.calc_j(sp, tm, self.Ex, self.Ey, self.Ez, self.Jx, self.Jy, self.Jz)
@@ -389,8 +448,13 @@ class Propagator:
         if self.cpml is not None:
             self.cpml.ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez,
                                   self.kx, self.ky, self.kz, dt)
+        elif self.absorber is not None:
+            self.absorber.apply(self.Bx, self.By, self.Bz)
         if self.source_type == "soft":
-            UpdateTFSC(self.Bz, self.tfsf, sp, tm, pu, Amp / self.v)
+            if self.polarization == "y":
+                UpdateTFSC(self.Bz, self.tfsf, sp, tm, pu, Amp / self.v)
+            else:
+                UpdateTFSC(self.By, self.tfsf, sp, tm, pu, -Amp / self.v)
         tm.t += dt / 2.0
         tm.n += 1
 

@@ -7,11 +7,14 @@ runs the leapfrog loop: E curl update, CPML-E (with the soft source and the
 wire point-sample fused into its real-space pass), t += dt/2, ``calc_j``,
 B curl update, CPML-B (+ source), t += dt/2.
 
-Differences from the reference, on purpose:
-  * no per-step diagnostics IFFTs, Emax.dat, snapshots or summary files; the
-    NaN/overflow guard checks the k-space fields every ``check_every`` steps and
-    raises RuntimeError as the reference does (PSTD3D.py:510-512);
-  * boundary_type "mask" (absorber.py) is not built; "cpml" and "none" are.
+Boundaries: "mask" (the reference's default, PSTD3D/absorber.py), "cpml" and "none"
+(periodic).  ``polarization`` (keyword extra, default "y" as the reference) picks the soft
+source's and the IC seed's field: "y" injects Ey and Bz = Ey/v, "z" injects Ez and
+By = -Ez/v (a plane wave along +x; E along x would be longitudinal there).
+
+Differences from the reference, on purpose: no per-step diagnostics IFFTs, Emax.dat,
+snapshots or summary files; the NaN/overflow guard checks the k-space fields every
+``check_every`` steps and raises RuntimeError as the reference does (PSTD3D.py:510-512).
 """
 
 import numpy as np
@@ -50,10 +53,11 @@ def UpdateB3D(space, time, Ex, Ey, Ez, Bx, By, Bz):
                "pss_update_b3d")
 
 
-def SeedInitialCondition(space, time, pulse, Ey, Bz, npml_x=0):
+def SeedInitialCondition(space, time, pulse, Ey, Bz, npml_x=0, b_sign=1.0):
     """Seed Ey, Bz with the pulse and FFT to k-space (PSTD3D/PSTD3D.py:134-226).
 
-    The seed is evaluated on the host once at set-up; the transform runs on the GPU."""
+    The seed is evaluated on the host once at set-up; the transform runs on the GPU.
+    b_sign = -1 seeds (Ez, By) of a z-polarised pulse through the same arguments."""
     from .fftw import fft_3D
 
     Nx = space.Nx
@@ -74,7 +78,7 @@ def SeedInitialCondition(space, time, pulse, Ey, Bz, npml_x=0):
             taper[i] = 0.5 * (1.0 - np.cos(np.pi * (Nx - 1 - i) / npml_x))
         E = E * taper[:, None, None]
     Ey.copy_(torch.as_tensor(E, dtype=torch.complex128))
-    Bz.copy_(torch.as_tensor(E / v, dtype=torch.complex128))
+    Bz.copy_(torch.as_tensor(b_sign * E / v, dtype=torch.complex128))
     fft_3D(Ey)
     fft_3D(Bz)
 
@@ -92,13 +96,16 @@ class PSTD3DPropagator:
     """PSTD3D/PSTD3D.py:233-629 on the GPU.  Keyword-only extras: ``npml`` (CPML
     thickness override), ``check_every`` (NaN guard period), ``device``."""
 
-    def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="cpml", calc_j=None,
+    def __init__(self, space, time, pulse, *, source_type="ic", boundary_type="mask", calc_j=None,
                  output_dir="output", snapshot_interval=500, npml=None, check_every=50, device=None,
-                 precision="fp64", device_time=False, overlap=None):
-        if boundary_type == "mask":
-            raise NotImplementedError("boundary_type='mask' (absorber) is not built on the device; use 'cpml'")
-        if boundary_type not in ("cpml", "none"):
+                 precision="fp64", device_time=False, overlap=None, polarization="y"):
+        if boundary_type not in ("mask", "cpml", "none"):
             raise ValueError(f"unknown boundary_type {boundary_type!r}")
+        if polarization not in ("y", "z"):
+            raise ValueError("polarization must be 'y' or 'z' (the source plane wave travels along +x)")
+        self.polarization = polarization
+        # soft-source / seed components: E and B indices and the sign of B = (x^ x E)/v
+        self._e_comp, self._b_comp, self._b_sign = (1, 2, 1.0) if polarization == "y" else (2, 1, -1.0)
         if source_type not in ("ic", "soft"):
             raise ValueError(f"unknown source_type {source_type!r}")
         self.space, self.time, self.pulse = space, time, pulse
@@ -110,9 +117,15 @@ class PSTD3DPropagator:
 
         self.cpml = CPML(space.Nx, space.Ny, space.Nz, space.dx, space.dy, space.dz, time.dt, GetEpsr(space),
                          npml=npml if boundary_type == "cpml" else 0, device=self.device)
-        if boundary_type == "none":
+        if boundary_type != "cpml":
             self.cpml.ctx.g.npml_x = self.cpml.ctx.g.npml_y = self.cpml.ctx.g.npml_z = 0
         self.ctx = self.cpml.ctx
+        self.absorber = None
+        if boundary_type == "mask":
+            from .absorber import Absorber
+
+            self.absorber = Absorber(space.Nx, space.Ny, space.Nz, time.dt, ctx=self.ctx)
+            self._partial = torch.zeros(6144, dtype=torch.float64, device=self.device)
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"unknown precision {precision!r}")
         # mixed: the CPML's spectral derivatives (real parts, PML cells only) in FP32;
@@ -139,7 +152,9 @@ class PSTD3DPropagator:
         if source_type == "soft":
             self._tfsf = torch.as_tensor(InitializeTFSF(space, pulse), device=dev)
         if source_type == "ic":
-            SeedInitialCondition(space, time, pulse, self.Ey, self.Bz, self.cpml.npml_x)
+            npx = self.absorber.npml_x if self.absorber is not None else self.cpml.npml_x
+            Es, Bs = (self.Ey, self.Bz) if polarization == "y" else (self.Ez, self.By)
+            SeedInitialCondition(space, time, pulse, Es, Bs, npx, b_sign=self._b_sign)
         if calc_j is not None and hasattr(calc_j, "bind"):
             calc_j.bind(self)
         self._nstep = 0
@@ -257,11 +272,21 @@ class PSTD3DPropagator:
                                            _lib.stream_ptr(stream)), "pss_update_e3d")
 
     def _cpml_e(self, stream, sample=None):
-        """CPML correction of E from B and psi_E, with the soft source on Ey at the current
-        (host or device) time; optionally the wire's point sample of the corrected E."""
+        """CPML correction of E from B and psi_E (or the masking absorber), with the soft
+        source on its E component at the current (host or device) time; optionally the
+        wire's point sample of the corrected E."""
         tm = self.time
+        src = self._src(self._e_comp, self.pulse.Amp, tm.t)
+        if self.absorber is not None:
+            self.absorber.apply(self.Ex, self.Ey, self.Ez, src=src, stream=stream)
+            if sample is not None:
+                sidx, sw, sout = sample
+                P = _lib.ptr
+                _lib.check(_lib.lib.pss_ksample(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(sidx), P(sw),
+                                                P(self._partial), P(sout), _lib.stream_ptr(stream)), "pss_ksample")
+            return
         self.cpml.ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, dt=tm.dt, v2=self.v2,
-                              src=self._src(1, self.pulse.Amp, tm.t), sample=sample, stream=stream)
+                              src=src, sample=sample, stream=stream)
 
     def _cpml_e_next(self, stream):
         """The next step's CPML correction of E in a pipelined run, soft source on Ey at the
@@ -273,7 +298,7 @@ class PSTD3DPropagator:
         tm = self.time
         if self._curl_early:
             self.cpml.CurlE_ApplyCPML_E(self.Ex, self.Ey, self.Ez, self.Bx, self.By, self.Bz, tm.dt, self.v2,
-                                        src=self._src(1, self.pulse.Amp, tm.t), stream=stream)
+                                        src=self._src(self._e_comp, self.pulse.Amp, tm.t), stream=stream)
         else:
             self._cpml_e(stream)
 
@@ -329,8 +354,10 @@ class PSTD3DPropagator:
         cur.wait_event(self._ev_j)
 
     def pipelined(self):
-        """True when run_pipelined applies: device time and an overlapped, separable calc_j."""
-        return self._tdev is not None and self.overlap and getattr(self.calc_j, "separable", False)
+        """True when run_pipelined applies: the CPML boundary, device time and an overlapped,
+        separable calc_j."""
+        return (self.boundary_type == "cpml" and self._tdev is not None and self.overlap
+                and getattr(self.calc_j, "separable", False))
 
     def capture_pipelined(self, steps_per_graph=None):
         """Record the pipelined step (with and without the next step's CPML-E correction)
@@ -401,8 +428,15 @@ class PSTD3DPropagator:
         of the B update was applied already)."""
         tm = self.time
         st = _lib.stream_ptr(stream)
-        src = self._src(2, self.pulse.Amp / self.v, tm.t)
-        if curl:
+        src = self._src(self._b_comp, self._b_sign * self.pulse.Amp / self.v, tm.t)
+        if self.absorber is not None:
+            if curl:
+                self.ctx.g.dt = tm.dt
+                P = _lib.ptr
+                _lib.check(_lib.lib.pss_update_b3d(self.ctx.gp, P(self.Ex), P(self.Ey), P(self.Ez), P(self.Bx),
+                                                   P(self.By), P(self.Bz), st), "pss_update_b3d")
+            self.absorber.apply(self.Bx, self.By, self.Bz, src=src, stream=stream)
+        elif curl:
             self.cpml.UpdateB_ApplyCPML_B(self.Bx, self.By, self.Bz, self.Ex, self.Ey, self.Ez, tm.dt, src=src,
                                           stream=stream)
         else:
@@ -430,4 +464,22 @@ class PSTD3DPropagator:
             self.step()
 
 
-PSTD_3D_Propagator = PSTD3DPropagator
+def InitializeFields(Nx, Ny, Nz, device=None):
+    """Nine zero (Nx, Ny, Nz) complex128 device tensors: Ex, Ey, Ez, Bx, By, Bz, Jx, Jy, Jz
+    (PSTD3D/PSTD3D.py:114-131)."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    return tuple(torch.zeros((Nx, Ny, Nz), dtype=torch.complex128, device=dev) for _ in range(9))
+
+
+_default = None  # the last propagator run by PSTD_3D_Propagator (not thread-safe, as the reference)
+
+
+def PSTD_3D_Propagator(space, time, pulse, *, source_type="ic", boundary_type="mask", calc_j=None,
+                       output_dir="output", snapshot_interval=500, **kw):
+    """Run a whole simulation (PSTD3D/PSTD3D.py:864-909): build the propagator, keep it as
+    the module's default and run it to the end of the time grid."""
+    global _default
+    _default = PSTD3DPropagator(space, time, pulse, source_type=source_type, boundary_type=boundary_type,
+                                calc_j=calc_j, output_dir=output_dir, snapshot_interval=snapshot_interval, **kw)
+    _default.run()
+    return _default

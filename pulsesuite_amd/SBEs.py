@@ -23,7 +23,7 @@ from . import _lib
 from .wire_model import MBFlags, QWParams, build_model, read_mb_params, read_qw_params
 
 __all__ = ["SBESolver", "InitializeSBE", "QWCalculator", "chiqw", "getqc", "QWArea", "ShutOffOptics",
-           "DeviceWires", "QWParams", "MBFlags"]
+           "WriteSBEsData", "ReadSBEsData", "dpdt", "dCdt", "dDdt", "DeviceWires", "QWParams", "MBFlags"]
 
 _CDT = torch.complex128
 
@@ -189,6 +189,18 @@ class DeviceWires:
         raw = bytes(memoryview(self.descs).cast("B"))
         host = torch.frombuffer(bytearray(raw), dtype=torch.uint8)
         self.d_descs.copy_(host)
+
+    def set_desc(self, i, **fields):
+        """Change fields of wire i's descriptor (flags, gam_e, gam_h, gam_eh, ...) and upload
+        the descriptors when anything changed."""
+        d = self.descs[i]
+        changed = False
+        for k, v in fields.items():
+            if getattr(d, k) != v:
+                setattr(d, k, v)
+                changed = True
+        if changed:
+            self.upload_descs()
 
     def set_dt(self, dt):
         dt = float(dt)
@@ -406,11 +418,95 @@ class SBESolver:
             self.dev.descs[i].flags &= ~_lib.OPTICS
         self.dev.upload_descs()
 
+    def write_sbes_data(self, n, backup_dir="dataQW/backup/"):
+        """Checkpoint in the reference's text format (PSTD3D/SBEs.py:5536-5558).  The
+        reference's YY2/CC2/DD2 (the averaged previous step) and YY3/CC3/DD3 (the newest
+        step) are the device's prev and cur buffers; its YY1/CC1/DD1 (one step older, never
+        read back by the step) are not kept on the device and are written as YY2/CC2/DD2."""
+        Nk, Nw = self.Nk, self.Nw
+        arrs = {k: np.zeros((Nk, Nk, Nw), complex) for k in BACKUP_KEYS}
+        for w in range(Nw):
+            st = self.dev.get_state(w)
+            for X, p1, p2 in (("YY", "p1", "p2"), ("CC", "C1", "C2"), ("DD", "D1", "D2")):
+                arrs[X + "1"][:, :, w] = st[p1]
+                arrs[X + "2"][:, :, w] = st[p1]
+                arrs[X + "3"][:, :, w] = st[p2]
+        write_backup(arrs, n, backup_dir)
+
+    def read_sbes_data(self, Nt, backup_dir="dataQW/backup/"):
+        """Resume from a checkpoint (PSTD3D/SBEs.py:5560-5581): prev <- YY2/CC2/DD2,
+        cur <- YY3/CC3/DD3, as the next step's _checkout reads them (:5073-5080)."""
+        a = read_backup(self.Nk, self.Nw, Nt, backup_dir)
+        for w in range(self.Nw):
+            self.dev.set_state(w, p1=a["YY2"][:, :, w], p2=a["YY3"][:, :, w], C1=a["CC2"][:, :, w],
+                               C2=a["CC3"][:, :, w], D1=a["DD2"][:, :, w], D2=a["DD3"][:, :, w])
+
     def get_state(self, w):
         return self.dev.get_state(w - 1)
 
     def set_state(self, w, **kw):
         self.dev.set_state(w - 1, **kw)
+
+
+def _rhs(which, C, D, p, Heh, Hee, Hhh, GamE=None, GamH=None, Off=None):
+    """One of dpdt / dCdt / dDdt on the device (pss_rhs_dense); NumPy in, NumPy out (copies
+    to and from the GPU), CUDA tensors in, a CUDA tensor out."""
+    host = not isinstance(p, torch.Tensor)
+    dev = p.device if not host else torch.device("cuda", torch.cuda.current_device())
+
+    def t(a, dt=_CDT):
+        return None if a is None else torch.as_tensor(np.asarray(a) if not isinstance(a, torch.Tensor) else a,
+                                                      dtype=dt, device=dev).contiguous()
+
+    args = [t(C), t(D), t(p), t(Heh), t(Hee), t(Hhh), t(GamE, torch.float64), t(GamH, torch.float64), t(Off)]
+    Nk = args[2].shape[0]
+    out = torch.empty((Nk, Nk), dtype=_CDT, device=dev)
+    _lib.check(_lib.lib.pss_rhs_dense(Nk, which, *(_lib.ptr(a) for a in args), _lib.ptr(out), _lib.stream_ptr()),
+               "pss_rhs_dense")
+    return out.cpu().numpy() if host else out
+
+
+def dpdt(C, D, p, Heh, Hee, Hhh, GamE, GamH, OffP):
+    """dp/dt of the SBEs (PSTD3D/SBEs.py:1850-1976), output indexed [kh, ke]."""
+    return _rhs(0, C, D, p, Heh, Hee, Hhh, GamE, GamH, OffP)
+
+
+def dCdt(Cee, Dhh, Phe, Heh, Hee, Hhh, GamE=None, GamH=None, OffE=None):
+    """dC/dt (PSTD3D/SBEs.py:2221-2338); GamE, GamH, OffE are not used, as in the reference."""
+    return _rhs(1, Cee, Dhh, Phe, Heh, Hee, Hhh)
+
+
+def dDdt(Cee, Dhh, Phe, Heh, Hee, Hhh, GamE=None, GamH=None, OffH=None):
+    """dD/dt (PSTD3D/SBEs.py:2513-2636); GamE, GamH, OffH are not used, as in the reference."""
+    return _rhs(2, Cee, Dhh, Phe, Heh, Hee, Hhh)
+
+
+BACKUP_KEYS = ("CC1", "CC2", "CC3", "DD1", "DD2", "DD3", "YY1", "YY2", "YY3")
+
+
+def write_backup(arrays, n, backup_dir="dataQW/backup/"):
+    """The reference's checkpoint files (PSTD3D/SBEs.py:5536-5558): for every key of
+    BACKUP_KEYS one text file ``{key}.{n}.dat`` holding arrays[key] (complex [Nk][Nk][Nw]) as
+    "re im" lines, wire by wire, k2 outer, k1 inner (shortest round-trip float text)."""
+    os.makedirs(backup_dir, exist_ok=True)
+    for key in BACKUP_KEYS:
+        a = np.asarray(arrays[key])
+        with open(os.path.join(backup_dir, f"{key}.{n}.dat"), "w", encoding="utf-8") as f:
+            for w in range(a.shape[2]):
+                col = a[:, :, w].T.reshape(-1)  # k1 fastest
+                f.write("".join(f"{x!r} {y!r}\n" for x, y in zip(col.real.tolist(), col.imag.tolist())))
+
+
+def read_backup(Nk, Nw, Nt, backup_dir="dataQW/backup/"):
+    """Inverse of write_backup (PSTD3D/SBEs.py:5560-5581): {key: complex [Nk][Nk][Nw]}."""
+    out = {}
+    for key in BACKUP_KEYS:
+        with open(os.path.join(backup_dir, f"{key}.{Nt}.dat"), "r", encoding="utf-8") as f:
+            vals = np.array([complex(float(a), float(b)) for a, b in (line.split()[:2] for line in f)])
+        if vals.size != Nk * Nk * Nw:
+            raise ValueError(f"{key}.{Nt}.dat holds {vals.size} values, expected {Nk * Nk * Nw}")
+        out[key] = vals.reshape(Nw, Nk, Nk).transpose(2, 1, 0).copy()
+    return out
 
 
 def _copy_out(dst, src):
@@ -448,3 +544,13 @@ def QWArea():
 
 def ShutOffOptics():
     _default_solver.ShutOffOptics()
+
+
+def WriteSBEsData(n):
+    """Module shim (PSTD3D/SBEs.py:6071-6073)."""
+    _default_solver.write_sbes_data(n)
+
+
+def ReadSBEsData(Nt):
+    """Module shim (PSTD3D/SBEs.py:6076-6078)."""
+    _default_solver.read_sbes_data(Nt)

@@ -1025,6 +1025,38 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
   dFz[o] = TF{(decltype(TF::x))fz, 0};
 }
 
+// Masking absorber (PSTD3D/absorber.py:102-191) on real-space fields, with the soft source
+// of the same half step (PSTD3D/tfsf.py:106-136) on src_comp: F *= (mx[i] my[j]) mz[k] (the
+// complex field, as the reference multiplies it), then Re F_src += S(x) E_inc dx.
+__global__ void __launch_bounds__(256) k_mask_src(pss_grid g, cplx* __restrict__ Fx, cplx* __restrict__ Fy,
+                                                  cplx* __restrict__ Fz, const double* __restrict__ mx,
+                                                  const double* __restrict__ my, const double* __restrict__ mz,
+                                                  int src_comp, SrcPar sp, const double* __restrict__ srcx,
+                                                  const double* __restrict__ rx, const double* __restrict__ ry,
+                                                  const double* __restrict__ rz) {
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  const int64_t n = (int64_t)Nx * Ny * Nz;
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= n) return;
+  const int lz = __builtin_ctz(Nz), ly = __builtin_ctz(Ny);
+  const int k = (int)(o & (Nz - 1)), j = (int)((o >> lz) & (Ny - 1)), i = (int)(o >> (lz + ly));
+  const double m = (mx[i] * my[j]) * mz[k];
+  cplx fx = Fx[o] * m, fy = Fy[o] * m, fz = Fz[o] * m;
+  if (src_comp >= 0) {
+    const double t = g.tdev ? *g.tdev + sp.t : sp.t;
+    const double tau = t - rx[i] / sp.v - sp.Tp;
+    const double gyz = exp(-(ry[j] * ry[j] + rz[k] * rz[k]) * sp.w0inv2);
+    const double einc = sp.amp * gyz * exp(-(tau * tau) / (sp.tau_G * sp.tau_G)) * cos(sp.omega0 * tau + sp.chirp * tau * tau);
+    const double add = srcx[i] * einc * sp.dx;
+    if (src_comp == 0) fx.x += add;
+    else if (src_comp == 1) fy.x += add;
+    else fz.x += add;
+  }
+  Fx[o] = fx;
+  Fy[o] = fy;
+  Fz[o] = fz;
+}
+
 // Trilinear sample (EAtXYZ, PSTD3D/usefulsubs.py:2016-2085) of the corrected
 // real-space field F + dF at 8 corner cells, with F given in k-space:
 //   F(x_c) = (1/N) sum_k F_k exp(+2 pi i (mx ix/Nx + my iy/Ny + mz iz/Nz)),
@@ -1572,6 +1604,34 @@ extern "C" int pss_cpml_half(const pss_grid* g, int32_t which, double* Fx, doubl
                              double* sample_out, void* stream) {
   return cpml_half_impl(g, which, Fx, Fy, Fz, Gx, Gy, Gz, psi, scratch, src_comp, src_par, srcx, rx, ry, rz,
                         sample_idx, sample_w, sample_out, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int pss_mask_half(const pss_grid* g, double* Fx, double* Fy, double* Fz, const double* mx, const double* my,
+                             const double* mz, int32_t src_comp, const double* src_par, const double* srcx,
+                             const double* rx, const double* ry, const double* rz, void* stream) {
+  if (!pow2_axes(g) || g->Nx > 2048 || g->Ny > 2048 || g->Nz > 2048)
+    return pss_set_msg(2, "pss_mask_half: axis lengths must be powers of two <= 2048");
+  if (!Fx || !Fy || !Fz || !mx || !my || !mz) return pss_set_msg(2, "pss_mask_half: null field or mask");
+  if (src_comp > 2 || (src_comp >= 0 && (!src_par || !srcx || !rx || !ry || !rz)))
+    return pss_set_msg(2, "pss_mask_half: src_comp 0..2 needs its parameters and coordinates");
+  hipStream_t s = (hipStream_t)stream;
+  cplx* F[3] = {(cplx*)Fx, (cplx*)Fy, (cplx*)Fz};
+  const cplx* Fc[3] = {F[0], F[1], F[2]};
+  // to real space (inverse passes, 1/N overall), mask and source, back to k-space
+  for (int axis = 0; axis < 3; ++axis)
+    if (launch_pass_b(g, Fc, F, 3, axis, +1, nullptr, -1, s)) return pss_set_msg(2, "pss_mask_half: fft");
+  SrcPar sp{};
+  if (src_comp >= 0) {
+    sp.t = src_par[0]; sp.amp = src_par[1]; sp.omega0 = src_par[2]; sp.tau_G = src_par[3];
+    sp.chirp = src_par[4]; sp.v = src_par[5]; sp.Tp = src_par[6]; sp.w0inv2 = src_par[7]; sp.dx = src_par[8];
+  }
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  hipLaunchKernelGGL(k_mask_src, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *g, F[0], F[1], F[2], mx, my, mz,
+                     src_comp, sp, srcx, rx, ry, rz);
+  for (int axis = 0; axis < 3; ++axis)
+    if (launch_pass_b(g, Fc, F, 3, axis, -1, nullptr, -1, s)) return pss_set_msg(2, "pss_mask_half: fft");
+  PSS_CHECK(hipGetLastError());
+  return 0;
 }
 
 extern "C" int pss_b_half(const pss_grid* g, double* Bx, double* By, double* Bz, const double* Ex, const double* Ey,

@@ -2287,6 +2287,14 @@ extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_
   return pss_sbe_step_p(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, stream);
 }
 
+extern "C" int pss_sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
+                              int32_t max_Nr, int32_t max_nrr, int32_t precisions, int32_t stages, int32_t opt,
+                              void* stream) {
+  if (stages < 1 || stages > ST_ALL) return pss_set_msg(2, "pss_sbe_stages: stages is a mask of 1 ... 64");
+  return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, precisions, 0, stages, opt,
+                    (hipStream_t)stream, "pss_sbe_stages");
+}
+
 // ---- kernel-level entries: parts of the step on the wires' own buffers (for tests and for
 // callers of the reference's module functions); the same kernels as the step
 extern "C" int pss_calc_h(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
@@ -2313,6 +2321,63 @@ extern "C" int pss_qwpol(const pss_wire_desc* d_descs, int32_t nwires, int32_t m
                          int32_t max_Nr, int32_t max_nrr, void* stream) {
   return sbe_stages(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, ST_TAIL, 2, (hipStream_t)stream,
                     "pss_qwpol");
+}
+
+// ---- dpdt / dCdt / dDdt of explicit matrices (the reference's module functions), one
+// output element per thread, plain FP64 sums in the order of the reference's terms
+__global__ void __launch_bounds__(256) k_rhs_dense(int Nk, int which, const cplx* __restrict__ C,
+                                                   const cplx* __restrict__ D, const cplx* __restrict__ p,
+                                                   const cplx* __restrict__ Heh, const cplx* __restrict__ Hee,
+                                                   const cplx* __restrict__ Hhh, const double* __restrict__ GamE,
+                                                   const double* __restrict__ GamH, const cplx* __restrict__ Off,
+                                                   cplx* __restrict__ out) {
+  const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (o >= (int64_t)Nk * Nk) return;
+  const int a = (int)(o / Nk), b = (int)(o % Nk);  // output [a][b]
+  cplx t1 = mk(0, 0), t2 = mk(0, 0), t3 = mk(0, 0), t4 = mk(0, 0);
+  cplx r;
+  if (which == 0) {  // dpdt[kh = a][ke = b], PSTD3D/SBEs.py:1953-1976
+    for (int j = 0; j < Nk; ++j) {
+      t1 = t1 + Hhh[(int64_t)a * Nk + j] * p[(int64_t)j * Nk + b];
+      t2 = t2 + Hee[(int64_t)b * Nk + j] * p[(int64_t)a * Nk + j];
+      t3 = t3 + Heh[(int64_t)j * Nk + a] * C[(int64_t)j * Nk + b];
+      t4 = t4 + Heh[(int64_t)b * Nk + j] * D[(int64_t)j * Nk + a];
+    }
+    const double g = kHbar * (GamE[b] + GamH[a]);
+    const cplx pv = p[o];
+    r = t1 + t2 - t3 - t4 + Heh[(int64_t)b * Nk + a] - mk(-g * pv.y, g * pv.x) + (Off ? Off[o] : mk(0, 0));
+  } else if (which == 1) {  // dCdt[k1 = a][k2 = b], PSTD3D/SBEs.py:2320-2338
+    for (int j = 0; j < Nk; ++j) {
+      t1 = t1 + Hee[(int64_t)b * Nk + j] * C[(int64_t)a * Nk + j];
+      t2 = t2 + Hee[(int64_t)j * Nk + a] * C[(int64_t)j * Nk + b];
+      t3 = t3 + Heh[(int64_t)b * Nk + j] * conj(p[(int64_t)j * Nk + a]);
+      t4 = t4 + conj(Heh[(int64_t)a * Nk + j]) * p[(int64_t)j * Nk + b];
+    }
+    r = t1 - t2 + t3 - t4;
+  } else {  // dDdt[k1 = a][k2 = b], PSTD3D/SBEs.py:2618-2636
+    for (int j = 0; j < Nk; ++j) {
+      t1 = t1 + Hhh[(int64_t)b * Nk + j] * D[(int64_t)a * Nk + j];
+      t2 = t2 + Hhh[(int64_t)j * Nk + a] * D[(int64_t)j * Nk + b];
+      t3 = t3 + Heh[(int64_t)j * Nk + b] * conj(p[(int64_t)a * Nk + j]);
+      t4 = t4 + conj(Heh[(int64_t)j * Nk + a]) * p[(int64_t)b * Nk + j];
+    }
+    r = t1 - t2 + t3 - t4;
+  }
+  out[o] = div_ih(r, kHbar);
+}
+
+extern "C" int pss_rhs_dense(int32_t Nk, int32_t which, const double* C, const double* D, const double* p,
+                             const double* Heh, const double* Hee, const double* Hhh, const double* GamE,
+                             const double* GamH, const double* Off, double* out, void* stream) {
+  if (Nk < 1 || Nk > 8192 || which < 0 || which > 2) return pss_set_msg(2, "pss_rhs_dense: 1 <= Nk <= 8192, which 0..2");
+  if (!C || !D || !p || !Heh || !Hee || !Hhh || !out || (which == 0 && (!GamE || !GamH)))
+    return pss_set_msg(2, "pss_rhs_dense: null operand");
+  const int64_t nn = (int64_t)Nk * Nk;
+  hipLaunchKernelGGL(k_rhs_dense, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, Nk, which,
+                     (const cplx*)C, (const cplx*)D, (const cplx*)p, (const cplx*)Heh, (const cplx*)Hee,
+                     (const cplx*)Hhh, GamE, GamH, (const cplx*)Off, (cplx*)out);
+  PSS_CHECK(hipGetLastError());
+  return 0;
 }
 
 // ---- state access: (p1, p2, C1, C2, D1, D2) of one wire <-> a packed [6][Nk][Nk] complex

@@ -75,18 +75,34 @@ def _first_token(line):
     return line.split()[0]
 
 
+def _num(tok):
+    """A number token; Fortran exponents (1d-9, 2D3) read as e (PSTD3D/typetime.py:31-50)."""
+    return float(tok.replace("d", "e").replace("D", "E"))
+
+
+def _flag(tok):
+    """0 / 1 as the reference reads them; Fortran .true. / .false. accepted too."""
+    t = tok.strip().lower()
+    if t in (".true.", "t", "true"):
+        return True
+    if t in (".false.", "f", "false"):
+        return False
+    return bool(int(t))
+
+
 def read_qw_params(path="params/qw.params"):
-    """PSTD3D/SBEs.py:4990-5014 (values kept in the file's units)."""
+    """PSTD3D/SBEs.py:4990-5014: the first token of each of 15 lines (values kept in the
+    file's units)."""
     with open(path, "r", encoding="utf-8") as f:
         vals = [_first_token(f.readline()) for _ in range(15)]
-    p = QWParams(*[float(v) for v in vals[:13]], int(vals[13]), int(vals[14]))
+    p = QWParams(*[_num(v) for v in vals[:13]], int(_num(vals[13])), int(_num(vals[14])))
     return p
 
 
 def read_mb_params(path="params/mb.params"):
-    """PSTD3D/SBEs.py:5016-5034"""
+    """PSTD3D/SBEs.py:5016-5034: the first token of each of 15 lines."""
     with open(path, "r", encoding="utf-8") as f:
-        vals = [bool(int(_first_token(f.readline()))) for _ in range(15)]
+        vals = [_flag(_first_token(f.readline())) for _ in range(15)]
     return MBFlags(*vals)
 
 
@@ -286,4 +302,74 @@ def _build_model(qw, mb, rr, r0=0.0):
                      density_max=min(ae / 2.0, ah / 2.0) / pi * 0.99, Delta0=qw.Delta0, epsr=qw.epsr,
                      vpref=e0**2 / (twopi * eps0 * qw.epsr * L), Oph=Oph,
                      dr=(r[2] - r[1]) * (Nr - 1) / float(Nr), qc=2.0 * ae * ah / (ae + ah))
+    return M
+
+
+def explicit_model(ky, Ee, Eh, mb, *, y=None, L=None, Delta0=5e-9, me=0.07 * me0, mh=0.45 * me0, ge=1e12, gh=1e12,
+                   alphae=None, alphah=None, er=9.1, Qy=None, kkp=None, Oph=0.0, Gph=0.0, epsr_ph=None):
+    """A WireModel built from explicitly given grids and constants, as the reference's module
+    initialisers receive them (InitializeCoulomb PSTD3D/coulomb.py:1351-1374,
+    InitializePhonons PSTD3D/phonons.py:55-136), for the kernel-level entries those modules'
+    functions run on.  Quantities those functions do not use (the optical tables, the
+    propagation-grid splines) hold neutral placeholders of the right sizes.
+
+    y: the wire's real-space grid (the V(q) integral runs over it); Qy, kkp: the momentum
+    difference grid and the index map of CalcCoulombArrays (PSTD3D/coulomb.py:502-577),
+    V(|Qy[kkp[k1, k2]]|) depends on k1 - k2 only."""
+    kr = np.asarray(ky, dtype=np.float64)
+    Ee = np.asarray(Ee, dtype=np.float64)
+    Eh = np.asarray(Eh, dtype=np.float64)
+    Nk = len(kr)
+    M = WireModel(qw=QWParams(), mb=MBFlags(**vars(mb)))
+    dk = kr[1] - kr[0]
+    L = float(L) if L is not None else twopi / (2.0 * dk)
+    y = np.asarray(y, dtype=np.float64) if y is not None else np.linspace(-L, L, 2 * Nk + 2)
+    Nr = len(y)
+    M.Nk, M.Nr = Nk, Nr
+    ae = alphae if alphae is not None else 1.0 / Delta0
+    ah = alphah if alphah is not None else 1.0 / Delta0
+    d = np.arange(Nk)
+    dky = np.abs(kr[d] - kr[0])
+    qe = np.maximum(dky, ae / 2.0)
+    qh = np.maximum(dky, ah / 2.0)
+    Re = np.sqrt((2.0 / ae) ** 2 + Delta0**2)
+    Rh = np.sqrt((2.0 / ah) ** 2 + Delta0**2)
+    scale = e0**2 / (twopi * eps0 * er * hbar**2)
+    chie = me * _k03(qe * Re) / qe * scale
+    chih = mh * _k03(qh * Rh) / qh * scale
+    if Qy is not None and kkp is not None:
+        Vq = np.abs(np.asarray(Qy, dtype=np.float64)[np.asarray(kkp)[d, 0]])
+    else:
+        Vq = d * dk
+    kmx, kmn, Nk0 = kr[Nk - 1] + dk, kr[0] - dk, (Nk - 1) // 2 + 1
+    K, Q = kr[:, None], kr[None, :]
+
+    def idx(v):
+        return (np.round(np.clip(v, kmn, kmx) / dk)).astype(np.int64) + Nk0 - 1
+
+    k1p0 = ((me + mh) * Q - 2 * mh * K) / 2.0 / me
+    k10 = ((me + mh) * Q - 2 * me * K) / 2.0 / mh
+    ktab = np.stack([idx(K + Q), idx(K - Q), idx(k1p0), idx(k1p0 - Q), idx(k10), idx(k10 - Q)]).astype(np.int32)
+    xe = me / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
+    xh = mh / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
+    iqv = np.round(np.abs(kr / dk)).astype(np.int32)
+    rr = np.linspace(y[0], y[-1], 8)
+    sa_h, sa_t, sa_b, sa_end = spline_factors(rr)
+    sa_idx, sa_dx = spline_eval_index(rr, y)
+    sb_h, sb_t, sb_b, sb_end = spline_factors(y)
+    sb_idx, sb_dx = spline_eval_index(y, rr)
+    epsr_ph = er if epsr_ph is None else epsr_ph
+    N0 = 1.0 / (np.exp(hbar * Oph / kB / TEMP) - 1.0) if Oph > 0 else 0.0
+    Vscale = hbar * Oph * epsr_ph * (1.0 / EPSRINF_HOST - 1.0 / EPSR0_HOST)
+    z = np.zeros(Nk)
+    M.arrays = dict(kr=kr, Ee=Ee, Eh=Eh, win=np.ones(Nr), r=y, chie=chie, chih=chih, qe=qe, qh=qh, ktab=ktab, xe=xe,
+                    xh=xh, iqv=iqv, twid=np.exp(-2j * np.pi * np.arange(Nr) / Nr), sa_x=rr, sa_t=sa_t, sa_b=sa_b,
+                    sa_h=sa_h, sa_end=sa_end, sa_idx=sa_idx, sa_dx=sa_dx, sb_t=sb_t, sb_b=sb_b, sb_h=sb_h,
+                    sb_end=sb_end, sb_idx=sb_idx, sb_dx=sb_dx, fe0=z, fh0=z, Vq=Vq)
+    M.scalars = dict(L=L, gap=0.0, me=me, mh=mh, ehint=1.0, area=1.0, dcv=0.0, dkr=dk, ae=ae, ah=ah,
+                     dr_r=y[1] - y[0], drr=rr[1] - rr[0], gam_e=ge, gam_h=gh, gam_eh=(ge + gh) / 2.0,
+                     Vscale=Vscale, N0=N0, hOph=hbar * Oph, hGph=hbar * Gph, Gph=Gph,
+                     density_max=min(ae / 2.0, ah / 2.0) / pi * 0.99, Delta0=Delta0, epsr=er,
+                     vpref=e0**2 / (twopi * eps0 * er * L), Oph=Oph, dr=(y[2] - y[1]) * (Nr - 1) / float(Nr),
+                     qc=2.0 * ae * ah / (ae + ah))
     return M

@@ -89,10 +89,11 @@ def coupled_params(name):
     x = np.linspace(-c["dx"] * (Nx - 1) / 2.0, c["dx"] * (Nx - 1) / 2.0, Nx)
     c["x0"] = float(x[int(round(0.25 * Nx))]) if c["src_plane"] else 0.0
     c["pulse"] = (800e-9, c["amp"], 100e-15, 50e-15, 0.0)
+    c.setdefault("current", "reference")
     return c
 
 
-def oracle_coupled(name):
+def oracle_coupled(name, current=None):
     """(propagator, coupling) of the FP64 oracle for config `name`."""
     from oracle import pstd as OP
     from oracle.coupling import OracleCoupling
@@ -101,14 +102,14 @@ def oracle_coupled(name):
     c = coupled_params(name)
     sp = OP.Space(3, *c["shape"], c["dx"], c["dx"], c["dx"], c["epsr"])
     oc = OracleCoupling(sp, 800e-9, 0.0, Nw=1, qw=QWParams(L=L_for_nk(c["nk"])), mb=MBFlags(Screened=c["screened"]),
-                        x0=c["x0"], current="reference")
+                        x0=c["x0"], current=current or c["current"])
     prop = OP.Propagator(sp, OP.Time(0.0, 1.0, c["dt"], 0), OP.Pulse(*c["pulse"]), "soft", "cpml", calc_j=oc.calc_j,
                          npml=c["npml"])
     assert oc.wire.Nk == c["nk"]
     return prop, oc
 
 
-def device_coupled(name, precision="fp64", feedback=True):
+def device_coupled(name, precision="fp64", feedback=True, current=None):
     """(propagator, QWArray) of the HIP path for config `name` (device time, pipelined-capable)."""
     from pulsesuite_amd.PSTD3D import PSTD3DPropagator
     from pulsesuite_amd.rhoPJ import QWArray
@@ -120,7 +121,7 @@ def device_coupled(name, precision="fp64", feedback=True):
     c = coupled_params(name)
     sp = ss(3, *c["shape"], c["dx"], c["dx"], c["dx"], c["epsr"])
     qa = QWArray(QWParams(L=L_of_nk(c["nk"])), MBFlags(Screened=c["screened"], ignorewire=not feedback), Nw=1,
-                 Emax0=0.0, x0=c["x0"], precision=precision, current="reference")
+                 Emax0=0.0, x0=c["x0"], precision=precision, current=current or c["current"])
     prop = PSTD3DPropagator(sp, ts(0.0, 1.0, c["dt"], 0), ps(*c["pulse"]), source_type="soft",
                             boundary_type="cpml", calc_j=qa, npml=c["npml"], check_every=0, precision=precision,
                             device_time=True)

@@ -199,3 +199,27 @@ def test_state_get_set_round_trip_resumes_the_run():
     sa, sb = a.get_state(1), b.get_state(1)
     for k in sa:
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_checkpoint_files_resume_the_run(tmp_path):
+    """write_sbes_data / read_sbes_data (the reference's text checkpoint, PSTD3D/SBEs.py:
+    5536-5581): a solver resumed from the files steps on exactly as the uninterrupted one."""
+    import torch
+
+    nk = 18
+    ref, a, st, lines, rr = make(nk, "fp64")
+    outs = [np.zeros(len(rr), complex) for _ in range(4)]
+    E = lines[0]
+    step = lambda s: s.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, 2e-17, 1, *outs, [False], [False])  # noqa
+    for _ in range(3):
+        step(a)
+    a.write_sbes_data(3, str(tmp_path) + "/")
+    _, b, _, _, _ = make(nk, "fp64", seed=21)
+    b.read_sbes_data(3, str(tmp_path) + "/")
+    for _ in range(2):
+        step(a)
+        step(b)
+    torch.cuda.synchronize()
+    sa, sb = a.get_state(1), b.get_state(1)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k

@@ -734,3 +734,114 @@ def test_fft_pass_acc_matches_numpy(shape, axis, in32, curl):
         assert np.linalg.norm(got - w) / scale < tol
     for t, a in zip(tin, A):  # input untouched
         assert np.array_equal(t.cpu().numpy(), a.astype(np.complex64) if in32 else a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("src_comp", [-1, 1, 2])
+def test_mask_half_matches_oracle(src_comp):
+    """pss_mask_half: the masking absorber (PSTD3D/absorber.py:160-191) of three k-space
+    fields and, on src_comp, the soft-source add of the same real-space pass, against the
+    oracle's ApplyAbsorber + UpdateTFSC (which round-trips the field once more)."""
+    import torch
+
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+
+    shape, dx, dt = (32, 16, 64), 20e-9, 5e-17
+    pulse = (800e-9, 3e8, 5e-15, 4e-15, 0.0)
+    prop = PSTD3DPropagator(ss(3, *shape, dx, dx, dx, 9.1), ts(2e-15, 1.0, dt, 0), ps(*pulse), source_type="soft",
+                            boundary_type="mask")
+    rng = np.random.default_rng(4 + src_comp)
+    F = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    T = [cu(f) for f in F]
+    src = prop._src(src_comp, 3e8, 2e-15) if src_comp >= 0 else None
+    prop.absorber.apply(*T, src=src)
+    torch.cuda.synchronize()
+    osp = OP.Space(3, *shape, dx, dx, dx, 9.1)
+    a = OP.Absorber(*shape, dt)
+    W = [f.copy() for f in F]
+    a.apply(*W)
+    if src_comp >= 0:
+        OP.UpdateTFSC(W[src_comp], OP.InitializeTFSF(osp, OP.Pulse(*pulse)), osp, OP.Time(2e-15, 1.0, dt, 0),
+                      OP.Pulse(*pulse), 3e8)
+    for t, w in zip(T, W):
+        assert rel(t, w) < 1e-13
+    assert not np.allclose(W[0], F[0])  # the mask acted
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("polarization", ["y", "z"])
+def test_mask_boundary_coupled_matches_oracle(polarization):
+    """The reference's default boundary (mask) in the coupled step: soft source of either
+    polarisation, one wire (Nw = 2) with the current fed back, against the oracle (E and B as
+    vector fields, the wire's p and C)."""
+    from oracle.coupling import OracleCoupling
+    from oracle.sbe import MBFlags as OMB
+    from oracle.sbe import QWParams as OQW
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.rhoPJ import QWArray
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    N, nk, dx, epsr, steps = 32, 14, 20e-9, 9.1, 10
+    dt = 0.9 * 2 * dx / (np.pi * 2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    pulse = dict(lambda_=800e-9, Amp=3e8, Tw=5e-15, Tp=4e-15, chirp=0.0)
+    x0 = -0.25 * N * dx  # near the source plane: the wire is driven from the start
+    osp = OP.Space(3, N, N, N, dx, dx, dx, epsr)
+    oc = OracleCoupling(osp, 800e-9, 0.0, Nw=2, qw=OQW(L=L_of_nk(nk)), mb=OMB(), x0=x0)
+    oprop = OP.Propagator(osp, OP.Time(0.0, 1.0, dt, 0), OP.Pulse(**pulse), "soft", "mask", calc_j=oc.calc_j,
+                          polarization=polarization)
+    qa = QWArray(QWParams(L=L_of_nk(nk)), MBFlags(), Nw=2, Emax0=0.0, x0=x0)
+    prop = PSTD3DPropagator(ss(3, N, N, N, dx, dx, dx, epsr), ts(0.0, 1.0, dt, 0), ps(**pulse), source_type="soft",
+                            calc_j=qa, polarization=polarization)
+    assert prop.boundary_type == "mask"  # the reference's default
+    for _ in range(steps):
+        oprop.step()
+        prop.step()
+    for names in (("Ex", "Ey", "Ez"), ("Bx", "By", "Bz")):
+        num = sum(np.linalg.norm(getattr(prop, n).cpu().numpy() - getattr(oprop, n)) ** 2 for n in names)
+        den = sum(np.linalg.norm(getattr(oprop, n)) ** 2 for n in names)
+        assert np.sqrt(num / den) < 1e-10, names
+    big = "Ey" if polarization == "y" else "Ez"
+    assert np.linalg.norm(getattr(oprop, big)) > 10 * np.linalg.norm(getattr(oprop, "Ez" if big == "Ey" else "Ey"))
+    for w in range(2):
+        st, so = qa.wires.get_state(w), oc.wire.state[w]
+        assert rel(st["p2"], so.p2) < 1e-10 and rel(st["C2"], so.C2) < 1e-10
+    assert np.abs(oc.wire.state[0].p2).max() > 0
+
+
+@pytest.mark.gpu
+def test_mask_closed_loop_stays_bounded():
+    """128^3 with an Nk = 256 wire on the source plane and its current fed back: with the
+    CPML the undriven components grow from the point current (~1e4 per 50 steps), with the
+    mask they stay small next to the driven field over 600 steps."""
+    import torch
+
+    from pulsesuite_amd.PSTD3D import PSTD3DPropagator
+    from pulsesuite_amd.rhoPJ import QWArray
+    from pulsesuite_amd.typepulse import ps
+    from pulsesuite_amd.typespace import ss
+    from pulsesuite_amd.typetime import ts
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    N, dx, epsr = 128, 20e-9, 9.1
+    dt = 0.9 * 2 * dx / (np.pi * 2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
+    qa = QWArray(QWParams(L=L_of_nk(256)), MBFlags(Screened=False), Nw=1, Emax0=0.0, x0=-0.25 * N * dx)
+    prop = PSTD3DPropagator(ss(3, N, N, N, dx, dx, dx, epsr), ts(0.0, 1.0, dt, 0), ps(800e-9, 1e7, 100e-15, 50e-15, 0.0),
+                            source_type="soft", boundary_type="mask", calc_j=qa, check_every=50, device_time=True)
+    prop.capture_step()
+    norms = []
+    for _ in range(6):
+        prop.run_graph(100)
+        torch.cuda.synchronize()
+        norms.append([float(torch.linalg.vector_norm(getattr(prop, n))) for n in ("Ex", "Ey", "Ez")])
+        print("mask closed loop", prop.time.n, ["%.3e" % v for v in norms[-1]], flush=True)
+    st = qa.wires.get_state(0)
+    assert np.abs(np.diag(st["C2"])).max() > 1e-8  # driven
+    for ex, ey, ez in norms:
+        assert np.isfinite([ex, ey, ez]).all()
+        assert ex < 1e-2 * ey and ez < 1e-2 * ey

@@ -152,3 +152,34 @@ def test_spline_rk4_golden(gold):
     D = lambda t, yy: A @ yy + np.sin(t)  # noqa: E731
     rk4(y, D(0.0, y), 0.0, 0.1, D)
     assert rel(y, gold["rk4_y1"]) < 1e-14
+
+
+def test_absorber_mask_closed_forms():
+    """The masking absorber's profile (PSTD3D/absorber.py:62-122): 1 - U0 = 0.5 at the outer
+    left cell, exp(-gmax rho^3 dt) with gmax = ln 2 / dt on both layers (the right one's grading
+    reaching (npml - 1)/npml at the last cell), 1 inside; thickness rule of :39-59; the 3-D mask
+    is the product of the three profiles.  The device module builds the same host profiles."""
+    dt = 3e-17
+    for N, want in ((4, 0), (32, 3), (64, 6), (128, 6), (256, 12), (1024, 51)):
+        assert P.CalcNPML_mask(N) == want, N
+    N, p = 128, 12
+    m = P.build_1d_mask(N, p, dt)
+    assert m[0] == 0.5
+    assert np.all(m[p:N - p] == 1.0)
+    for i in range(p):
+        assert np.isclose(m[i], np.exp(-np.log(2.0) * ((p - 1 - i) / (p - 1)) ** 3), rtol=1e-15)
+    for i in range(N - p, N):
+        assert np.isclose(m[i], np.exp(-np.log(2.0) * ((i - (N - p)) / p) ** 3), rtol=1e-15)
+    assert m[N - 1] > 0.5 and m[N - p] == 1.0  # the right layer is graded one cell short
+    a = P.Absorber(32, 8, 64, dt)
+    mx, my, mz = (P.build_1d_mask(n, P.CalcNPML_mask(n), dt) for n in (32, 8, 64))
+    assert np.array_equal(a.mask, mx[:, None, None] * my[None, :, None] * mz[None, None, :])
+    import importlib.util
+
+    if importlib.util.find_spec("torch") is not None:
+        from pulsesuite_amd import absorber as A  # host-side profiles only (no device work)
+
+        # the same formula element by element (scalar exp as the reference's loop; the oracle's
+        # vectorised exp may differ in the last bit)
+        assert np.max(np.abs(A.build_1d_mask(N, p, dt) - m) / m) <= 4e-16
+        assert all(A.CalcNPML(n) == P.CalcNPML_mask(n) for n in (4, 5, 31, 64, 100, 300, 2048))
