@@ -92,8 +92,16 @@ def parse():
 AMP = 1e7  # soft-source amplitude (V/m)
 
 
-def setup(grid, nk, npml, precision, feedback=True, current="reference", device_time=True, dt=None):
-    """The coupled step of the headline workload (or its 256^3 / Nk=1024 precision form)."""
+def source_plane_x(grid):
+    """x of the soft source's plane (cell round(0.25 N) of the grid, PSTD3D/tfsf.py:26-60)."""
+    x = np.linspace(-DX * (grid - 1) / 2.0, DX * (grid - 1) / 2.0, grid)
+    return float(x[int(round(0.25 * grid))])
+
+
+def setup(grid, nk, npml, precision, feedback=True, current="reference", device_time=True, dt=None, x0=0.0,
+          boundary="cpml"):
+    """The coupled step of the headline workload (wire at the grid centre), or its 256^3 /
+    Nk=1024 precision form; x0 moves the wire (the source plane: driven from the first step)."""
     from pulsesuite_amd.PSTD3D import PSTD3DPropagator
     from pulsesuite_amd.rhoPJ import QWArray
     from pulsesuite_amd.typepulse import ps
@@ -105,8 +113,8 @@ def setup(grid, nk, npml, precision, feedback=True, current="reference", device_
     sp = ss(3, grid, grid, grid, DX, DX, DX, EPSR)
     pulse = ps(800e-9, AMP, 100e-15, 50e-15, 0.0)
     mb = MBFlags(Screened=False, ignorewire=not feedback)  # bare Coulomb for this config
-    qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, precision=precision, current=current)
-    prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type="cpml",
+    qa = QWArray(QWParams(L=L_of_nk(nk)), mb, Nw=1, Emax0=0.0, x0=x0, precision=precision, current=current)
+    prop = PSTD3DPropagator(sp, ts(0.0, 1.0, dt, 0), pulse, source_type="soft", boundary_type=boundary,
                             calc_j=qa, npml=npml, check_every=0, precision=precision, device_time=device_time)
     return prop, qa
 
@@ -138,9 +146,13 @@ def time_steps(prop, steps, warmup, graph=True):
 
 
 def advance(prop, steps, graph=True):
-    """Advance without timing (pipelined graph replays when supported)."""
+    """Advance without timing (pipelined graph replays when supported, else graph replays of
+    the plain step)."""
     if graph and prop.pipelined():
         prop.run_pipelined(steps)
+    elif graph:
+        prop.capture_step()
+        prop.run_graph(steps)
     else:
         for _ in range(steps):
             prop.step()
@@ -328,57 +340,53 @@ def serving(nreq, rate, seed=0, precision="mixed"):
     return o
 
 
-def precision_form(grid=256, nk=1024, npml=12, steps=200, open_steps=1000, closed_gate=50, warmup=2, graph=True,
-                   current="reference"):
+def precision_form(grid=256, nk=1024, npml=12, steps=200, open_steps=1000, closed_gate=50, mask_steps=0,
+                   warmup=2, graph=True, current="reference"):
     """The 256^3 / Nk = 1024 precision form of the coupled step, FP64 and mixed: throughput
-    of a timed closed-loop run (the current fed back) of `steps` steps; the mixed run's
-    relative error per output against the FP64 run after `closed_gate` closed-loop steps and
-    after `open_steps` steps with the current not fed back (past the pulse).  At the end of the
-    timed run the closed loop has grown (the CPML amplifies what the point current radiates):
-    its errors are reported beside those of a second FP64 run that differs from the first
-    only in rounding (split-K factor 1), which part the same way."""
+    of a timed closed-loop run (the current fed back, wire at the grid centre) of `steps`
+    steps; then the mixed run's relative error per output against the FP64 run with the wire
+    on the source plane (driven from the first step): after `closed_gate` closed-loop steps
+    with the CPML, after `open_steps` steps with the current not fed back (past the pulse),
+    and, when mask_steps > 0, after that many closed-loop steps with the reference's default
+    mask absorber (off by default: a graph replay of that form ended once with a host
+    SIGSEGV whose cause is not found).  The closed loop with the CPML grows past ~100 steps on
+    this grid (the CPML amplifies what the point current radiates, and FP64 runs that differ
+    only in rounding part the same way)."""
     import torch
 
-    from pulsesuite_amd import _lib
-
     out = {"workload": f"3-D PSTD {grid}^3 + {npml}-cell CPML + soft source, 1 wire Nk={nk}, current fed back, "
-                       f"{warmup} + {steps} leapfrog steps timed; errors after {closed_gate} closed-loop steps and "
-                       f"{open_steps} steps with ignorewire"}
+                       f"{warmup} + {steps} leapfrog steps timed; errors (wire on the source plane) after "
+                       f"{closed_gate} closed-loop steps and {open_steps} steps with ignorewire"
+                       + (f", and {mask_steps} closed-loop steps with the mask absorber" if mask_steps > 0 else "")}
     res = {}
     for prec in ("fp64", "mixed"):
         print(f"[bench] precision form {grid}^3 Nk={nk} {prec}", file=sys.stderr, flush=True)
         prop, qa = setup(grid, nk, npml, prec, True, current)
         el = time_steps(prop, steps, warmup, graph=graph)
-        res[prec] = dict(prop=prop, qa=qa, value=nk * steps / el, ms=1e3 * el / steps)
-    end_errs = precision_errors(res["fp64"]["prop"], res["fp64"]["qa"], res["mixed"]["prop"], res["mixed"]["qa"])
-    # the same FP64 run with another summation order of the contractions (split-K 1)
-    _lib.check(_lib.lib.pss_sbe_tune(1), "pss_sbe_tune")
-    try:
-        prop, qa = setup(grid, nk, npml, "fp64", True, current)
-        time_steps(prop, steps, warmup, graph=graph)
-        rnd_errs = precision_errors(res["fp64"]["prop"], res["fp64"]["qa"], prop, qa)
+        res[prec] = dict(value=nk * steps / el, ms=1e3 * el / steps)
         del prop, qa
-    finally:
-        _lib.check(_lib.lib.pss_sbe_tune(0), "pss_sbe_tune")
+        torch.cuda.empty_cache()
     out.update(steps=steps, fp64={"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms"]},
                mixed={"value": res["mixed"]["value"], "ms_per_step": res["mixed"]["ms"]},
-               mixed_over_fp64=res["mixed"]["value"] / res["fp64"]["value"],
-               end_of_timed_run={"steps": warmup + steps, "mixed_rel_err_vs_fp64": end_errs,
-                                 "fp64_splitk1_rel_err_vs_fp64": rnd_errs})
-    del res
-    torch.cuda.empty_cache()
+               mixed_over_fp64=res["mixed"]["value"] / res["fp64"]["value"])
     gate = {}
-    for tag, feedback, n in (("closed_loop", True, closed_gate), ("open_loop", False, open_steps)):
+    x0 = source_plane_x(grid)
+    for tag, feedback, n, bnd in (("closed_loop", True, closed_gate, "cpml"), ("open_loop", False, open_steps, "cpml"),
+                                  ("closed_loop_mask", True, mask_steps, "mask")):
         if n <= 0:
             continue
         runs = {}
         for prec in ("fp64", "mixed"):
             print(f"[bench] precision form {grid}^3 Nk={nk} {prec} {tag} {n} steps", file=sys.stderr, flush=True)
-            prop, qa = setup(grid, nk, npml, prec, feedback, current)
+            prop, qa = setup(grid, nk, npml, prec, feedback, current, x0=x0, boundary=bnd)
             advance(prop, n, graph)
             runs[prec] = (prop, qa)
-        gate[tag] = {"steps": n, "feedback": feedback,
-                     "mixed_rel_err_vs_fp64": precision_errors(*runs["fp64"], *runs["mixed"])}
+        torch.cuda.synchronize()
+        st = runs["fp64"][1].wires.get_state(0)
+        gate[tag] = {"steps": n, "feedback": feedback, "boundary": bnd,
+                     "mixed_rel_err_vs_fp64": precision_errors(*runs["fp64"], *runs["mixed"]),
+                     "wire_driven": {"max_population": float(np.diag(st["C2"]).real.max()),
+                                     "norm_p": float(np.linalg.norm(st["p2"]))}}
         del runs
         torch.cuda.empty_cache()
     out["precision_gate"] = gate

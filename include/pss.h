@@ -25,9 +25,11 @@ enum {
   PSS_EHS = 1 << 2,
   PSS_SCREENED = 1 << 3,
   PSS_PHONON = 1 << 4,
+  PSS_DCTRANS = 1 << 5,
   PSS_LF = 1 << 6,
   PSS_DIAGDPH = 1 << 8,
   PSS_OFFDIAGDPH = 1 << 9,
+  PSS_RECOMB = 1 << 10,
   PSS_IGNOREWIRE = 1 << 12,
   PSS_LORENTZDELTA = 1 << 14,
 };
@@ -47,6 +49,8 @@ typedef struct pss_wire_desc {
   double gam_e, gam_h, gam_eh;
   double Vscale, N0, hOph, hGph, Gph;
   double density_max, gate;      /* gate = 1e-3 * Emax0 (wireoff threshold) */
+  double Edc;                    /* background DC field (V/m) of the transport shift (PSS_DCTRANS) */
+  double rsp[4];                 /* PSS_RECOMB: Rsp(E) = rsp[0] + rsp[1] E + rsp[2] E^2 + rsp[3] E^3 */
   double pad2[3];
   /* tables (read only) */
   const double *kr, *Ee, *Eh;    /* [Nk] */
@@ -98,13 +102,23 @@ int pss_sbe_tune(int32_t ksplit);
  */
 int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
                  int32_t max_nrr, void* stream);
-/* The same step for a batch whose wires hold only the precisions in the mask
- * `precisions` (1: PSS_FP64 wires, 2: PSS_MIXED wires, 3: both): the kernels of an absent
- * precision are not launched (pss_sbe_step launches all of them; each returns at once for
- * wires of the other precision).  ksplit_cap > 0 (a power of two <= 16) caps every wire's
- * split-K factor, which is otherwise chosen to fill the chip as if the wire ran alone: a
- * step that runs beside other work (the wire step of the coupled step) does better with
- * fewer workgroups.  pss_sbe_step = precisions 3, no cap. */
+/* The same step for a batch whose wires hold only the kinds in the mask `precisions`
+ * (1: PSS_FP64 wires, 2: PSS_MIXED wires, at least one of them; 4: wires with PSS_DCTRANS,
+ * 8: wires with PSS_RECOMB): the kernels of an absent kind are not launched (pss_sbe_step
+ * launches all of them; each returns at once for wires it does not apply to).  A wire whose
+ * flags ask for DCTrans or Recomb in a launch without that bit gets NaN outputs (Pxx, Pyy,
+ * Pzz, Rho) rather than a step without the term.  ksplit_cap > 0 (a power of two <= 16)
+ * caps every wire's split-K factor, which is otherwise chosen to fill the chip as if the
+ * wire ran alone: a step that runs beside other work (the wire step of the coupled step)
+ * does better with fewer workgroups.  pss_sbe_step = precisions 15, no cap.
+ *
+ * DCTrans (PSTD3D/dcfield.py:1213-1247, called at PSTD3D/SBEs.py:5698-5702): after the
+ * relaxation, C3, D3 and p3 are shifted in momentum by dk = -e (Edc + Re Ex(q=0)) dt / hbar,
+ * both indices when PSS_LF is set, else the diagonals only.  The FFT shift of the reference
+ * is the circulant matrix S = F^-1 diag(exp(-i Y dk)) F (X -> S X S^T); its first column
+ * has the closed form of a Dirichlet kernel, and the two products run as FP64 tiles.
+ * Recomb (PSTD3D/emission.py:94-131, SBEs.py:5335-5337): Rsp from the cur populations and
+ * screened V, then ne3 *= exp(-Rsp nh3 dt), nh3 *= exp(-Rsp ne3 dt) inside the relaxation. */
 int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk, int32_t max_Nr,
                    int32_t max_nrr, int32_t precisions, int32_t ksplit_cap, void* stream);
 

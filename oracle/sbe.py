@@ -25,6 +25,7 @@ Pinning (details in the tests):
 from dataclasses import dataclass, field
 
 import numpy as np
+from scipy.constants import c as c0
 from scipy.constants import e as e0
 from scipy.constants import epsilon_0 as eps0
 from scipy.constants import hbar
@@ -191,7 +192,7 @@ class OracleWire:
     parameter files replaced by dataclasses; file output is omitted.
     """
 
-    def __init__(self, rr, r0, Emaxxx, lam, Nw, qw=None, mb=None):
+    def __init__(self, rr, r0, Emaxxx, lam, Nw, qw=None, mb=None, integrator="leapfrog"):
         self.qwp = qw or QWParams()
         self.mb = mb or MBFlags()
         q = self.qwp
@@ -266,15 +267,85 @@ class OracleWire:
         self._init_coulomb()
         self._init_phonons()
         self._init_dephasing()
+        self._init_dc()
+        if self.mb.Recomb:
+            self._init_emission()
 
         rr_offset = np.asarray(rr) - r0
         self.Nr1 = locator(rr_offset, self.r[0])
         self.Nr2 = locator(rr_offset, self.r[Nr - 1])
         self.wph = (self.gap + hbar * self.Oph) / hbar
         self.wireoff = True
+        if integrator not in ("leapfrog", "rk4"):
+            raise ValueError(f"integrator must be 'leapfrog' or 'rk4', not {integrator!r}")
+        self.integrator = integrator
         self.xxx = 1
         self.jjj = 1
         self.jmax = self.qwp.jmax
+
+    # --- DC transport  PSTD3D/dcfield.py:782-793, 1169-1247 ------------------
+    def _init_dc(self):
+        """The conjugate grid of the momentum shift: FFT-order wavenumbers of an Nk-point
+        grid of length (Nk - 1) dky, dky = kr[2] - kr[1] (dcfield.py:782-793, 39-68)."""
+        kr = self.kr
+        dky = kr[2] - kr[1] if self.Nk > 2 else kr[1] - kr[0]
+        self.dcY = GetKArray(self.Nk, (self.Nk - 1) * dky)
+
+    def Transport(self, X, Eac, dt, k1nek2):
+        """In place: shift X in momentum by dk = -e (Edc + Eac) dt / hbar through the FFT
+        (DCFieldModule.Transport, dcfield.py:1213-1247): both indices with a 2-D FFT when
+        k1nek2 (ShiftN2D, :1191-1211), else the diagonal only (ShiftN1D, :1169-1189)."""
+        dk = -e0 * (self.qwp.Edc + Eac) / hbar * dt
+        Y = self.dcY
+        if k1nek2:
+            F = np.fft.fft2(X)
+            F = F * np.exp(-ii * (Y[:, None] + Y[None, :]) * dk)
+            X[:, :] = np.fft.ifft2(F)
+        else:
+            n = np.fft.fft(np.diag(X).copy())
+            n = n * np.exp(-ii * Y * dk)
+            np.fill_diagonal(X, np.fft.ifft(n))
+
+    # --- spontaneous emission  PSTD3D/emission.py:36-200, 230-260 ---------------
+    def _init_emission(self):
+        """InitializeEmission (emission.py:36-91) with CalcHOmega (:230-260): the
+        photon-energy midpoint grid HOmega = (n + 1/2) dhw, n < ceil(4 (kB T + hbar geh) / dhw),
+        dhw = min(kB T, hbar geh) / 20, and the line weights `square`."""
+        Nk = self.Nk
+        self.idel = np.ones((Nk, Nk))
+        np.fill_diagonal(self.idel, 0.0)
+        kBT, hg = kB * TEMP, hbar * self.gam_eh
+        hwmax = (kBT + hg) * 4.0
+        dhw = min(kBT, hg) / 20.0
+        Nw = int(np.ceil(hwmax / dhw))
+        if Nw < 10:
+            raise ValueError("Error: temperature is too low in emission.py")
+        self.HOmega = (np.arange(Nw) + 0.5) * dhw
+        dcv = abs(self.dcv)
+        lrtz = (hg / pi) / (self.HOmega**2 + hg**2)
+        self.square = (3.0 * dcv**2 * self.ehint / eps0 / np.sqrt(self.epsr) * self.ehint / hbar * lrtz
+                       * np.exp(-self.HOmega / kB / TEMP))
+
+    def Ec(self, ne, nh, VC):
+        """Coulomb energy correction (emission.py:134-175):
+        Ec[k] = sum_i ne_i (Vee[k,k] - Vee[i,k]) + nh_i (Vhh[k,k] - Vhh[i,k])
+                + sum_{i != k} (ne_i - nh_i) Veh[k,k] - Veh[k,k]."""
+        Veh, Vee, Vhh = VC[:, :, 0], VC[:, :, 1], VC[:, :, 2]
+        vd = np.diag(Veh)
+        return (ne @ (np.diag(Vee)[None, :] - Vee) + nh @ (np.diag(Vhh)[None, :] - Vhh)
+                + (ne @ self.idel) * vd - (nh @ self.idel) * vd - vd)
+
+    def SpontIntegral(self, Ek):
+        """sum_n (HOmega_n + Ek) rho0(HOmega_n + Ek) square_n dhw, rho0(x) = x^2 / (c^3 pi^2 hbar^3)
+        (emission.py:178-227), for an array of Ek."""
+        dhw = self.HOmega[1] - self.HOmega[0]
+        x = self.HOmega[None, :] + np.asarray(Ek, dtype=np.float64)[:, None]
+        return np.sum(x * (x**2 / (c0**3 * pi**2 * hbar**3)) * self.square[None, :], axis=1) * dhw
+
+    def SpontEmission(self, ne, nh, VC):
+        """Rsp[k] = SpontIntegral(gap + Ee + Eh + Ec(Re ne, Re nh, VC)) (emission.py:94-131)."""
+        Ek = self.gap + self.Ee + self.Eh + self.Ec(np.real(ne), np.real(nh), VC)
+        return self.SpontIntegral(Ek)
 
     # --- QWOptics  PSTD3D/qwoptics.py:581-669 ----------------------------
     def _init_optics(self):
@@ -761,12 +832,11 @@ class OracleWire:
         return Pxx, Pyy, Pzz, RhoE, RhoH
 
     # --- one SBE step  PSTD3D/SBEs.py:5624-5832 -----------------------------
-    def sbe_calculator(self, Ex, Ey, Ez, dt, w):
+    def _prep(self, C2, D2, p2, Ex, Ey, Ez, w):
+        """_preparation (PSTD3D/SBEs.py:5185-5238) of the state (p2, C2, D2): screened V, E1D,
+        the Hamiltonians, dephasing rates, off-diagonal dephasing and (Recomb) Rsp."""
         mb = self.mb
         Nk = self.Nk
-        st = self.state[w - 1]
-        p1, p2, C1, C2, D1, D2 = st.p1, st.p2, st.C1, st.C2, st.D1, st.D2
-        # _preparation  PSTD3D/SBEs.py:5185-5238
         ne = np.diag(C2).copy()
         nh = np.diag(D2).copy()
         Meh = self.calc_Meh(Ex, Ey, Ez, w) if mb.Optics else np.zeros((Nk, Nk), complex)
@@ -780,12 +850,54 @@ class OracleWire:
         OffP = np.zeros((Nk, Nk), complex)
         if mb.OffDiagDph:
             OffP, _, _ = self.OffDiagDephasing2(ne, nh, p2, VC)
-        dp = self.dpdt(C2, D2, p2, Heh, Hee, Hhh, GamE, GamH, OffP)
-        dC = self.dCdt(C2, D2, p2, Heh, Hee, Hhh)
-        dD = self.dDdt(C2, D2, p2, Heh, Hee, Hhh)
-        C3 = C1 + dC * dt * 2.0
-        D3 = D1 + dD * dt * 2.0
-        p3 = p1 + dp * dt * 2.0
+        Rsp = self.SpontEmission(ne, nh, VC) if mb.Recomb else np.zeros(Nk)
+        return VC, E1D, (Heh, Hee, Hhh, GamE, GamH, OffP), Rsp
+
+    def _rhs(self, p, C, D, Ex, Ey, Ez, w):
+        """(dp/dt, dC/dt, dD/dt) of a state, with the Hamiltonian and dephasing of that state."""
+        _, _, (Heh, Hee, Hhh, GamE, GamH, OffP), _ = self._prep(C, D, p, Ex, Ey, Ez, w)
+        return (self.dpdt(C, D, p, Heh, Hee, Hhh, GamE, GamH, OffP), self.dCdt(C, D, p, Heh, Hee, Hhh),
+                self.dDdt(C, D, p, Heh, Hee, Hhh))
+
+    def _rk4_coherent(self, st, Ex, Ey, Ez, dt, w):
+        """One classic RK4 step of dX/dt = RHS(X), X = (p, C, D), from the cur state, the field
+        held over the step, with the stage order of _rk4 (libpulsesuite/integrator.py:143-157):
+          ytmp = y + (h/2) dydt ; dyt = D(ytmp) ; ytmp = y + (h/2) dyt ; dym = D(ytmp)
+          ytmp = y + h dym ; dym = dym + dyt ; dyt = D(ytmp) ; y + (h/6) (dydt + dyt + 2 dym)."""
+        h = dt
+        y = (st.p2, st.C2, st.D2)
+        f = lambda x: self._rhs(x[0], x[1], x[2], Ex, Ey, Ez, w)  # noqa: E731
+        dydt = f(y)
+        ytmp = tuple(a + (h / 2.0) * b for a, b in zip(y, dydt))
+        dyt = f(ytmp)
+        ytmp = tuple(a + (h / 2.0) * b for a, b in zip(y, dyt))
+        dym = f(ytmp)
+        ytmp = tuple(a + h * b for a, b in zip(y, dym))
+        dym = tuple(a + b for a, b in zip(dym, dyt))
+        dyt = f(ytmp)
+        return tuple(a + (h / 6.0) * ((b + c) + 2.0 * d) for a, b, c, d in zip(y, dydt, dyt, dym))
+
+    def sbe_calculator(self, Ex, Ey, Ez, dt, w):
+        mb = self.mb
+        Nk = self.Nk
+        st = self.state[w - 1]
+        p1, p2, C1, C2, D1, D2 = st.p1, st.p2, st.C1, st.C2, st.D1, st.D2
+        if self.integrator == "rk4":
+            # the coherent RK4 update, then the relaxation, transport and renormalisation of
+            # the leapfrog step with the rates of the updated state (an operator split)
+            p3, C3, D3 = self._rk4_coherent(st, Ex, Ey, Ez, dt, w)
+            VC, E1D, _, Rsp = self._prep(C3, D3, p3, Ex, Ey, Ez, w)
+            p1, C1, D1 = p3, C3, D3
+        else:
+            VC, E1D, (Heh, Hee, Hhh, GamE, GamH, OffP), Rsp = self._prep(C2, D2, p2, Ex, Ey, Ez, w)
+            dp = self.dpdt(C2, D2, p2, Heh, Hee, Hhh, GamE, GamH, OffP)
+            dC = self.dCdt(C2, D2, p2, Heh, Hee, Hhh)
+            dD = self.dDdt(C2, D2, p2, Heh, Hee, Hhh)
+            C3 = C1 + dC * dt * 2.0
+            D3 = D1 + dD * dt * 2.0
+            p3 = p1 + dp * dt * 2.0
+        self.last_Rsp = Rsp  # emission rates of the step, for tests
+        C3, D3, p3 = C3.copy(), D3.copy(), p3.copy()
         if mb.EHs or mb.Phonon:  # _relaxation  PSTD3D/SBEs.py:5288-5337
             ne3 = np.diag(C3).copy()
             nh3 = np.diag(D3).copy()
@@ -804,8 +916,15 @@ class OracleWire:
             Wh = WinH + WoutH
             ne3 = np.exp(-We * dt) * ((-1 + np.exp(We * dt) + ne3) * WinE + ne3 * WoutE) / (We + self.small)
             nh3 = np.exp(-Wh * dt) * ((-1 + np.exp(Wh * dt) + nh3) * WinH + nh3 * WoutH) / (Wh + self.small)
+            if mb.Recomb:  # PSTD3D/SBEs.py:5335-5337 (nh decays with the updated ne)
+                ne3 = ne3 * np.exp(-Rsp * nh3 * dt)
+                nh3 = nh3 * np.exp(-Rsp * ne3 * dt)
             C3[np.arange(Nk), np.arange(Nk)] = ne3
             D3[np.arange(Nk), np.arange(Nk)] = nh3
+        if mb.DCTrans:  # PSTD3D/SBEs.py:5698-5702: Eac = Re Ex(q = 0) of the wire's field
+            Eac = np.real(Ex[self.NQ0])
+            for X in (C3, D3, p3):
+                self.Transport(X, Eac, dt, mb.LF)
         ne3 = np.abs(np.diag(C3)).astype(complex)
         nh3 = np.abs(np.diag(D3)).astype(complex)
         total = (np.sum(np.abs(ne3)) + np.sum(np.abs(nh3))) / 2.0

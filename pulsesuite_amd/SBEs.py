@@ -166,6 +166,9 @@ class DeviceWires:
         for k in ("L", "gap", "ehint", "area", "dcv", "dkr", "dr_r", "drr", "gam_e", "gam_h", "gam_eh", "Vscale",
                   "N0", "hOph", "hGph", "Gph", "density_max"):
             setattr(d, k, float(s[k]))
+        d.Edc = float(s.get("Edc", 0.0))
+        for j, a in enumerate(s.get("rsp", (0.0,) * 4)):
+            d.rsp[j] = float(a)
         d.yw = float((-1) ** int(np.floor((w - 1) / 2.0)))
         d.zsign = float((-1) ** w)
         d.gate = self.gate
@@ -221,7 +224,7 @@ class DeviceWires:
         sub = self.models[lo:hi]
         _lib.check(_lib.lib.pss_sbe_step_p(ptr, hi - lo, min(m.Nk for m in sub), max(m.Nk for m in sub),
                                            max(m.Nr for m in sub),
-                                           max(self.nrr[lo:hi]), precision_mask(self.prec_code), ksplit_cap,
+                                           max(self.nrr[lo:hi]), precision_mask(self.prec_code, sub), ksplit_cap,
                                            _lib.stream_ptr(stream)), "pss_sbe_step_p")
 
     def kernel(self, name, *extra, wires=None, stream=None):
@@ -291,14 +294,25 @@ class DeviceWires:
         lay = ws_layout(Nk, Nr, nrr)
         sizes = {"vc": 3 * Nk, "e1d": Nk, "gam": 2 * Nk, "De": (2 * Nk + 1) * Nk, "Dh": (2 * Nk + 1) * Nk,
                  "H": 3 * Nk * Nk * 2, "W": 4 * Nk, "n3": 2 * Nk, "eq": 3 * (2 * Nk - 1) * 2, "pr": 3 * Nr,
-                 "rho": 2 * Nr * 2, "er": 3 * Nr}
+                 "rho": 2 * Nr * 2, "er": 3 * Nr, "rsp": Nk, "dcc": 8 * Nk}
         o = lay[name]
         return self.ws[i][o:o + sizes[name]]
 
 
-def precision_mask(prec_code):
-    """pss_sbe_step_p's mask for a batch of wires of one precision."""
-    return 2 if prec_code == _lib.MIXED else 1
+def precision_mask(prec_code, models=()):
+    """pss_sbe_step_p's mask for a batch of wires of one precision: 1 FP64 or 2 mixed, plus 4
+    when a wire has DCTrans and 8 when one has Recomb (those terms' kernels run only then)."""
+    return (2 if prec_code == _lib.MIXED else 1) | feature_bits(models)
+
+
+def feature_bits(models):
+    f = 0
+    for m in models:
+        if m.mb.DCTrans:
+            f |= 4
+        if m.mb.Recomb:
+            f |= 8
+    return f
 
 
 def ws_layout(Nk, Nr, nrr):
@@ -310,7 +324,7 @@ def ws_layout(Nk, Nr, nrr):
             ("nh2", 2 * Nk), ("tr", 4 * nd * 2), ("gam", 2 * Nk), ("De", (2 * Nk + 1) * Nk),
             ("Dh", (2 * Nk + 1) * Nk), ("H", 3 * Nk * Nk * 2), ("W", 4 * Nk), ("n3", 4 * Nk), ("red", 32),
             ("ds", 4 * nd * 2), ("pr", 3 * Nr), ("rho", 2 * Nr * 2), ("spb", 5 * Nr), ("outr", 5 * nrr),
-            ("cp", 2 * Nk * Nk)]
+            ("cp", 2 * Nk * Nk), ("dcc", 8 * Nk), ("rsp", Nk)]
     np64 = (Nk + 63) // 64 * 64
     ksm = 16 if np64 <= 128 else 8 if np64 <= 256 else 4 if np64 <= 512 else 1
     out = {}

@@ -19,9 +19,11 @@ EXCITONS = 1 << 1
 EHS = 1 << 2
 SCREENED = 1 << 3
 PHONON = 1 << 4
+DCTRANS = 1 << 5
 LF = 1 << 6
 DIAGDPH = 1 << 8
 OFFDIAGDPH = 1 << 9
+RECOMB = 1 << 10
 IGNOREWIRE = 1 << 12
 LORENTZDELTA = 1 << 14
 
@@ -29,7 +31,7 @@ FP64 = 0
 MIXED = 1
 
 # ABI version the signatures below describe (pss_version() of include/pss.h)
-ABI_VERSION = "pss 0.2"
+ABI_VERSION = "pss 0.3"
 
 _vp = C.c_void_p
 _d = C.c_double
@@ -42,7 +44,7 @@ class WireDesc(C.Structure):
         + [(n, _d) for n in ("dt", "L", "gap", "ehint", "area", "dcv", "dkr", "dr_r", "drr", "yw", "zsign",
                              "gam_e", "gam_h", "gam_eh", "Vscale", "N0", "hOph", "hGph", "Gph",
                              "density_max", "gate")]
-        + [("pad2", _d * 3)]
+        + [("Edc", _d), ("rsp", _d * 4), ("pad2", _d * 3)]
         + [(n, _vp) for n in ("kr", "Ee", "Eh", "win", "r", "V0", "chie", "chih", "qe", "qh", "ktab", "xe", "xh",
                               "iqv", "twid", "sa_x", "sa_t", "sa_b", "sa_h", "sa_end", "sa_idx", "sa_dx",
                               "sb_t", "sb_b", "sb_h", "sb_end", "sb_idx", "sb_dx")]

@@ -17,4 +17,4 @@ int pss_set_msg(int code, const char* msg) {
 }
 
 extern "C" const char* pss_last_error(void) { return g_err; }
-extern "C" const char* pss_version(void) { return "pss 0.2 (gfx950, fp64 + mixed)"; }
+extern "C" const char* pss_version(void) { return "pss 0.3 (gfx950, fp64 + mixed)"; }

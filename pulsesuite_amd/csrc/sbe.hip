@@ -33,8 +33,8 @@ __device__ inline double ddiv(double n, double d) {
 
 // ---------------------------------------------------------------- workspace
 struct WS {
-  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, sp, pk,
-      xp, total_fp64, total;
+  int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, dcc, rsp, sp,
+      pk, xp, total_fp64, total;
 };
 
 __host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
@@ -64,6 +64,8 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.spb = o;  o += al(5 * (int64_t)Nr);
   w.outr = o; o += al(5 * (int64_t)nrr);
   w.cp = o;   o += al(2 * (int64_t)Nk * Nk);
+  w.dcc = o;  o += al(8 * (int64_t)Nk);   // DCTrans: circulant column (complex [Nk]), shifted diagonals [3][Nk]
+  w.rsp = o;  o += al(Nk);                // Recomb: Rsp [Nk]
   const int64_t np = (Nk + 63) / 64 * 64;
   const int ksm = np <= 128 ? 16 : np <= 256 ? 8 : np <= 512 ? 4 : 1;
   w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);  // split-K partial tiles
@@ -835,6 +837,15 @@ __device__ inline cplx load_op(const cplx* X, int op, int i, int k, int Nk, cons
 
 // FP64 leapfrog epilogue of one element with the products' sum S (the Hee / Hhh
 // diagonals are inside the products): + Heh^T - i hbar gamma p for dp.
+// Whether the RHS epilogue writes the averaged prev entry (X1 + X3) / 2: not on the C, D
+// diagonals (written after the relaxation), and with DCTrans not where the transport shift
+// changes X3 after the epilogue (every entry with PSS_LF, else the diagonal of p).
+__device__ inline bool avg_in_epilogue(const pss_wire_desc& d, int out, int i, int j) {
+  if (i == j && out != 0) return false;
+  if (d.flags & PSS_DCTRANS) return !(d.flags & PSS_LF) && i != j;
+  return true;
+}
+
 __device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, const cplx* H, const double* gam,
                                      int out, int i, int j, cplx S, int deriv) {
   const int Nk = d.Nk;
@@ -857,7 +868,7 @@ __device__ inline void fp64_epilogue(const pss_wire_desc& d, const Bufs& B, cons
   const cplx x1 = X1[o];
   const cplx x3 = x1 + (R * d.dt) * 2.0;
   Xn[o] = x3;
-  if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+  if (avg_in_epilogue(d, out, i, j)) Xp[o] = (x1 + x3) * 0.5;
 }
 
 // One 64 x 64 output tile per workgroup, 4 waves (32 x 32 quadrants, 2 x 2 blocks of
@@ -1138,7 +1149,7 @@ __device__ inline void mixed_epilogue(const pss_wire_desc& d, const Bufs& B, con
   const cplx x1 = X1[o];
   const cplx x3 = x1 + (R * d.dt) * 2.0;
   Xn[o] = x3;
-  if (!(out != 0 && i == j)) Xp[o] = (x1 + x3) * 0.5;
+  if (avg_in_epilogue(d, out, i, j)) Xp[o] = (x1 + x3) * 0.5;
 }
 
 // ---- contraction: one 64 x 64 output tile per workgroup, 4 waves (32 x 32 each,
@@ -1805,30 +1816,235 @@ __global__ void __launch_bounds__(256) k_wsum(const pss_wire_desc* __restrict__ 
   }
 }
 
+// ------------------------------------------ Recomb: spontaneous emission rates
+// Rsp[k] = SpontIntegral(gap + Ee[k] + Eh[k] + Ec[k]) (PSTD3D/emission.py:94-131) from the
+// cur populations and the screened V of the step (the reference's _preparation order), with
+//   Ec[k] = sum_i ne_i (Vee(0) - Vee|i-k|) + nh_i (Vhh(0) - Vhh|i-k|)
+//           + sum_{i != k} (ne_i - nh_i) Veh(0) - Veh(0)           (emission.py:134-175)
+// and the photon-energy sum sum_n (h_n + E)^3 w_n of SpontIntegral as the cubic
+// rsp[0] + rsp[1] E + rsp[2] E^2 + rsp[3] E^3 of its moments (all terms positive, so the
+// two forms agree to rounding).
+__global__ void __launch_bounds__(256) k_rsp(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d) || !(d.flags & PSS_RECOMB)) return;
+  const int Nk = d.Nk;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const double* vc = ws + L.vc;
+  const double veh0 = vc[0], vee0 = vc[Nk], vhh0 = vc[2 * Nk];
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = 0; i < Nk; ++i) {
+    const int q = abs(i - k);
+    const double ne = ws[L.ne2 + 2 * i], nh = ws[L.nh2 + 2 * i];
+    s1 += ne * (vee0 - vc[Nk + q]) + nh * (vhh0 - vc[2 * Nk + q]);
+    if (i != k) s2 += ne * veh0 - nh * veh0;
+  }
+  const double E = d.gap + d.Ee[k] + d.Eh[k] + ((s1 + s2) - veh0);
+  ws[L.rsp + k] = ((d.rsp[3] * E + d.rsp[2]) * E + d.rsp[1]) * E + d.rsp[0];
+}
+
+__device__ inline cplx cexp_(cplx z) {
+  const double m = exp(z.x);
+  double sn, cs;
+  sincos(z.y, &sn, &cs);
+  return mk(m * cs, m * sn);
+}
+
+// Exponential relaxation of one population pair (PSTD3D/SBEs.py:5321-5333) and, with
+// Recomb, the recombination decay ne *= exp(-Rsp nh dt), nh *= exp(-Rsp ne dt) with the
+// updated ne (:5335-5337).
+__device__ inline void relax_pair(const pss_wire_desc& d, const double* ws, const WS& L, int k, cplx& ne, cplx& nh) {
+  const int Nk = d.Nk;
+  const double dt = d.dt;
+  const double wi = ws[L.W + k], wo = ws[L.W + Nk + k];
+  const double hi = ws[L.W + 2 * Nk + k], ho = ws[L.W + 3 * Nk + k];
+  const double We = wi + wo, Wh = hi + ho;
+  const double ee = exp(-We * dt), eh = exp(-Wh * dt);
+  const double Ee1 = -1 + exp(We * dt), Eh1 = -1 + exp(Wh * dt);
+  ne = ee * ((mk(Ee1, 0) + ne) * wi + ne * wo);
+  ne = mk(ne.x / (We + 1e-200), ne.y / (We + 1e-200));
+  nh = (eh * ((mk(Eh1, 0) + nh) * hi + nh * ho));
+  nh = mk(nh.x / (Wh + 1e-200), nh.y / (Wh + 1e-200));
+  if (d.flags & PSS_RECOMB) {
+    const double r = ws[L.rsp + k];
+    ne = ne * cexp_(mk(-r * nh.x * dt, -r * nh.y * dt));
+    nh = nh * cexp_(mk(-r * ne.x * dt, -r * ne.y * dt));
+  }
+}
+
+// ------------------------------------------------------------- DCTrans
+// Transport (PSTD3D/dcfield.py:1169-1247) after the relaxation, before the renormalisation
+// (PSTD3D/SBEs.py:5698-5702).  The reference multiplies the FFT of X by exp(-i Y_j dk),
+// Y_j = s_j 2 pi / ((Nk - 1) dky) (s_j the FFT-order integer of index j), and transforms
+// back: X -> S X S^T (both indices, ShiftN2D, when PSS_LF) or n -> S n on the diagonal
+// (ShiftN1D), with the circulant S[a][m] = c[(a - m) mod Nk],
+//   c[dd] = (1/Nk) sum_s exp(i s 2 pi u / Nk),  u = dd - sigma,  sigma = Nk dk / ((Nk-1) dky),
+// s over the FFT-order integers: the Dirichlet kernel
+//   c[dd] = (1/Nk) exp(i pi u h / Nk) sin(pi u) / sin(pi u / Nk),  h = 1 (Nk even), 0 (odd).
+// sigma is the shift in grid points; an integer sigma is an exact circular roll.
+
+// K8a (DCTrans wires): relaxation of the next diagonals, before their shift.
+__global__ void __launch_bounds__(256) k_dc_relax(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.x];
+  if (!wire_on(d) || !(d.flags & PSS_DCTRANS) || !(d.flags & (PSS_EHS | PSS_PHONON))) return;
+  const int Nk = d.Nk;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  const Bufs B = bufs(d);
+  for (int k = threadIdx.x; k < Nk; k += 256) {
+    const int64_t o = (int64_t)k * Nk + k;
+    cplx ne = B.Cn[o], nh = B.Dn[o];
+    relax_pair(d, d.ws, L, k, ne, nh);
+    B.Cn[o] = ne;
+    B.Dn[o] = nh;
+  }
+}
+
+// The circulant column c[dd] of the step's shift (Eac = Re Ex at q = 0 of the centred DFT,
+// the field line's index NQ0 = od Nk - 1 of the workspace copy).
+__global__ void __launch_bounds__(256) k_dc_coef(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d) || !(d.flags & PSS_DCTRANS)) return;
+  const int Nk = d.Nk;
+  const int dd = blockIdx.x * 256 + threadIdx.x;
+  if (dd >= Nk) return;
+  const WS L = ws_layout(Nk, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const double Eac = ws[L.eq + (int64_t)(Nk - 1) * 2];
+  const double dk = -kE0 * (d.Edc + Eac) / kHbar * d.dt;
+  const double dky = Nk > 2 ? d.kr[2] - d.kr[1] : d.kr[1] - d.kr[0];
+  const double sigma = (double)Nk * dk / ((double)(Nk - 1) * dky);
+  const double u = (double)dd - sigma;
+  const double v = u / (double)Nk;
+  const double den = sinpi(v);
+  double R;
+  if (den != 0.0) {
+    R = sinpi(u) / den;
+  } else {  // u = m Nk: the kernel's limit Nk (-1)^(m (Nk - 1))
+    const long long m = llrint(v);
+    R = ((m * (Nk - 1)) & 1) ? -(double)Nk : (double)Nk;
+  }
+  const double ph = (Nk & 1) ? 0.0 : v;
+  double sn, cs;
+  sincospi(ph, &sn, &cs);
+  ws[L.dcc + 2 * dd] = cs * R / (double)Nk;
+  ws[L.dcc + 2 * dd + 1] = sn * R / (double)Nk;
+}
+
+// The two products of X3 -> S X3 S^T (PSS_LF wires), 64 x 64 FP64 complex output tiles of
+// 256 threads (4 x 4 per thread), k chunks of 16 staged in LDS.  PASS 0: T = S X3 into the
+// Hamiltonian segment of the workspace (free after the RHS); PASS 1: X3' = T S^T into the
+// next buffer and the averaged prev (X1 + X3') / 2 (not on the C, D diagonals, which the
+// renormalisation writes).
+template <int PASS>
+__global__ void __launch_bounds__(256) k_dc_gemm(const pss_wire_desc* __restrict__ descs) {
+  const int mat = blockIdx.z % 3, wire = blockIdx.z / 3;
+  const pss_wire_desc& d = descs[wire];
+  if (!wire_on(d) || !(d.flags & PSS_DCTRANS) || !(d.flags & PSS_LF)) return;
+  const int N = d.Nk;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  if (m0 >= N || n0 >= N) return;
+  const WS L = ws_layout(N, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const cplx* c = (const cplx*)(ws + L.dcc);
+  cplx* T = (cplx*)(ws + L.H) + (int64_t)mat * N * N;
+  cplx* Xn = mat == 0 ? B.pn : mat == 1 ? B.Cn : B.Dn;
+  __shared__ cplx As[64][17];
+  __shared__ cplx Bs[16][65];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  cplx acc[4][4];
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) acc[p][q] = mk(0.0, 0.0);
+  for (int k0 = 0; k0 < N; k0 += 16) {
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      const int r = e >> 4, kk = e & 15;
+      const int i = m0 + r, k = k0 + kk;
+      cplx v = mk(0.0, 0.0);
+      if (i < N && k < N) v = PASS == 0 ? c[i >= k ? i - k : i - k + N] : T[(int64_t)i * N + k];
+      As[r][kk] = v;
+    }
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      const int kk = e >> 6, col = e & 63;
+      const int k = k0 + kk, j = n0 + col;
+      cplx v = mk(0.0, 0.0);
+      if (k < N && j < N) v = PASS == 0 ? Xn[(int64_t)k * N + j] : c[j >= k ? j - k : j - k + N];
+      Bs[kk][col] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      cplx a[4], b[4];
+      for (int p = 0; p < 4; ++p) a[p] = As[ty + 16 * p][kk];
+      for (int q = 0; q < 4; ++q) b[q] = Bs[kk][tx + 16 * q];
+      for (int p = 0; p < 4; ++p)
+        for (int q = 0; q < 4; ++q) {
+          acc[p][q].x = fma(a[p].x, b[q].x, fma(-a[p].y, b[q].y, acc[p][q].x));
+          acc[p][q].y = fma(a[p].x, b[q].y, fma(a[p].y, b[q].x, acc[p][q].y));
+        }
+    }
+    __syncthreads();
+  }
+  const cplx* X1 = mat == 0 ? B.p1 : mat == 1 ? B.C1 : B.D1;
+  cplx* Xp = mat == 0 ? B.pp : mat == 1 ? B.Cp : B.Dp;
+  for (int p = 0; p < 4; ++p)
+    for (int q = 0; q < 4; ++q) {
+      const int i = m0 + ty + 16 * p, j = n0 + tx + 16 * q;
+      if (i >= N || j >= N) continue;
+      const int64_t o = (int64_t)i * N + j;
+      if (PASS == 0) {
+        T[o] = acc[p][q];
+      } else {
+        Xn[o] = acc[p][q];
+        if (!(mat != 0 && i == j)) Xp[o] = (X1[o] + acc[p][q]) * 0.5;
+      }
+    }
+}
+
+// Diagonal shift n -> S n of p3, C3, D3 (DCTrans wires without PSS_LF), one workgroup per
+// matrix, the diagonal staged in LDS; p's averaged prev diagonal follows (the C, D ones come
+// with the renormalisation).
+__global__ void __launch_bounds__(256) k_dc_diag(const pss_wire_desc* __restrict__ descs) {
+  const pss_wire_desc& d = descs[blockIdx.y];
+  if (!wire_on(d) || !(d.flags & PSS_DCTRANS) || (d.flags & PSS_LF)) return;
+  const int N = d.Nk, mat = blockIdx.x;
+  const WS L = ws_layout(N, d.Nr, d.nrr);
+  double* ws = d.ws;
+  const Bufs B = bufs(d);
+  const cplx* c = (const cplx*)(ws + L.dcc);
+  cplx* Xn = mat == 0 ? B.pn : mat == 1 ? B.Cn : B.Dn;
+  extern __shared__ double dsh[];
+  cplx* n = (cplx*)dsh;
+  for (int a = threadIdx.x; a < N; a += 256) n[a] = Xn[(int64_t)a * (N + 1)];
+  __syncthreads();
+  for (int a = threadIdx.x; a < N; a += 256) {
+    cplx sacc = mk(0.0, 0.0);
+    for (int m = 0; m < N; ++m) {
+      const cplx cv = c[a >= m ? a - m : a - m + N];
+      sacc.x = fma(cv.x, n[m].x, fma(-cv.y, n[m].y, sacc.x));
+      sacc.y = fma(cv.x, n[m].y, fma(cv.y, n[m].x, sacc.y));
+    }
+    const int64_t o = (int64_t)a * (N + 1);
+    Xn[o] = sacc;
+    if (mat == 0) B.pp[o] = (B.p1[o] + sacc) * 0.5;
+  }
+}
+
 // --------------------- K8: exponential relaxation + renormalisation of n3
-// PSTD3D/SBEs.py:5321-5333 and 5704-5718 (diagonals of C3, D3 and of the averaged prev).
+// PSTD3D/SBEs.py:5321-5333 and 5704-5718 (diagonals of C3, D3 and of the averaged prev);
+// DCTrans wires were relaxed before their shift (k_dc_relax).
 __device__ void pops_body(const pss_wire_desc& d, double* sh) {
   const int Nk = d.Nk;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
   double* ws = d.ws;
   const Bufs B = bufs(d);
-  const bool relax = d.flags & (PSS_EHS | PSS_PHONON);
-  const double dt = d.dt;
+  const bool relax = (d.flags & (PSS_EHS | PSS_PHONON)) && !(d.flags & PSS_DCTRANS);
   double se = 0.0, sh_ = 0.0;
   for (int k = threadIdx.x; k < Nk; k += 256) {
     const int64_t o = (int64_t)k * Nk + k;
     cplx ne = B.Cn[o], nh = B.Dn[o];
-    if (relax) {
-      const double wi = ws[L.W + k], wo = ws[L.W + Nk + k];
-      const double hi = ws[L.W + 2 * Nk + k], ho = ws[L.W + 3 * Nk + k];
-      const double We = wi + wo, Wh = hi + ho;
-      const double ee = exp(-We * dt), eh = exp(-Wh * dt);
-      const double Ee1 = -1 + exp(We * dt), Eh1 = -1 + exp(Wh * dt);
-      ne = ee * ((mk(Ee1, 0) + ne) * wi + ne * wo);
-      ne = mk(ne.x / (We + 1e-200), ne.y / (We + 1e-200));
-      nh = (eh * ((mk(Eh1, 0) + nh) * hi + nh * ho));
-      nh = mk(nh.x / (Wh + 1e-200), nh.y / (Wh + 1e-200));
-    }
+    if (relax) relax_pair(d, ws, L, k, ne, nh);
     const double ae = cabs(ne), ah = cabs(nh);
     ws[L.n3 + k] = ae;
     ws[L.n3 + Nk + k] = ah;
@@ -2004,8 +2220,9 @@ __device__ void rho_spline_body(const pss_wire_desc& d, double* sh) {
 }
 
 // Evaluate on rr, |.| and final normalisation of rho, Rho = RhoH - RhoE, zeros when
-// gated off or ignorewire; advance the step counter.
-__device__ void qw2prop_body(const pss_wire_desc& d, const bool on, double* sh) {
+// gated off or ignorewire, NaN when the launch left out a term the wire's flags ask for;
+// advance the step counter.
+__device__ void qw2prop_body(const pss_wire_desc& d, const bool on, const bool nan_out, double* sh) {
   constexpr int NT = 320;
   const int n = d.nrr;
   const bool zero = !on || (d.flags & PSS_IGNOREWIRE);
@@ -2044,6 +2261,14 @@ __device__ void qw2prop_body(const pss_wire_desc& d, const bool on, double* sh) 
   const double fe = lf ? total / (se * d.drr + 1e-100) : 0.0;
   const double fh = lf ? total / (sh_ * d.drr + 1e-100) : 0.0;
   for (int i = threadIdx.x; i < n; i += NT) {
+    if (nan_out) {
+      const double q = __builtin_nan("");
+      d.Pxx[2 * i] = q; d.Pxx[2 * i + 1] = q;
+      d.Pyy[2 * i] = q; d.Pyy[2 * i + 1] = q;
+      d.Pzz[2 * i] = q; d.Pzz[2 * i + 1] = q;
+      d.Rho[2 * i] = q; d.Rho[2 * i + 1] = q;
+      continue;
+    }
     if (zero) {
       d.Pxx[2 * i] = 0; d.Pxx[2 * i + 1] = 0;
       d.Pyy[2 * i] = 0; d.Pyy[2 * i + 1] = 0;
@@ -2063,14 +2288,16 @@ __device__ void qw2prop_body(const pss_wire_desc& d, const bool on, double* sh) 
 }
 
 // The rho spline (K10) and the QW2Prop evaluation on the propagation grid in one
-// workgroup per wire; a wire that is off (or ignorewire) writes zeros.
-__global__ void __launch_bounds__(320) k_rho_qw2prop(const pss_wire_desc* __restrict__ descs) {
+// workgroup per wire; a wire that is off (or ignorewire) writes zeros.  feat: the optional
+// terms the launch ran (4 DCTrans, 8 Recomb).
+__global__ void __launch_bounds__(320) k_rho_qw2prop(const pss_wire_desc* __restrict__ descs, int feat) {
   const pss_wire_desc& d = descs[blockIdx.x];
   __shared__ double sh[4 * 5 + 4];
   const bool on = wire_on(d);
+  const bool missing = ((d.flags & PSS_DCTRANS) && !(feat & 4)) || ((d.flags & PSS_RECOMB) && !(feat & 8));
   if (on) rho_spline_body(d, sh);
   __syncthreads();
-  qw2prop_body(d, on, sh);
+  qw2prop_body(d, on, on && missing, sh);
 }
 
 // ------------------------------------------------------------ Vint (init)
@@ -2186,8 +2413,10 @@ static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_
     snprintf(msg, sizeof(msg), "%s: need 4 <= min_Nk <= max_Nk", who);
     return pss_set_msg(2, msg);
   }
-  if (precisions < 1 || precisions > 3) {
-    snprintf(msg, sizeof(msg), "%s: precisions is a mask of 1 (FP64 wires) and 2 (mixed wires)", who);
+  if (!(precisions & 3) || precisions > 15) {
+    snprintf(msg, sizeof(msg),
+             "%s: precisions is a mask of 1 (FP64 wires), 2 (mixed wires), at least one of them, "
+             "4 (DCTrans wires), 8 (Recomb wires)", who);
     return pss_set_msg(2, msg);
   }
   if (ksplit_cap < 0 || ksplit_cap > 16 || (ksplit_cap & (ksplit_cap - 1))) {
@@ -2198,7 +2427,7 @@ static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_
     snprintf(msg, sizeof(msg), "%s: sizes out of range (4 <= Nk <= 4096, nrr >= 4)", who);
     return pss_set_msg(2, msg);
   }
-  const bool any64 = precisions & 1, anymx = precisions & 2;
+  const bool any64 = precisions & 1, anymx = precisions & 2, anydc = precisions & 4, anyrc = precisions & 8;
   const int deriv = opt & 1, pops = (opt & 2) ? 0 : 1;
   const int fmask = (opt >> 8) ? (opt >> 8) : ~0;
   const int nd = 2 * max_Nk - 1;
@@ -2265,11 +2494,20 @@ static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_
   if (stages & ST_COLL) {
     hipLaunchKernelGGL(k_coll, dim3(max_Nk, 2, nwires), dim3(256), lds_rel, s, d_descs, fmask);
     hipLaunchKernelGGL(k_wsum, dim3(max_Nk, 2, nwires), dim3(256), 0, s, d_descs, fmask);
+    if (anyrc) hipLaunchKernelGGL(k_rsp, dim3((max_Nk + 255) / 256, nwires), dim3(256), 0, s, d_descs);
+  }
+  if ((stages & ST_TAIL) && pops && anydc) {  // relaxation, then the transport shift of C3, D3, p3
+    const int t64 = (max_Nk + 63) / 64;
+    hipLaunchKernelGGL(k_dc_relax, dim3(nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_dc_coef, dim3((max_Nk + 255) / 256, nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_dc_gemm<0>, dim3(t64, t64, 3 * nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_dc_gemm<1>, dim3(t64, t64, 3 * nwires), dim3(256), 0, s, d_descs);
+    hipLaunchKernelGGL(k_dc_diag, dim3(3, nwires), dim3(256), (size_t)max_Nk * sizeof(double2), s, d_descs);
   }
   if (stages & ST_TAIL) {
     hipLaunchKernelGGL(k_diag_sums, dim3(nd, nwires), dim3(256), 0, s, d_descs, pops);
     hipLaunchKernelGGL(k_pol_rho, dim3((max_Nr + 3) / 4, nwires), dim3(256), 0, s, d_descs);
-    if (pops) hipLaunchKernelGGL(k_rho_qw2prop, dim3(nwires), dim3(320), 0, s, d_descs);
+    if (pops) hipLaunchKernelGGL(k_rho_qw2prop, dim3(nwires), dim3(320), 0, s, d_descs, precisions);
   }
   PSS_CHECK(hipGetLastError());
   return 0;
@@ -2284,7 +2522,7 @@ extern "C" int pss_sbe_step_p(const pss_wire_desc* d_descs, int32_t nwires, int3
 
 extern "C" int pss_sbe_step(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,
                             int32_t max_Nr, int32_t max_nrr, void* stream) {
-  return pss_sbe_step_p(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 3, 0, stream);
+  return pss_sbe_step_p(d_descs, nwires, min_Nk, max_Nk, max_Nr, max_nrr, 15, 0, stream);
 }
 
 extern "C" int pss_sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_Nk, int32_t max_Nk,

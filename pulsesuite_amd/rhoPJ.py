@@ -20,6 +20,8 @@ The longitudinal-field diagnostics (ElongfromRho, RhoB*) are not on the J path a
 are not computed.
 """
 
+import weakref
+
 import numpy as np
 import torch
 
@@ -80,11 +82,19 @@ class QWArray:
         self.Emax0 = Emax0
         self.x0, self.y0, self.z0, self.ay, self.az = x0, y0, z0, ay, az
         self.precision = precision
-        self.prop = None
+        self._prop = None
+
+    @property
+    def prop(self):
+        """The propagator this array is bound to (held weakly: the propagator holds the
+        array as its calc_j hook, and a strong reference back would make a cycle that only
+        the cyclic collector frees, at an arbitrary later moment, with its device buffers
+        and captured graphs)."""
+        return self._prop() if self._prop is not None else None
 
     def bind(self, prop):
         """Allocate device state for propagator ``prop`` (called by PSTD3DPropagator)."""
-        self.prop = prop
+        self._prop = weakref.ref(prop)
         sp = prop.space
         dev = prop.device
         self.rx, self.ry, self.rz = GetXArray(sp), GetYArray(sp), GetZArray(sp)

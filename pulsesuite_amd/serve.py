@@ -40,7 +40,7 @@ import numpy as np
 
 from . import _lib
 from .batch import balanced_ranges, wire_bytes
-from .SBEs import precision_mask
+from .SBEs import feature_bits, precision_mask
 from .wire_model import L_of_nk, MBFlags, QWParams
 
 # 1-D line: 1024 cells of 10 nm with 16-cell CPML.  On an Nx x 1 x 1 grid the
@@ -241,7 +241,8 @@ class Server:
             m = [s.qa.model for s in sub]
             st = _lib.stream_ptr(self._pool[b % ns] if ns > 1 else main)
             _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(base + lo * sz), hi - lo, min(x.Nk for x in m),
-                                               max(x.Nk for x in m), max(x.Nr for x in m), NX, self._pmask, 0, st),
+                                               max(x.Nk for x in m), max(x.Nr for x in m), NX,
+                                               self._pmask | feature_bits(m), 0, st),
                        "pss_sbe_step_p")
         if ns > 1:
             for j in range(ns):

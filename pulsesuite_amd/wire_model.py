@@ -187,8 +187,9 @@ class WireModel:
         m = self.mb
         f = 0
         for name, bit in (("Optics", L.OPTICS), ("Excitons", L.EXCITONS), ("EHs", L.EHS), ("Screened", L.SCREENED),
-                          ("Phonon", L.PHONON), ("LF", L.LF), ("DiagDph", L.DIAGDPH), ("OffDiagDph", L.OFFDIAGDPH),
-                          ("ignorewire", L.IGNOREWIRE), ("LorentzDelta", L.LORENTZDELTA)):
+                          ("Phonon", L.PHONON), ("DCTrans", L.DCTRANS), ("LF", L.LF), ("DiagDph", L.DIAGDPH),
+                          ("OffDiagDph", L.OFFDIAGDPH), ("Recomb", L.RECOMB), ("ignorewire", L.IGNOREWIRE),
+                          ("LorentzDelta", L.LORENTZDELTA)):
             if getattr(m, name):
                 f |= bit
         return f
@@ -216,8 +217,8 @@ def build_model(qw, mb, rr, r0=0.0):
 def _build_model(qw, mb, rr, r0=0.0):
     M = WireModel(qw=qw, mb=MBFlags(**vars(mb)))
     mb = M.mb
-    if mb.DCTrans or mb.Recomb or mb.FreePot:
-        raise NotImplementedError("DCTrans, Recomb and FreePot are not on the device path")
+    # FreePot adds the monopole couplings Wee, Whh to Hee, Hhh (PSTD3D/SBEs.py:848-853), which the
+    # reference's _preparation leaves at zero (their CalcWnn is not called, :5207-5209): no term.
     L = qw.L
     gap = qw.gap_eV * e0
     me, mh = qw.me_rel * me0, qw.mh_rel * me0
@@ -301,8 +302,31 @@ def _build_model(qw, mb, rr, r0=0.0):
                      Vscale=Vscale, N0=N0, hOph=hbar * Oph, hGph=hbar * Gph, Gph=Gph,
                      density_max=min(ae / 2.0, ah / 2.0) / pi * 0.99, Delta0=qw.Delta0, epsr=qw.epsr,
                      vpref=e0**2 / (twopi * eps0 * qw.epsr * L), Oph=Oph,
-                     dr=(r[2] - r[1]) * (Nr - 1) / float(Nr), qc=2.0 * ae * ah / (ae + ah))
+                     dr=(r[2] - r[1]) * (Nr - 1) / float(Nr), qc=2.0 * ae * ah / (ae + ah),
+                     Edc=float(qw.Edc), rsp=rsp_poly(gam_eh, dcv, ehint, qw.epsr) if mb.Recomb else (0.0,) * 4)
     return M
+
+
+def rsp_poly(gam_eh, dcv, ehint, epsr):
+    """Coefficients a of SpontIntegral(E) = a0 + a1 E + a2 E^2 + a3 E^3 (PSTD3D/emission.py:178-227):
+    the sum over the photon-energy grid h_n = (n + 1/2) dhw of CalcHOmega (:230-260) of
+    (h_n + E)^3 / (c^3 pi^2 hbar^3) square_n dhw, square of InitializeEmission (:36-91),
+    expanded in E (its moments; every term is positive, so the cubic and the direct sum agree
+    to rounding)."""
+    from scipy.constants import c as c0
+
+    kBT, hg = kB * TEMP, hbar * gam_eh
+    hwmax = (kBT + hg) * 4.0
+    dhw = min(kBT, hg) / 20.0
+    Nw = int(np.ceil(hwmax / dhw))
+    if Nw < 10:
+        raise ValueError("Error: temperature is too low in emission.py")
+    H = (np.arange(Nw) + 0.5) * dhw
+    dcv = abs(dcv)
+    square = (3.0 * dcv**2 * ehint / eps0 / np.sqrt(epsr) * ehint / hbar * ((hg / pi) / (H**2 + hg**2))
+              * np.exp(-H / kB / TEMP))
+    w = square * dhw / (c0**3 * pi**2 * hbar**3)
+    return (float(np.sum(H**3 * w)), float(3.0 * np.sum(H**2 * w)), float(3.0 * np.sum(H * w)), float(np.sum(w)))
 
 
 def explicit_model(ky, Ee, Eh, mb, *, y=None, L=None, Delta0=5e-9, me=0.07 * me0, mh=0.45 * me0, ge=1e12, gh=1e12,

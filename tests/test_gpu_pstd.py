@@ -818,7 +818,9 @@ def test_mask_boundary_coupled_matches_oracle(polarization):
 def test_mask_closed_loop_stays_bounded():
     """128^3 with an Nk = 256 wire on the source plane and its current fed back: with the
     CPML the undriven components grow from the point current (~1e4 per 50 steps), with the
-    mask they stay small next to the driven field over 600 steps."""
+    mask they track the driven field over 600 steps: the wire's own polarisation has x and z
+    parts (its dipole elements Xcv, Zcv), so Ex and Ez are not small, but their ratio to Ey
+    settles instead of growing (measured: Ex / Ey 0.098 -> 0.107, Ez / Ey 0.005 -> 0.008)."""
     import torch
 
     from pulsesuite_amd.PSTD3D import PSTD3DPropagator
@@ -842,6 +844,8 @@ def test_mask_closed_loop_stays_bounded():
         print("mask closed loop", prop.time.n, ["%.3e" % v for v in norms[-1]], flush=True)
     st = qa.wires.get_state(0)
     assert np.abs(np.diag(st["C2"])).max() > 1e-8  # driven
-    for ex, ey, ez in norms:
-        assert np.isfinite([ex, ey, ez]).all()
-        assert ex < 1e-2 * ey and ez < 1e-2 * ey
+    assert np.isfinite(norms).all()
+    r0 = (norms[0][0] / norms[0][1], norms[0][2] / norms[0][1])
+    for ex, ey, ez in norms[1:]:
+        assert ex / ey < 2.0 * r0[0] and ez / ey < 3.0 * r0[1]
+        assert ey < 10.0 * norms[0][1]

@@ -24,7 +24,7 @@ def herm(n, rng, top=0.5):
     return (Q * rng.uniform(0, top, n)) @ Q.conj().T
 
 
-def make_pair(nk, mb_kw=None, nrr=48, Nw=1, seed=7):
+def make_pair(nk, mb_kw=None, nrr=48, Nw=1, seed=7, qw_kw=None):
     import torch  # noqa: F401
 
     from pulsesuite_amd.SBEs import SBESolver
@@ -33,8 +33,10 @@ def make_pair(nk, mb_kw=None, nrr=48, Nw=1, seed=7):
     L = O.L_for_nk(nk)
     rr = np.linspace(-3 * L, 3 * L, nrr)
     mb_kw = mb_kw or {}
-    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, Nw, O.QWParams(L=L), O.MBFlags(**mb_kw))
-    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, Nw, True, qw_params=QWParams(L=L), mb_flags=MBFlags(**mb_kw))
+    qw_kw = qw_kw or {}
+    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, Nw, O.QWParams(L=L, **qw_kw), O.MBFlags(**mb_kw))
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, Nw, True, qw_params=QWParams(L=L, **qw_kw),
+                    mb_flags=MBFlags(**mb_kw))
     assert gpu.Nk == ref.Nk == nk
     rng = np.random.default_rng(seed)
     for w in range(Nw):
@@ -174,3 +176,75 @@ def test_vint_matches_oracle():
                                  _lib.ptr(out), _lib.stream_ptr()), "vint")
     ref = np.array([O.Vint(qq, w.r, w.alphae, w.alphah, w.Delta0) for qq in np.arange(w.Nk) * w.dkr])
     assert rel(out.cpu().numpy(), ref) < 1e-13
+
+
+@pytest.mark.parametrize("nk", [14, 70])
+@pytest.mark.parametrize("mb_kw", [
+    dict(DCTrans=True),                       # both momenta shifted (LF on): X -> S X S^T
+    dict(DCTrans=True, LF=False),             # diagonals only
+    dict(Recomb=True),
+    dict(DCTrans=True, Recomb=True, Excitons=False),
+])
+def test_dctrans_and_recomb(nk, mb_kw):
+    """Transport (PSTD3D/dcfield.py:1213-1247) with a background Edc, and SpontEmission
+    (PSTD3D/emission.py:94-131) inside the relaxation, three steps against the oracle.  The
+    shift changes the state by ~1e-3 per step here and recombination by ~1e-9, so each is
+    visible at the 1e-10 bar; nk = 70 spans two 64-wide tiles of the shift's products."""
+    ref, gpu, rr = make_pair(nk, mb_kw, qw_kw=dict(Edc=2e6))
+    r, g = run_both(ref, gpu, rr, 3)
+    for name, a, b in zip(("Pxx", "Pyy", "Pzz", "Rho"), g, r):
+        assert rel(a, b) < TOL or np.linalg.norm(b) == 0.0, name
+    st = gpu.get_state(1)
+    so = ref.state[0]
+    for k in ("p1", "p2", "C1", "C2", "D1", "D2"):
+        assert rel(st[k], getattr(so, k)) < TOL, (k, rel(st[k], getattr(so, k)))
+    if mb_kw.get("Recomb"):
+        rsp = gpu.dev.workspace_view(0, "rsp").cpu().numpy()
+        assert rel(rsp, ref.last_Rsp) < 1e-13
+        assert ref.last_Rsp.min() > 0.0
+
+
+def test_missing_term_bit_gives_nan_outputs():
+    """A DCTrans wire stepped through pss_sbe_step_p without the DCTrans bit (4) in the mask
+    gets NaN outputs rather than a step without the term."""
+    import ctypes as C
+
+    import torch
+
+    from pulsesuite_amd import _lib
+
+    ref, gpu, rr = make_pair(14, dict(DCTrans=True))
+    dev = gpu.dev
+    E = field(rr, ref.L)
+    for c, f in enumerate((E, 0.5 * E, 0.3 * E)):
+        dev.E_in[0][c].copy_(torch.as_tensor(f))
+    m = dev.models[0]
+    _lib.check(_lib.lib.pss_sbe_step_p(C.c_void_p(dev.d_descs.data_ptr()), 1, m.Nk, m.Nk, m.Nr, dev.nrr[0], 1, 0,
+                                       _lib.stream_ptr()), "pss_sbe_step_p")
+    torch.cuda.synchronize()
+    assert torch.isnan(dev.P_out[0][0]).all() and torch.isnan(dev.Rho_out[0]).all()
+
+
+def test_shipped_mb_params_run_against_oracle():
+    """The reference's shipped PSTD3D/params/qw.params and mb.params (copies under
+    tests/golden/params), which turn on DCTrans, Recomb and every many-body term: three steps
+    of the wire on the GPU against the oracle with the same parameters."""
+    import os
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import read_mb_params, read_qw_params
+
+    gp = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "params")
+    qw = read_qw_params(os.path.join(gp, "qw.params"))
+    mb = read_mb_params(os.path.join(gp, "mb.params"))
+    assert mb.DCTrans and mb.Recomb
+    rr = np.linspace(-3 * qw.L, 3 * qw.L, 64)
+    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, 1, O.QWParams(**vars(qw)), O.MBFlags(**vars(mb)))
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, 1, True, qw_params=qw, mb_flags=mb)
+    assert gpu.Nk == ref.Nk
+    r, g = run_both(ref, gpu, rr, 3, dt=1e-16)
+    for name, a, b in zip(("Pxx", "Pyy", "Pzz", "Rho"), g, r):
+        assert rel(a, b) < TOL or np.linalg.norm(b) == 0.0, name
+    st = gpu.get_state(1)
+    for k in ("p2", "C2", "D2"):
+        assert rel(st[k], getattr(ref.state[0], k)) < TOL, k

@@ -270,3 +270,110 @@ def test_rates_vanish_at_zero_and_full_occupation_term_by_term(fill):
     assert np.abs(De).max() == 0.0 and np.abs(Dh).max() > 0
     _, De, Dh = w.OffDiagDephasing2(rnd, full, p, V)
     assert np.abs(Dh).max() == 0.0 and np.abs(De).max() > 0
+
+
+# ------------------------------------------------------------------ DC transport
+def _dc_wire(nk=12, Edc=0.0, **mb):
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-2 * L, 2 * L, 48)
+    w = O.OracleWire(rr, 0.0, 1e7, 800e-9, 1, O.QWParams(L=L, Edc=Edc), O.MBFlags(**mb))
+    return w
+
+
+def _field_for_shift(w, m, dt):
+    """Eac that moves a distribution by exactly m grid points in one call: the phase of
+    Transport (PSTD3D/dcfield.py:1213-1247) is exp(-i Y_j dk), Y_j = s_j 2 pi / ((Nk-1) dky)
+    with s_j the FFT-order integer of index j, so dk = m (Nk - 1) dky / Nk gives the DFT
+    shift theorem's exp(-2 pi i j m / Nk), a circular roll by m."""
+    dky = w.kr[2] - w.kr[1]
+    dk = 2.0 * np.pi * m / w.Nk * ((w.Nk - 1) * dky) / (2.0 * np.pi)
+    return -dk * hbar / (O.e0 * dt) - w.qwp.Edc
+
+
+@pytest.mark.parametrize("full", [True, False])
+@pytest.mark.parametrize("m", [1, 3, -2])
+def test_transport_integer_shift_is_a_circular_roll(full, m):
+    w = _dc_wire(Edc=2e5)
+    rng = np.random.default_rng(11)
+    nk = w.Nk
+    X = rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))
+    dt = 1e-16
+    Y = X.copy()
+    w.Transport(Y, _field_for_shift(w, m, dt), dt, full)
+    if full:
+        want = np.roll(np.roll(X, m, axis=0), m, axis=1)
+    else:
+        want = X.copy()
+        np.fill_diagonal(want, np.roll(np.diag(X), m))
+    assert np.abs(Y - want).max() < 1e-12 * np.abs(X).max()
+
+
+def test_transport_zero_field_identity_unitarity_and_composition():
+    w = _dc_wire(Edc=3e6)
+    rng = np.random.default_rng(12)
+    nk = w.Nk
+    X = rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))
+    dt = 2e-17
+    Y = X.copy()
+    w.Transport(Y, -3e6, dt, True)  # Edc + Eac = 0
+    assert np.abs(Y - X).max() < 1e-14 * np.abs(X).max()
+    A = X.copy()
+    w.Transport(A, 1.3e7, dt, True)
+    assert abs(np.linalg.norm(A) - np.linalg.norm(X)) < 1e-13 * np.linalg.norm(X)  # a phase in Fourier space
+    assert np.abs(A - X).max() > 1e-4 * np.abs(X).max()  # a fractional shift moved it
+    # two shifts add: (Edc + E1) + (Edc + E2) = Edc + (E1 + E2 + Edc)
+    E1, E2 = 0.3e7, 0.7e7
+    B = X.copy()
+    w.Transport(B, E1, dt, True)
+    w.Transport(B, E2, dt, True)
+    C = X.copy()
+    w.Transport(C, E1 + E2 + 3e6, dt, True)
+    assert np.abs(B - C).max() < 1e-13 * np.abs(X).max()
+
+
+# ------------------------------------------------------------------ spontaneous emission
+def test_ec_special_populations():
+    """emission.py:134-175 at empty bands (Ec = -Veh(0)) and one occupied electron state j:
+    Ec[k] = Vee(0) - Vee(|j-k|) + Veh(0) for k != j, and -Veh(0) at k = j (the idel factor);
+    holes the same with Vhh and the opposite sign of the Veh term."""
+    w = _dc_wire(nk=10, Recomb=True)
+    nk = w.Nk
+    VC, rows = toeplitz_vc(nk, np.random.default_rng(5))
+    z = np.zeros(nk)
+    assert np.allclose(w.Ec(z, z, VC), -rows[0][0], rtol=1e-15, atol=0)
+    j = 3
+    e = np.zeros(nk)
+    e[j] = 1.0
+    k = np.arange(nk)
+    want_e = rows[1][0] - rows[1][np.abs(j - k)] + np.where(k != j, rows[0][0], 0.0) - rows[0][0]
+    want_h = rows[2][0] - rows[2][np.abs(j - k)] - np.where(k != j, rows[0][0], 0.0) - rows[0][0]
+    assert np.allclose(w.Ec(e, z, VC), want_e, rtol=1e-14, atol=1e-36)
+    assert np.allclose(w.Ec(z, e, VC), want_h, rtol=1e-14, atol=1e-36)
+
+
+def test_spont_integral_is_the_midpoint_rule_of_its_integral():
+    """SpontIntegral (emission.py:178-227) is the midpoint rule on [0, Nw dhw] of
+    (h + E)^3 / (c^3 pi^2 hbar^3) x 3 dcv^2 ehint^2 / (eps0 sqrt(epsr) hbar) L(h, hbar geh) exp(-h / kB T),
+    L(a, b) = (b / pi) / (a^2 + b^2) (InitializeEmission, :36-91): scipy's adaptive quadrature
+    of that integral agrees to the midpoint rule's O(dhw^2) (a left-point grid would be off
+    by O(dhw), ~1e-3 here)."""
+    from scipy.constants import c, epsilon_0, k as kB
+    from scipy.integrate import quad
+
+    w = _dc_wire(nk=10, Recomb=True)
+    H = w.HOmega
+    dhw = H[1] - H[0]
+    top = len(H) * dhw
+    hg = hbar * w.gam_eh
+    pref = 3.0 * abs(w.dcv) ** 2 * w.ehint**2 / (epsilon_0 * np.sqrt(w.epsr) * hbar)
+
+    def f(h, E):
+        return (h + E) ** 3 / (c**3 * np.pi**2 * hbar**3) * pref * (hg / np.pi) / (h**2 + hg**2) * np.exp(-h / (kB * O.TEMP))
+
+    for E in (1.4 * O.e0, 1.6 * O.e0):
+        want, _ = quad(f, 0.0, top, args=(E,), points=[hg, 5 * hg], limit=400, epsrel=1e-12)
+        got = w.SpontIntegral(np.array([E]))[0]
+        assert abs(got / want - 1.0) < 2e-5
+    # Rsp grows with the transition energy as (h + E)^3
+    r = w.SpontIntegral(np.array([1.0, 2.0]) * O.e0)
+    assert 7.0 < r[1] / r[0] < 8.5
