@@ -217,9 +217,9 @@ def precision_gate(grid, nk, npml, current="reference", closed_steps=GATE_CLOSED
 DOM_BYTES_PER_CELL = 168
 # HBM bytes per launch of that kernel at 128^3, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `bench.py --workloads traffic`, each counter scaled by what it
-# reads for a 512 MiB device copy (profiles/dominant_pass_128cube_traffic.md); None until
-# measured.
-DOM_TRAFFIC_128 = None
+# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 172282.7 KiB x 2 read +
+# 98304 KiB written (profiles/dominant_pass_128cube_traffic.md).
+DOM_TRAFFIC_128 = (172282.7 * 2.0 + 98304.0) * 1024.0
 
 
 def dominant_pass_timing(prop, reps=50):
@@ -259,6 +259,35 @@ def dominant_pass_timing(prop, reps=50):
     torch.cuda.synchronize()
     us = 1e3 * ev0.elapsed_time(ev1) / reps
     return us, DOM_BYTES_PER_CELL * n
+
+
+def in_step_pass_timing(prop, steps=12):
+    """Average duration of the dominant kernel inside the pipelined step: `steps` eager
+    pipelined steps (the same launches on the same two streams as a replayed graph, the wire
+    step beside the field branch) with the library's launch timing armed, so each launch of
+    the fused last pass records HIP events around itself on its own stream.  Run after the
+    timed region (it advances the state)."""
+    import ctypes as C
+
+    import torch
+
+    from pulsesuite_amd import _lib
+
+    n = 2 * steps + 2
+    _lib.check(_lib.lib.pss_pass_timing_arm(n), "pss_pass_timing_arm")
+    try:
+        prop._cpml_e_next(None)
+        for _ in range(steps):
+            prop._pipelined_step(True)
+        prop._pipelined_step(False)
+        torch.cuda.synchronize()
+        ms = (C.c_float * n)()
+        got = C.c_int32(0)
+        _lib.check(_lib.lib.pss_pass_timing_read(ms, n, C.byref(got)), "pss_pass_timing_read")
+    finally:
+        _lib.check(_lib.lib.pss_pass_timing_arm(0), "pss_pass_timing_arm")
+    vals = [1e3 * ms[i] for i in range(got.value)][2:]  # the first step's launches warm up
+    return (sum(vals) / len(vals) if vals else None), len(vals)
 
 
 def step_bytes_per_cell(grid, npml, n_fft=31):
@@ -582,13 +611,18 @@ def main():
         line["precision_gate"] = gate
         line["mixed_within_1e-6"] = gate["within_1e-6"]
     if not a.no_kernel_timing:
-        us, b = dominant_pass_timing(r["prop"])
+        us_alone, b = dominant_pass_timing(r["prop"])
+        us, nl = (in_step_pass_timing(r["prop"]) if main_prec == "mixed" else (None, 0))
+        where = "inside eager pipelined steps (wire step beside it), HIP events around each launch on its stream"
+        if us is None:  # FP64: the standalone launch
+            us, where = us_alone, "alone, HIP events on its stream"
         ach = b / (us * 1e-6)
         line["roofline"] = {"bound": "hbm",
                             "kernel": f"k_fft2<8,16,3,float>: last CPML dF pass + fused curl update, {a.grid}^3",
                             "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
                             "traffic": DOM_TRAFFIC_128 if a.grid == 128 else None,
-                            "algorithmic_bytes": b, "avg_us": us,
+                            "algorithmic_bytes": b, "avg_us": us, "timed": where, "launches_timed": nl,
+                            "alone_avg_us": us_alone, "alone_frac": b / (us_alone * 1e-6) / HBM_PEAK,
                             "launches_per_step": 2 if main_prec == "mixed" else None}
         n = a.grid ** 3
         spb = step_bytes_per_cell(a.grid, a.npml)

@@ -295,6 +295,16 @@ int pss_mask_half(const pss_grid* g, double* Fx, double* Fy, double* Fz, const d
                   const double* mz, int32_t src_comp, const double* src_par, const double* srcx, const double* rx,
                   const double* ry, const double* rz, void* stream);
 
+/* Launch timing of the accumulating last pass with the curl term (pss_b_half and
+ * pss_cpml_e_curl in mixed precision): no counterpart in the reference; for measurement.
+ * pss_pass_timing_arm(n) arms n event pairs (created on the first call that needs them, so
+ * call it outside any stream capture); each following such launch on a stream that is not
+ * capturing records one pair around itself, until n are taken.  n = 0 disarms.
+ * pss_pass_timing_read, after the work has completed, writes the durations (ms) of the
+ * first min(n, taken) launches and the count taken. */
+int pss_pass_timing_arm(int32_t n);
+int pss_pass_timing_read(float* ms, int32_t n, int32_t* got);
+
 /* The trilinear point sample of pss_cpml_half taken on its own, from k-space fields that
  * already hold the whole half step (EAtXYZ of the IFFT, PSTD3D/usefulsubs.py:2016-2085):
  * sample_out[6] = (Re, Im) of each component.  partial: 6144 doubles of device scratch.

@@ -62,7 +62,7 @@ class OracleCoupling:
     """Nw sub-band wires along x through (x0, y0, z0) feeding J into the grid."""
 
     def __init__(self, space, lam, Emax0, Nw=1, qw=None, mb=None, y0=0.0, z0=0.0, ay=5e-9, az=5e-9, x0=0.0,
-                 current="reference"):
+                 current="reference", integrator="leapfrog"):
         self.current = current
         self.space = space
         self.rx = GetXArray(space)
@@ -71,7 +71,7 @@ class OracleCoupling:
         self.qx = GetKxArray(space)
         self.Nw = Nw
         self.y0, self.z0, self.x0 = y0, z0, x0
-        self.wire = OracleWire(self.rx, 0.0, Emax0, lam, Nw, qw, mb)
+        self.wire = OracleWire(self.rx, 0.0, Emax0, lam, Nw, qw, mb, integrator=integrator)
         Nx = space.Nx
         self.Pold = np.zeros((3, Nx, Nw), dtype=np.complex128)
         self.RhoOld = np.zeros((Nx, Nw), dtype=np.complex128)

@@ -27,6 +27,10 @@ __all__ = ["SBESolver", "InitializeSBE", "QWCalculator", "chiqw", "getqc", "QWAr
 
 _CDT = torch.complex128
 
+# stage bits of pss_sbe_stages (include/pss.h)
+ST_FRONT, ST_TRACES, ST_OFFDIAG, ST_CALCH, ST_RHS, ST_COLL, ST_TAIL = 1, 2, 4, 8, 16, 32, 64
+ST_DERIV = ST_FRONT | ST_TRACES | ST_OFFDIAG | ST_CALCH | ST_RHS
+
 
 def _dev(device):
     if device is None:
@@ -227,6 +231,85 @@ class DeviceWires:
                                            max(self.nrr[lo:hi]), precision_mask(self.prec_code, sub), ksplit_cap,
                                            _lib.stream_ptr(stream)), "pss_sbe_step_p")
 
+    def step_rk4(self, wires=None, stream=None):
+        """One step of the wires in ``wires`` with the classic RK4 integrator in place of the
+        leapfrog: the coherent right-hand side (dpdt, dCdt, dDdt with the Hamiltonian,
+        screening and dephasing of each stage's state, pss_sbe_stages with dX/dt only) in
+        the stage order of _rk4 (libpulsesuite/integrator.py:143-157), the combines on
+        pss_rk4_stage; then the relaxation, transport, renormalisation and polarisation of
+        the step (its collision, phonon and emission rates from the updated state).  The
+        prev buffer holds y during the step.  Eager only: it reads the wires' step counters
+        and the wireoff latch on the host.  A wire group that is gated off takes the plain
+        step (zero outputs, nothing advanced)."""
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("the RK4 wire step reads step counters on the host and cannot be captured")
+        lo, hi = (0, self.Nw) if wires is None else wires
+        sub = self.models[lo:hi]
+        pm = precision_mask(self.prec_code, sub)
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        sp = _lib.stream_ptr(s)
+        self.kernel("pss_sbe_stages", pm, ST_FRONT, 0, wires=(lo, hi), stream=s)  # the wireoff gate
+        on = self.on.cpu().numpy()
+        if not all(on[self.group[i]] for i in range(lo, hi)):
+            if any(on[self.group[i]] for i in range(lo, hi)):
+                raise RuntimeError("RK4 step: wires of one launch must be all on or all off")
+            self.step((lo, hi), s)
+            return
+        par = [int(v) & 1 for v in self.ctl[lo:hi, 0].cpu().numpy()]
+        h = self.dt
+        work = []
+        with torch.cuda.stream(s):
+            for i, pb in zip(range(lo, hi), par):
+                st = self.state[i]
+                cur, nxt = 1 + pb, 2 - pb
+                scr = self._rk_scratch(i)
+                for n in ("p", "C", "D"):
+                    st[n][0].copy_(st[n][cur])  # y
+                work.append((st, cur, nxt, scr))
+
+            def rk(stage, n, y, dydt=None, dyt=None, dym=None, ytmp=None):
+                _lib.check(_lib.lib.pss_rk4_stage(stage, 2 * y.numel(), float(h), _lib.ptr(y), _lib.ptr(dydt),
+                                                  _lib.ptr(dyt), _lib.ptr(dym), _lib.ptr(ytmp), sp), "pss_rk4_stage")
+
+            def deriv():
+                self.kernel("pss_sbe_stages", pm, ST_DERIV, 1, wires=(lo, hi), stream=s)
+
+            deriv()  # dydt -> next
+            for st, cur, nxt, scr in work:
+                for j, n in enumerate(("p", "C", "D")):
+                    scr[0][j].copy_(st[n][nxt])
+                    rk(0, n, st[n][0], dydt=st[n][nxt], ytmp=st[n][cur])
+            deriv()  # dyt -> next
+            for st, cur, nxt, scr in work:
+                for j, n in enumerate(("p", "C", "D")):
+                    scr[1][j].copy_(st[n][nxt])
+                    rk(1, n, st[n][0], dyt=st[n][nxt], ytmp=st[n][cur])
+            deriv()  # dym -> next
+            for st, cur, nxt, scr in work:
+                for j, n in enumerate(("p", "C", "D")):
+                    rk(2, n, st[n][0], dyt=scr[1][j], dym=st[n][nxt], ytmp=st[n][cur])
+                    scr[2][j].copy_(st[n][nxt])
+            deriv()  # dyt -> next
+            for st, cur, nxt, scr in work:
+                for j, n in enumerate(("p", "C", "D")):
+                    rk(3, n, st[n][0], dydt=scr[0][j], dyt=st[n][nxt], dym=scr[2][j])
+                    st[n][cur].copy_(st[n][0])
+                    st[n][nxt].copy_(st[n][0])
+        # rates of the updated state, then relaxation ... polarisation on the next buffers
+        self.kernel("pss_sbe_stages", pm, ST_FRONT, 0, wires=(lo, hi), stream=s)
+        self.kernel("pss_sbe_stages", pm, ST_COLL | ST_TAIL, 0, wires=(lo, hi), stream=s)
+
+    def _rk_scratch(self, i):
+        """dydt, dyt, dym of wire i ([3][3][Nk][Nk]: stage value, then p, C, D), allocated on
+        the first RK4 step."""
+        sc = getattr(self, "_rk", None)
+        if sc is None:
+            sc = self._rk = {}
+        if i not in sc:
+            Nk = self.models[i].Nk
+            sc[i] = torch.zeros((3, 3, Nk, Nk), dtype=_CDT, device=self.device)
+        return sc[i]
+
     def kernel(self, name, *extra, wires=None, stream=None):
         """Run a kernel-level entry (pss_calc_h, pss_sbe_rhs, pss_coll_rates, pss_qwpol) on the
         wires in the contiguous range ``wires`` (default: all); ``extra`` are the entry's
@@ -348,12 +431,16 @@ class SBESolver:
 
     Keyword-only extras: ``qw_params`` / ``mb_flags`` replace the files
     params/qw.params and params/mb.params read from the CWD; ``precision``
-    selects "fp64" (default) or "mixed" contractions; ``device`` the GPU.
+    selects "fp64" (default) or "mixed" contractions; ``device`` the GPU;
+    ``integrator`` "leapfrog" (the reference's step) or "rk4" (DeviceWires.step_rk4).
     """
 
     def __init__(self, q, rr, r0, Emaxxx, lam, Nw, QW, backend="auto", *, qw_params=None, mb_flags=None,
-                 precision="fp64", device=None):
-        self._backend = backend
+                 precision="fp64", device=None, integrator="leapfrog"):
+        self._backend = backend  # stored and not read, as the reference does (PSTD3D/SBEs.py:4750)
+        if integrator not in ("leapfrog", "rk4"):
+            raise ValueError(f"integrator must be 'leapfrog' or 'rk4', not {integrator!r}")
+        self.integrator = integrator
         self.Nw = Nw
         self.QW = bool(QW)
         if not QW:
@@ -391,7 +478,10 @@ class SBESolver:
         dev.set_dt(dt)
         for c, F in enumerate((Exx, Eyy, Ezz)):
             dev.E_in[i][c].copy_(torch.as_tensor(F).to(dev.device, _CDT), non_blocking=True)
-        dev.step((i, i + 1))
+        if self.integrator == "rk4":
+            dev.step_rk4((i, i + 1))
+        else:
+            dev.step((i, i + 1))
         for c, F in enumerate((Pxx, Pyy, Pzz)):
             _copy_out(F, dev.P_out[i][c])
         _copy_out(Rho, dev.Rho_out[i])
@@ -406,7 +496,10 @@ class SBESolver:
         self.t += dt
         self.xxx += 1
         self.dev.set_dt(dt)
-        self.dev.step(None, stream)
+        if self.integrator == "rk4":
+            self.dev.step_rk4(None, stream)
+        else:
+            self.dev.step(None, stream)
 
     # ---------------------------------------------------------- accessors
     def chiqw(self):

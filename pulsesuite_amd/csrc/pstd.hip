@@ -6,6 +6,7 @@
 // half steps exactly as PSTD3D/PSTD3D.py keeps them.
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "../../include/pss.h"
@@ -646,6 +647,21 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
   return 0;
 }
 
+// Launch timing of the accumulating last pass with the curl term (the largest kernel of a
+// mixed-precision coupled step), for measurement: armed for n launches, each such launch on a
+// stream that is not capturing records an event pair around itself.
+std::vector<hipEvent_t> g_tev;
+int g_tev_armed = 0, g_tev_next = 0;
+std::mutex g_tev_mu;
+
+int timed_slot(hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_tev_mu);
+  if (g_tev_next >= g_tev_armed) return -1;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return -1;
+  return g_tev_next++;
+}
+
 // out += FFT(tmp) along every axis of length > 1; tmp is overwritten (used in place
 // for all passes but the last, which accumulates into out).  tmp32: tmp holds
 // complex64 and the passes compute in FP32 (out stays complex128).
@@ -661,8 +677,11 @@ int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, in
     if (N == 1) continue;
     const bool fin = axis == last;
     const cplx* ins[3] = {tmps[0], nb > 1 ? tmps[1] : nullptr, nb > 2 ? tmps[2] : nullptr};
+    const int slot = fin && cu ? timed_slot(s) : -1;
+    if (slot >= 0) hipEventRecord(g_tev[2 * slot], s);
     const int rc = launch_pass_b(g, ins, fin ? outs : tmps, nb, axis, dir, nullptr, -1, s, fin ? (cu ? 3 : 1) : 0,
                                  tmp32, fin ? 0 : tmp32, 0, fin ? cu : nullptr);
+    if (slot >= 0) hipEventRecord(g_tev[2 * slot + 1], s);
     if (rc) return rc;
   }
   return 0;
@@ -1711,5 +1730,28 @@ extern "C" int pss_jline(const double* P, const double* Rho, double* Pold, doubl
   hipLaunchKernelGGL(k_jline, dim3(1), dim3(256), 0, (hipStream_t)stream, (const cplx*)P, (const cplx*)Rho,
                      (cplx*)Pold, (cplx*)RhoOld, Nw, Nx, dt, dx, (cplx*)jline, (int)(fft != 0), (int)(keep_old != 0));
   PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---- launch timing of the fused last pass (diagnostics; no counterpart in the reference)
+extern "C" int pss_pass_timing_arm(int32_t n) {
+  if (n < 0 || n > 4096) return pss_set_msg(2, "pss_pass_timing_arm: 0 <= n <= 4096");
+  std::lock_guard<std::mutex> lock(g_tev_mu);
+  while ((int)g_tev.size() < 2 * n) {
+    hipEvent_t e;
+    PSS_CHECK(hipEventCreate(&e));
+    g_tev.push_back(e);
+  }
+  g_tev_armed = n;
+  g_tev_next = 0;
+  return 0;
+}
+
+extern "C" int pss_pass_timing_read(float* ms, int32_t n, int32_t* got) {
+  if (!ms || !got || n < 0) return pss_set_msg(2, "pss_pass_timing_read: null output or n < 0");
+  std::lock_guard<std::mutex> lock(g_tev_mu);
+  const int k = std::min(n, g_tev_next);
+  for (int i = 0; i < k; ++i) PSS_CHECK(hipEventElapsedTime(&ms[i], g_tev[2 * i], g_tev[2 * i + 1]));
+  *got = k;
   return 0;
 }

@@ -67,15 +67,19 @@ class QWArray:
 
     qw_params / mb_flags configure the wires (see SBEs.SBESolver); Emax0 sets
     the wireoff gate (1e-3 Emax0); current selects the current semantics, "reference"
-    (J = P/dt, the reference's code) or "delta" (J = dP/dt).
+    (J = P/dt, the reference's code) or "delta" (J = dP/dt); integrator "leapfrog" (the
+    reference's wire step) or "rk4" (DeviceWires.step_rk4: eager, not capturable).
     """
 
     separable = True
 
     def __init__(self, qw_params, mb_flags, Nw=1, Emax0=1.0, y0=0.0, z0=0.0, x0=0.0, ay=5e-9, az=5e-9,
-                 precision="fp64", current="reference"):
+                 precision="fp64", current="reference", integrator="leapfrog"):
         if current not in ("reference", "delta"):
             raise ValueError(f"current must be 'reference' or 'delta', not {current!r}")
+        if integrator not in ("leapfrog", "rk4"):
+            raise ValueError(f"integrator must be 'leapfrog' or 'rk4', not {integrator!r}")
+        self.integrator = integrator
         self.current = current
         self.qw, self.mb = qw_params, mb_flags
         self.Nw = Nw
@@ -159,6 +163,9 @@ class QWArray:
         # (FP64) keeps the contractions' workgroups from crowding the field passes (measured
         # at Nk = 256, 128^3: caps 1/2/4/8 gave 0.634/0.598/0.591/0.596 ms mixed and
         # 1.137/0.865/0.813/0.807 ms FP64 per pipelined step)
+        if self.integrator == "rk4":
+            self.wires.step_rk4()
+            return
         self.wires.step(ksplit_cap=4 if self.precision == "mixed" else 8)
 
     def drive(self, space, time):

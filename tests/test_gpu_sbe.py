@@ -248,3 +248,65 @@ def test_shipped_mb_params_run_against_oracle():
     st = gpu.get_state(1)
     for k in ("p2", "C2", "D2"):
         assert rel(st[k], getattr(ref.state[0], k)) < TOL, k
+
+
+@pytest.mark.parametrize("nk", [16, 34])
+@pytest.mark.parametrize("mb_kw", [
+    dict(),
+    dict(DCTrans=True, Recomb=True),
+    dict(OBE=True),
+])
+def test_rk4_mode_against_oracle(nk, mb_kw):
+    """integrator="rk4": the coherent RK4 step (stage order of _rk4,
+    libpulsesuite/integrator.py:143-157) then the step's relaxation, transport and
+    renormalisation, three steps against the oracle's RK4 mode."""
+    import torch
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import MBFlags, QWParams
+
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, 48)
+    ref = O.OracleWire(rr, 0.0, 1e7, 800e-9, 1, O.QWParams(L=L, Edc=2e6), O.MBFlags(**mb_kw), integrator="rk4")
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, 1, True, qw_params=QWParams(L=L, Edc=2e6), mb_flags=MBFlags(**mb_kw),
+                    integrator="rk4")
+    rng = np.random.default_rng(9)
+    st = dict(p1=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+              p2=0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk))),
+              C1=herm(nk, rng), C2=herm(nk, rng), D1=herm(nk, rng), D2=herm(nk, rng))
+    s = ref.state[0]
+    s.p1, s.p2, s.C1, s.C2, s.D1, s.D2 = (st[k].copy() for k in ("p1", "p2", "C1", "C2", "D1", "D2"))
+    gpu.set_state(1, **st)
+    r, g = run_both(ref, gpu, rr, 3)
+    torch.cuda.synchronize()
+    for name, a, b in zip(("Pxx", "Pyy", "Pzz", "Rho"), g, r):
+        assert rel(a, b) < TOL or np.linalg.norm(b) == 0.0, (name, rel(a, b))
+    got = gpu.get_state(1)
+    for k in ("p2", "C2", "D2"):
+        assert rel(got[k], getattr(s, k)) < TOL, (k, rel(got[k], getattr(s, k)))
+
+
+def test_rk4_mode_converges_at_fourth_order():
+    """The RK4 mode's coherent update on the free two-level wire (OBE flags, no carriers):
+    halving dt shrinks the error of p against the oracle's dt/8 solution ~16-fold."""
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import MBFlags, QWParams
+
+    nk = 16
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, 48)
+    E = field(rr, L, amp=1e6)
+    T = 8e-16
+
+    def run(dt):
+        gpu = SBESolver(None, rr, 0.0, 1e5, 800e-9, 1, True, qw_params=QWParams(L=L), mb_flags=MBFlags(OBE=True),
+                        integrator="rk4")
+        outs = [np.zeros(len(rr), complex) for _ in range(4)]
+        for _ in range(int(round(T / dt))):
+            gpu.QWCalculator(E, 0.5 * E, 0.3 * E, 0 * E, rr, None, dt, 1, *outs, [False], [False])
+        return gpu.get_state(1)["p2"]
+
+    ref = run(T / 64)
+    e1 = np.linalg.norm(run(T / 8) - ref)
+    e2 = np.linalg.norm(run(T / 16) - ref)
+    assert 10.0 < e1 / e2 < 20.0, (e1, e2)

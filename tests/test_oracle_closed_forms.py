@@ -377,3 +377,33 @@ def test_spont_integral_is_the_midpoint_rule_of_its_integral():
     # Rsp grows with the transition energy as (h + E)^3
     r = w.SpontIntegral(np.array([1.0, 2.0]) * O.e0)
     assert 7.0 < r[1] / r[0] < 8.5
+
+
+# ------------------------------------------------------------------ RK4 mode
+def test_rk4_mode_is_fourth_order_and_leapfrog_is_not():
+    """The oracle's RK4 mode (the stage order of _rk4, libpulsesuite/integrator.py:143-157,
+    on the wire's coherent right-hand side) on the free two-level wire (OBE flags): halving
+    dt shrinks the error of p against a dt/8 run ~16-fold; a wrong stage weight or node
+    drops the order.  The reference's leapfrog with its prev averaging converges far more
+    slowly on the same problem."""
+    nk = 16
+    L = O.L_for_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, 48)
+    x = rr / L
+    E = (1e6 * np.exp(-((x / 1.5) ** 2)) * np.cos(2.0 * x)).astype(complex)
+    T = 8e-16
+
+    def run(dt, integ):
+        w = O.OracleWire(rr, 0.0, 1e5, 800e-9, 1, O.QWParams(L=L), O.MBFlags(OBE=True), integrator=integ)
+        for _ in range(int(round(T / dt))):
+            w.QWCalculator(E, 0.5 * E, 0.3 * E, rr, dt, 1)
+        return w.state[0].p2
+
+    ref = run(T / 64, "rk4")
+    e1 = np.linalg.norm(run(T / 8, "rk4") - ref)
+    e2 = np.linalg.norm(run(T / 16, "rk4") - ref)
+    assert 14.0 < e1 / e2 < 18.5, (e1, e2)
+    lref = run(T / 64, "leapfrog")
+    l1 = np.linalg.norm(run(T / 8, "leapfrog") - lref)
+    l2 = np.linalg.norm(run(T / 16, "leapfrog") - lref)
+    assert l1 / l2 < 4.5 and l1 > 100 * e1
