@@ -285,7 +285,7 @@ __device__ __forceinline__ void store_out(const PassArgs& a, cplx* out, int64_t 
 // per element instead of log2(N) staged radix-2 sweeps.
 // by value: a reference to the kernel's by-value argument would put it in scratch memory
 template <int N1, int N2, int MODE, class T, bool SLAB>
-__device__ __forceinline__ void fft2_body(const PassArgs a, const int bz) {
+__device__ __forceinline__ void fft2_body(const PassArgs a, const int bz, const int bx) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
   constexpr int TB0 = 256 / G, TBL = 2048 / N;
@@ -305,7 +305,7 @@ __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz) {
   int64_t lb = 0;            // first line of the block (axis 2)
   if (a.axis == 2) {
     const int64_t nlines = (int64_t)Nx * Ny;
-    const int64_t l0 = (int64_t)blockIdx.x * TB;
+    const int64_t l0 = (int64_t)bx * TB;
     if (l0 >= nlines) return;
     lb = l0;
     nvalid = (int)min((int64_t)TB, nlines - l0);
@@ -319,8 +319,8 @@ __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz) {
     osa = 1;
   } else {
     const int nch = (Nz + TB - 1) / TB;
-    const int other = (int)(blockIdx.x / nch);
-    cz0 = (int)(blockIdx.x % nch) * TB;
+    const int other = (int)(bx / nch);
+    cz0 = (int)(bx % nch) * TB;
     if (a.axis == 1) {
       if (other >= Nx) return;
       cx = other;
@@ -432,9 +432,18 @@ __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz) {
   }
 }
 
+// MODE 3 runs on a 1-D grid that deals the three components of one line group to the same
+// XCD back to back (workgroups go to the 8 XCDs in turn): line group g of component c is
+// workgroup 24 (g / 8) + 8 c + g % 8, so the other field's lines that two components' curl
+// terms both read are still in that XCD's L2 for the second.
 template <int N1, int N2, int MODE, class T, bool SLAB = false>
 __global__ void __launch_bounds__(256) k_fft2(PassArgs a) {
-  fft2_body<N1, N2, MODE, T, SLAB>(a, blockIdx.y);
+  if (MODE == 3) {
+    const int lin = blockIdx.x, grp = lin / 24, r = lin - 24 * grp;
+    fft2_body<N1, N2, MODE, T, SLAB>(a, r >> 3, grp * 8 + (r & 7));
+  } else {
+    fft2_body<N1, N2, MODE, T, SLAB>(a, blockIdx.y, blockIdx.x);
+  }
 }
 
 // Up to three passes of one length, each over up to two arrays, possibly of different
@@ -449,7 +458,7 @@ struct MultiPass {
 template <int N1, int N2, int MODE, class T, bool SLAB>
 __global__ void __launch_bounds__(256) k_fft2_multi(MultiPass m) {
   const int e = blockIdx.y;
-  fft2_body<N1, N2, MODE, T, SLAB>(m.p[m.q[e]], m.b[e]);
+  fft2_body<N1, N2, MODE, T, SLAB>(m.p[m.q[e]], m.b[e], blockIdx.x);
 }
 
 // workgroups of one two-level pass along `axis` of grid (Nx, Ny, Nz)
@@ -496,7 +505,8 @@ void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s, int nb) {
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);
-  const dim3 gr((unsigned)blocks, (unsigned)nb), bl(TB * G);
+  const dim3 gr(a.accum == 3 ? (unsigned)((blocks + 7) / 8 * 24) : (unsigned)blocks, a.accum == 3 ? 1u : (unsigned)nb),
+      bl(TB * G);
   if (a.slab) {  // first pass of a slab derivative: store (complex) or real part
     if (a.out32) {
       if (a.accum == 2) hipLaunchKernelGGL((k_fft2<N1, N2, 2, float, true>), gr, bl, 0, s, a);
