@@ -107,6 +107,12 @@ __device__ inline double sdiff(const double* __restrict__ y, int ys, const doubl
   return (y[(int64_t)(i + 1) * ys] - y[(int64_t)i * ys]) / h[i];
 }
 
+// The loops below take a lane's elements in chunks of SCH whose loads are all issued before
+// the chunk's arithmetic: the recurrences are serial per lane, so one memory round trip per
+// element would set their latency (the wire kernels run beside HBM-bound field passes,
+// where a round trip costs microseconds).  The arithmetic is that of the element loop.
+constexpr int SCH = 4;
+
 __device__ void spline_solve_wave(const double* __restrict__ y, int ys, int n, const double* __restrict__ h,
                                   const double* __restrict__ t, const double* __restrict__ b,
                                   const double* __restrict__ e, double* __restrict__ c) {
@@ -115,29 +121,53 @@ __device__ void spline_solve_wave(const double* __restrict__ y, int ys, int n, c
   const int lo = min(n, lane * m), hi = min(n, lo + m);
   // right-hand side: interior second differences, FMM end conditions
   // e = {x3-x1, x2-x0, h0^2, x3-x0, x[n-1]-x[n-3], x[n-2]-x[n-4], h[n-2]^2, x[n-1]-x[n-4]}
-  for (int i = lo; i < hi; ++i) {
-    double r;
-    if (i == 0) {
-      const double c1 = sdiff(y, ys, h, 1) - sdiff(y, ys, h, 0);
-      const double c2 = sdiff(y, ys, h, 2) - sdiff(y, ys, h, 1);
-      r = c2 / e[0] - c1 / e[1];
-      r = r * e[2] / e[3];
-    } else if (i == n - 1) {
-      const double cm2 = sdiff(y, ys, h, n - 2) - sdiff(y, ys, h, n - 3);
-      const double cm3 = sdiff(y, ys, h, n - 3) - sdiff(y, ys, h, n - 4);
-      r = cm2 / e[4] - cm3 / e[5];
-      r = -r * e[6] / e[7];
-    } else {
-      r = sdiff(y, ys, h, i) - sdiff(y, ys, h, i - 1);
+  for (int i0 = lo; i0 < hi; i0 += SCH) {
+    double yy[SCH + 2], hq[SCH + 1];  // y[i0-1 .. i0+SCH], h[i0-1 .. i0+SCH-1]
+#pragma unroll
+    for (int u = 0; u < SCH + 2; ++u) {
+      const int i = i0 - 1 + u;
+      yy[u] = (i >= 0 && i < n) ? y[(int64_t)i * ys] : 0.0;
     }
-    c[i] = r;
+#pragma unroll
+    for (int u = 0; u < SCH + 1; ++u) {
+      const int i = i0 - 1 + u;
+      hq[u] = (i >= 0 && i < n - 1) ? h[i] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u)  // sdiff(i) - sdiff(i - 1); the two ends are set below
+      if (i0 + u < hi) c[i0 + u] = (yy[u + 2] - yy[u + 1]) / hq[u + 1] - (yy[u + 1] - yy[u]) / hq[u];
   }
-  // forward elimination x_i = r_i - t_i x_{i-1} (x_0 = r_0)
+  if (lo == 0 && hi > 0) {
+    const double c1 = sdiff(y, ys, h, 1) - sdiff(y, ys, h, 0);
+    const double c2 = sdiff(y, ys, h, 2) - sdiff(y, ys, h, 1);
+    double r = c2 / e[0] - c1 / e[1];
+    c[0] = r * e[2] / e[3];
+  }
+  if (lo <= n - 1 && n - 1 < hi) {
+    const double cm2 = sdiff(y, ys, h, n - 2) - sdiff(y, ys, h, n - 3);
+    const double cm3 = sdiff(y, ys, h, n - 3) - sdiff(y, ys, h, n - 4);
+    double r = cm2 / e[4] - cm3 / e[5];
+    c[n - 1] = -r * e[6] / e[7];
+  }
+  // forward elimination x_i = r_i - t_i x_{i-1} (x_0 = r_0): the lane's affine map first
   double A = 1.0, B = 0.0;
-  for (int i = lo; i < hi; ++i) {
-    const double a = i == 0 ? 0.0 : -t[i];
-    B = a * B + c[i];
-    A = a * A;
+  for (int i0 = lo; i0 < hi; i0 += SCH) {
+    double cc[SCH], tt[SCH];
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i0 + u;
+      cc[u] = i < hi ? c[i] : 0.0;
+      tt[u] = i < hi ? t[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i0 + u;
+      if (i < hi) {
+        const double a = i == 0 ? 0.0 : -tt[u];
+        B = a * B + cc[u];
+        A = a * A;
+      }
+    }
   }
   for (int off = 1; off < 64; off <<= 1) {
     const double Ap = __shfl_up(A, off, 64), Bp = __shfl_up(B, off, 64);
@@ -148,17 +178,47 @@ __device__ void spline_solve_wave(const double* __restrict__ y, int ys, int n, c
   }
   double x = __shfl_up(B, 1, 64);
   if (lane == 0) x = 0.0;
-  for (int i = lo; i < hi; ++i) {
-    x = i == 0 ? c[0] : c[i] - t[i] * x;
-    c[i] = x;
+  for (int i0 = lo; i0 < hi; i0 += SCH) {
+    double cc[SCH], tt[SCH];
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i0 + u;
+      cc[u] = i < hi ? c[i] : 0.0;
+      tt[u] = i < hi ? t[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i0 + u;
+      if (i < hi) {
+        x = i == 0 ? cc[u] : cc[u] - tt[u] * x;
+        cc[u] = x;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u)
+      if (i0 + u < hi) c[i0 + u] = cc[u];
   }
   // back substitution x_i = (x_i - h_i x_{i+1}) / b_i (x_{n-1} = x_{n-1} / b_{n-1})
   A = 1.0;
   B = 0.0;
-  for (int i = hi - 1; i >= lo; --i) {
-    const double a = i == n - 1 ? 0.0 : -h[i] / b[i];
-    B = a * B + c[i] / b[i];
-    A = a * A;
+  for (int i1 = hi - 1; i1 >= lo; i1 -= SCH) {
+    double cc[SCH], hh[SCH], bb[SCH];
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i1 - u;
+      cc[u] = i >= lo ? c[i] : 0.0;
+      hh[u] = i >= lo ? h[i] : 0.0;
+      bb[u] = i >= lo ? b[i] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i1 - u;
+      if (i >= lo) {
+        const double a = i == n - 1 ? 0.0 : -hh[u] / bb[u];
+        B = a * B + cc[u] / bb[u];
+        A = a * A;
+      }
+    }
   }
   for (int off = 1; off < 64; off <<= 1) {
     const double Ap = __shfl_down(A, off, 64), Bp = __shfl_down(B, off, 64);
@@ -169,9 +229,26 @@ __device__ void spline_solve_wave(const double* __restrict__ y, int ys, int n, c
   }
   x = __shfl_down(B, 1, 64);
   if (lane == 63) x = 0.0;
-  for (int i = hi - 1; i >= lo; --i) {
-    x = i == n - 1 ? c[i] / b[i] : (c[i] - h[i] * x) / b[i];
-    c[i] = x;
+  for (int i1 = hi - 1; i1 >= lo; i1 -= SCH) {
+    double cc[SCH], hh[SCH], bb[SCH];
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i1 - u;
+      cc[u] = i >= lo ? c[i] : 0.0;
+      hh[u] = i >= lo ? h[i] : 0.0;
+      bb[u] = i >= lo ? b[i] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u) {
+      const int i = i1 - u;
+      if (i >= lo) {
+        x = i == n - 1 ? cc[u] / bb[u] : (cc[u] - hh[u] * x) / bb[u];
+        cc[u] = x;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SCH; ++u)
+      if (i1 - u >= lo) c[i1 - u] = cc[u];
   }
 }
 
@@ -269,14 +346,18 @@ __global__ void __launch_bounds__(256) k_prop_front(const pss_wire_desc* __restr
   __syncthreads();
   for (int j = threadIdx.x; j < d.Nr; j += 256) {
     const int i = d.sa_idx[j];
+    double v[3];  // the three evaluations' loads before the stores
+#pragma unroll
     for (int f = 0; f < 3; ++f) {
-      double v = 0.0;
+      v[f] = 0.0;
       if (i >= 0) {
         const double* y = (f == 0 ? d.Exx : f == 1 ? d.Eyy : d.Ezz);
-        v = spline_eval(y, 2, ws + L.spa + f * d.nrr, d.sa_h, d.nrr, i, d.sa_dx[j]);
+        v[f] = spline_eval(y, 2, ws + L.spa + f * d.nrr, d.sa_h, d.nrr, i, d.sa_dx[j]);
       }
-      ws[L.er + f * d.Nr + j] = v * d.win[j];
     }
+    const double wj = d.win[j];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) ws[L.er + f * d.Nr + j] = v[f] * wj;
   }
 }
 
@@ -441,7 +522,9 @@ __device__ inline void offdiag_sum32(const float4* __restrict__ P, int n, float 
     for (int j = j0; j < j1; ++j) {
       const float4 v = P[j];
       const float z = Fs + v.x;
-      const float l = __frcp_rn(fmaf(z, z, 1.0f));
+      // the hardware reciprocal (v_rcp_f32, 1 ulp): one instruction instead of the
+      // correctly rounded division sequence; z^2 + 1 >= 1, so no denormal or range case
+      const float l = __builtin_amdgcn_rcpf(fmaf(z, z, 1.0f));
       sa = fmaf(l, v.y, sa);
       sb = fmaf(l, v.z, sb);
     }
@@ -1397,6 +1480,9 @@ __global__ void __launch_bounds__(256) k_xsplit(const pss_wire_desc* __restrict_
   // source rows r = u - d for dd in [dd0, dd0 + 32), u in [u0, u0 + 64): r = rb + rr, rr < 95
   const int rb = u0 - (dd0 + 31 - (Nk - 1));
   __shared__ _Float16 S[4][95][64];
+  // unrolled so that a thread's ~24 row loads are in flight together (the loop body's
+  // only other memory traffic is LDS)
+#pragma unroll 8
   for (int e = threadIdx.x; e < 95 * 64; e += 256) {
     const int rr = e >> 6, uu = e & 63;
     const int r = rb + rr, u = u0 + uu;
@@ -1606,12 +1692,12 @@ struct CollCtx {
 
 // GaussDelta: (2 pi/hbar) exp(-(E/hw)^2) / (sqrt(pi) hw)  (coulomb.py:245-263);
 // Lorentz: 2 g / (E^2 + hw^2).  FAST (mixed precision): the exponential, or the
-// reciprocal (2 g / hw^2) / (z^2 + 1) of the Lorentzian, in FP32.
+// reciprocal (2 g / hw^2) / (z^2 + 1) of the Lorentzian (hardware v_rcp_f32), in FP32.
 template <bool LORENTZ, bool FAST>
 __device__ inline double line_shape(const CollCtx& x, int kind, double Ed) {
   const double z = Ed * x.ihw[kind];
   if (LORENTZ) {
-    if (FAST) return x.pre[kind] / x.hw2[kind] * (double)__frcp_rn((float)(z * z + 1.0));
+    if (FAST) return x.pre[kind] / x.hw2[kind] * (double)__builtin_amdgcn_rcpf((float)(z * z + 1.0));
     return ddiv(x.pre[kind], Ed * Ed + x.hw2[kind]);
   }
   if (FAST) return x.pre[kind] * (double)__expf(-(float)(z * z));
@@ -2236,14 +2322,17 @@ __device__ void qw2prop_body(const pss_wire_desc& d, const bool on, const bool n
     const cplx* rho = (const cplx*)(ws + L.rho);
     for (int i = threadIdx.x; i < n; i += NT) {
       const int ix = d.sb_idx[i];
+      double v[5];  // all five evaluations' loads before the stores
+#pragma unroll
       for (int f = 0; f < 5; ++f) {
-        double v = 0.0;
+        v[f] = 0.0;
         if (ix >= 0 && (f < 3 || lf)) {
           const double* y = f < 3 ? (y5 + f * Nr) : (const double*)(rho + (f - 3) * Nr);
-          v = spline_eval(y, f < 3 ? 1 : 2, ws + L.spb + f * Nr, d.sb_h, Nr, ix, d.sb_dx[i]);
+          v[f] = spline_eval(y, f < 3 ? 1 : 2, ws + L.spb + f * Nr, d.sb_h, Nr, ix, d.sb_dx[i]);
         }
-        ws[L.outr + f * n + i] = v;
       }
+#pragma unroll
+      for (int f = 0; f < 5; ++f) ws[L.outr + f * n + i] = v[f];
       if (lf) {
         const double a = fabs(ws[L.outr + 3 * n + i]), b = fabs(ws[L.outr + 4 * n + i]);
         ws[L.outr + 3 * n + i] = a;
