@@ -279,6 +279,15 @@ __device__ __forceinline__ void store_out(const PassArgs& a, cplx* out, int64_t 
   }
 }
 
+// Lines per workgroup of a two-level pass: 256 threads of N2 each, and at most 2048
+// complex128 or 4096 complex64 elements (the LDS transpose tile's budget).  Along axes 0
+// and 1 a workgroup's lines are consecutive in z, so TB elements of one line position are
+// contiguous: 16 complex64 lines fill a 128-byte line where 8 would fill half of it.
+template <int N1, int N2, bool F32>
+constexpr int fft2_tb() {
+  return (256 / N2) < ((F32 ? 4096 : 2048) / (N1 * N2)) ? (256 / N2) : ((F32 ? 4096 : 2048) / (N1 * N2));
+}
+
 // Two-level transform N = N1 x N2 along `axis`: each line is read straight from
 // HBM into registers (N2 threads, N1 points each), N1-point DFTs, inter-level
 // twiddles, one padded LDS transpose, N2-point DFTs, store.  One LDS round trip
@@ -288,8 +297,7 @@ template <int N1, int N2, int MODE, class T, bool SLAB>
 __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz, const int bx) {
   constexpr int N = N1 * N2;
   constexpr int G = N2;
-  constexpr int TB0 = 256 / G, TBL = 2048 / N;
-  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  constexpr int TB = fft2_tb<N1, N2, sizeof(T) == 4>();
   __shared__ cpx<T> Y[TB * N1 * (N2 + 1)];
   __shared__ cpx<T> tws[N / 2];
   __shared__ T kl[N];  // wavenumbers along the pass axis (derivative passes)
@@ -462,23 +470,26 @@ __global__ void __launch_bounds__(256) k_fft2_multi(MultiPass m) {
 }
 
 // workgroups of one two-level pass along `axis` of grid (Nx, Ny, Nz)
-template <int N1, int N2>
+template <int N1, int N2, bool F32>
 int64_t fft2_blocks(int axis, int Nx, int Ny, int Nz) {
-  constexpr int N = N1 * N2, G = N2, TB0 = 256 / G, TBL = 2048 / N;
-  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  constexpr int TB = fft2_tb<N1, N2, F32>();
   if (axis == 2) return ((int64_t)Nx * Ny + TB - 1) / TB;
   return (int64_t)(axis == 1 ? Nx : Ny) * ((Nz + TB - 1) / TB);
 }
 
 template <int N1, int N2>
 void launch_fft2_multi(const MultiPass& m, int np, int nsub, hipStream_t s) {
-  constexpr int N = N1 * N2, G = N2, TB0 = 256 / G, TBL = 2048 / N;
-  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  constexpr int G = N2;
+  const PassArgs& a = m.p[0];  // the stage's flags are common to the sub-passes
+  // the kernel's arithmetic type: FP32 whenever either side is single precision (the slab
+  // variant: its output side)
+  const bool f32 = a.slab ? a.out32 != 0 : (a.in32 || a.out32);
+  const int TB = f32 ? fft2_tb<N1, N2, true>() : fft2_tb<N1, N2, false>();
   int64_t blocks = 0;
   for (int q = 0; q < np; ++q)
-    blocks = std::max(blocks, fft2_blocks<N1, N2>(m.p[q].axis, m.p[q].Nx, m.p[q].Ny, m.p[q].Nz));
+    blocks = std::max(blocks, f32 ? fft2_blocks<N1, N2, true>(m.p[q].axis, m.p[q].Nx, m.p[q].Ny, m.p[q].Nz)
+                                  : fft2_blocks<N1, N2, false>(m.p[q].axis, m.p[q].Nx, m.p[q].Ny, m.p[q].Nz));
   const dim3 gr((unsigned)blocks, (unsigned)nsub), bl(TB * G);
-  const PassArgs& a = m.p[0];  // the stage's flags are common to the sub-passes
   if (a.slab) {
     if (a.out32) {
       if (a.accum == 2) hipLaunchKernelGGL((k_fft2_multi<N1, N2, 2, float, true>), gr, bl, 0, s, m);
@@ -498,10 +509,9 @@ void launch_fft2_multi(const MultiPass& m, int np, int nsub, hipStream_t s) {
 
 template <int N1, int N2>
 void launch_fft2(const PassArgs& a, const pss_grid* g, hipStream_t s, int nb) {
-  constexpr int N = N1 * N2;
   constexpr int G = N2;
-  constexpr int TB0 = 256 / G, TBL = 2048 / N;
-  constexpr int TB = TB0 < TBL ? TB0 : TBL;
+  const bool f32 = a.slab ? a.out32 != 0 : (a.in32 || a.out32);  // as the launches below choose T
+  const int TB = f32 ? fft2_tb<N1, N2, true>() : fft2_tb<N1, N2, false>();
   int64_t blocks;
   if (a.axis == 2) blocks = ((int64_t)g->Nx * g->Ny + TB - 1) / TB;
   else blocks = (int64_t)(a.axis == 1 ? g->Nx : g->Ny) * ((g->Nz + TB - 1) / TB);

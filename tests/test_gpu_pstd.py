@@ -689,7 +689,8 @@ def test_jline_fft_in_kernel(Nx, Nw, keep_old):
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape,axis,in32,curl", [((16, 8, 128), 2, 1, True), ((16, 8, 128), 2, 0, True),
                                                   ((64, 4, 2), 0, 1, False), ((4, 256, 2), 1, 0, False),
-                                                  ((8, 2, 32), 2, 0, False)])
+                                                  ((8, 2, 32), 2, 0, False), ((256, 4, 32), 0, 1, True),
+                                                  ((4, 256, 8), 1, 1, False), ((1024, 2, 8), 0, 1, False)])
 def test_fft_pass_acc_matches_numpy(shape, axis, in32, curl):
     """pss_fft_pass_acc: out_b += FFT_axis(in_b) (complex64 input computed in FP32: 1e-6 of the
     transform; complex128: 1e-14), plus coef i (k x G)_b when G is given (the curl term of the
