@@ -1752,19 +1752,58 @@ __device__ inline double coll_line_gauss(const CollCtx& x, int kind, double base
   const double c = exp(-2.0 * beta * beta);
   const double e0 = exp(-(zp * zp));
   double acc = e0 * (m[jp] * n[jp + s]);
+  // Walking away from the centre each weight is the last one times r < 1, r itself shrinking,
+  // and (1 - n) n <= 1, so once e r / (1 - r) <= 2^-56 of the sum so far the rest of that side
+  // cannot change it beyond rounding: stop there (the line is a few widths across, a few
+  // grid points where the momentum transfer is large; a line whose centre lies far outside
+  // the range has weights that underflow to 0 from the start and stops at once).
+  // The test holds at any point of the walk; it is made once per 8 terms so that the loads
+  // of a group issue together.
+  constexpr double kTail = 0x1p-56;
+  constexpr int G = 8;
   double e = e0, r = exp(-(2.0 * zp * beta + beta * beta));
-  for (int j = jp + 1; j <= hi; ++j) {
-    e *= r;
-    r *= c;
-    acc = fma(e, m[j] * n[j + s], acc);
+  int j = jp + 1;
+  bool done = false;
+  for (; j + G - 1 <= hi; j += G) {
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      e *= r;
+      r *= c;
+      acc = fma(e, m[j + u] * n[j + u + s], acc);
+    }
+    if (r < 1.0 && e * r <= kTail * (1.0 - r) * acc) {
+      done = true;
+      break;
+    }
   }
+  if (!done)
+    for (; j <= hi; ++j) {
+      e *= r;
+      r *= c;
+      acc = fma(e, m[j] * n[j + s], acc);
+    }
   e = e0;
   r = exp(2.0 * zp * beta - beta * beta);
-  for (int j = jp - 1; j >= lo; --j) {
-    e *= r;
-    r *= c;
-    acc = fma(e, m[j] * n[j + s], acc);
+  j = jp - 1;
+  done = false;
+  for (; j - G + 1 >= lo; j -= G) {
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      e *= r;
+      r *= c;
+      acc = fma(e, m[j - u] * n[j - u + s], acc);
+    }
+    if (r < 1.0 && e * r <= kTail * (1.0 - r) * acc) {
+      done = true;
+      break;
+    }
   }
+  if (!done)
+    for (; j >= lo; --j) {
+      e *= r;
+      r *= c;
+      acc = fma(e, m[j] * n[j + s], acc);
+    }
   return x.pre[kind] * acc;
 }
 
