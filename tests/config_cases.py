@@ -145,9 +145,9 @@ def coupled_record(fields, wire_state, P, Rho, seed):
     for i, n in enumerate(FIELDS):
         rec["sk_" + n] = sk[i]
         rec["norm_" + n] = np.linalg.norm(fields[n])
-    if fields["Ex"].size <= 1 << 16:  # 1-D lines: the fields themselves
+    if fields["Ex"].size <= 1 << 16:  # 1-D lines: the fields themselves (copies: the run goes on)
         for n in FIELDS:
-            rec["f_" + n] = np.asarray(fields[n]).reshape(-1)
+            rec["f_" + n] = np.array(fields[n], copy=True).reshape(-1)
     sk = sketch([wire_state["p2"], wire_state["C2"], wire_state["D2"]], seed + 1)
     for i, n in enumerate(("p2", "C2", "D2")):
         rec["sk_" + n] = sk[i]
