@@ -34,7 +34,7 @@ __device__ inline double ddiv(double n, double d) {
 // ---------------------------------------------------------------- workspace
 struct WS {
   int64_t er, spa, eq, vc, e1d, ne2, nh2, tr, gam, De, Dh, H, W, n3, red, ds, pr, rho, spb, outr, cp, dcc, rsp, sp,
-      pk, xp, total_fp64, total;
+      pk, total_fp64, total;
 };
 
 __host__ __device__ inline int64_t al(int64_t n) { return (n + 31) / 32 * 32; }
@@ -71,7 +71,6 @@ __host__ __device__ inline WS ws_layout(int Nk, int Nr, int nrr) {
   w.pk = o;   o += al(ksm > 1 ? 6 * (int64_t)ksm * Nk * Nk : 0);  // split-K partial tiles
   w.total_fp64 = o;
   w.sp = o;   o += al(16 * np * np);  // mixed precision only: FP16 operand planes
-  w.xp = o;   o += al(8 * np * np);   // mixed precision only: exchange planes (X~ 24 NP^2, T 6 NP^2 halves)
   w.total = o;
   return w;
 }
@@ -289,7 +288,6 @@ __device__ void prep_body(const pss_wire_desc& d, double* sh) {
   double n1 = s / 2.0 / d.L;
   n1 = fmin(n1, d.density_max);
   const bool scr = d.flags & PSS_SCREENED;
-  double vmax[3] = {0.0, 0.0, 0.0};
   for (int q = threadIdx.x; q < Nk; q += 256) {
     double e1 = 1.0;
     if (scr) {
@@ -300,13 +298,10 @@ __device__ void prep_body(const pss_wire_desc& d, double* sh) {
     for (int n = 0; n < 3; ++n) {
       const double v = scr ? d.V0[n * Nk + q] / e1 : d.V0[n * Nk + q];
       ws[L.vc + n * Nk + q] = v;
-      if (q > 0) vmax[n] = fmax(vmax[n], fabs(v));
     }
   }
-  block_max_n<256, 3>(vmax, sh);
   if (threadIdx.x == 0) ws[L.red + 0] = n1;
   if (threadIdx.x < 8) ws[L.red + 8 + threadIdx.x] = 0.0;  // max-abs slots of the mixed path
-  if (threadIdx.x < 3) ws[L.red + 16 + threadIdx.x] = vmax[threadIdx.x];  // max V(|q|), q != 0
 }
 
 // -------------------------------------------------- K1: Prop2QW (rr -> r)
@@ -706,7 +701,7 @@ __global__ void __launch_bounds__(256) k_calc_h(const pss_wire_desc* __restrict_
   if (d.flags & PSS_EXCITONS) {
     const bool lf = d.flags & PSS_LF;
     // the exchange sums sum_{q != 0} V(|q|) X^T[k1+q, k2+q] of Heh (and of Hee, Hhh with
-    // LF) are added by k_exch; without LF Hee / Hhh keep only their diagonal, summed here
+    // LF) are added by k_exch_fft; without LF Hee / Hhh keep only their diagonal, summed here
     if (!lf && k1 == k2) {
       cplx b = mk(0, 0), c = mk(0, 0);
       for (int q = -k1; q <= Nk - 1 - k1; ++q) {
@@ -763,100 +758,261 @@ __host__ __device__ inline int wire_ks(int Nk, int knob, int cap, int KS) {
   return ks < KS ? ks : KS;
 }
 
-// ------------------------------- K5b: exchange sums as Toeplitz MFMA products
+// ------------------------------- K5b: exchange sums as FFT convolutions along diagonals
 // For a matrix X (p, C or D) and potential row V, the exchange term of CalcH
 // (PSTD3D/SBEs.py:820-907) is  E[k1][k2] = sum_{q != 0} V(|q|) X[k2+q][k1+q].
-// Along a diagonal d = k1 - k2 it is a Toeplitz product: with x_d[u] = X[u-d][u]
-// (zero where u - d leaves [0, Nk)),  E[k1][k1-d] = sum_u T[k1][u] x_d[u],
-// T[k1][u] = V(|u - k1|) (u != k1).  Every diagonal shares T, so the diagonals
-// are the columns of one matrix and E is the real x complex product
-//   Y[k1][dd] = sum_u T[k1][u] X~[u][dd],  X~[u][dd] = X[u-d][u], d = dd - (Nk-1),
-// on v_mfma_f64_16x16x4f64.  A workgroup owns 64 k1 x 64 dd; T fragments come
-// from the potential row in LDS, X~ is gathered row by row of X (contiguous u)
-// into LDS per 16-wide u chunk, and chunks where the parallelogram is empty
-// are skipped.  m = 0: Heh += Y (from p, V_eh); with LF m = 1: Hee -= Y (C,
-// V_ee), m = 2: Hhh -= Y (D, V_hh).  Tiles of 64 (or 32 when few tiles would fill
-// the chip); grid (ceil((2Nk-1)/T), ceil(Nk/T), 3 nwires).
-template <int NB>  // 16 x 16 blocks per wave and dimension: tile = 32 NB x 32 NB
-__global__ void __launch_bounds__(256) k_exch(const pss_wire_desc* __restrict__ descs) {
-  constexpr int TT = 32 * NB;
-  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
+// Along a diagonal d = k1 - k2 it is a symmetric Toeplitz product: with
+// x_d[s] = X[r][r+d], r = s + max(0, -d), s in [0, n), n = Nk - |d|,
+//   E[t + max(0, d)][t + max(0, -d)] = sum_{s != t} V(|s - t|) x_d[s],
+// the linear convolution of x_d with c[j] = V(|j|) (0 < |j| < Nk, c[0] = 0).  On a
+// line of M >= 2 Nk - 1 points (a power of two) with x_d zero-padded and c wrapped
+// (c[M - j] = c[j]) the circular convolution equals it on [0, n), so
+//   E_d = IFFT(FFT(c) FFT(x_d)),  FFT(c) real (c is real and even):
+// O(M log M) per diagonal instead of O(Nk^2), the whole exchange O(Nk^2 log Nk).
+// A workgroup holds LN lines of M complex points in LDS: line 0 the kernel c, lines
+// 1 .. LN-1 consecutive diagonals.  The transforms are in-place Stockham autosort
+// steps of radix 8 (the last step radix 4 or 2; Govindaraju et al., "High performance
+// discrete Fourier transforms on graphics processors", SC 2008), each thread holding 8
+// points per step; the inverse is conj(FFT(conj(.))) / M.  X is read and H updated row
+// by row over the block's diagonals (contiguous elements).  m = 0: Heh += E (p, V_eh);
+// with LF m = 1: Hee -= E (C, V_ee), m = 2: Hhh -= E (D, V_hh).  FP64 in both
+// precisions.  Grid (ceil((2 Nk - 1) / (LN - 1)), 3, nwires), LN M / 8 threads for the
+// launch's largest M, dynamic LDS (LN (9 M / 8 + 1) + exch_tw(M)) complex values (at most
+// 153 KiB).
+__host__ __device__ inline int exch_m(int Nk) {
+  int M = 8;
+  while (M < 2 * Nk - 1) M <<= 1;
+  return M;
+}
+__host__ __device__ inline int exch_lines(int M) { return M <= 128 ? 16 : M <= 256 ? 8 : M <= 2048 ? 4 : 2; }
+// Twiddles: a step with span Ns = 8^a > 1 and radix R reads w = W_{Ns R}^k, k < Ns, from
+// its own table at offset (Ns - 8) / 7 (consecutive butterflies read consecutive entries);
+// the tables of all steps hold exch_tw(M) entries.
+__host__ __device__ inline int exch_tw(int M) {
+  int n = 0;
+  for (int Ns = 8; Ns < M; Ns *= 8) n += Ns;
+  return n;
+}
+
+// forward DFTs of 2, 4 and 8 points in registers (W = exp(-2 pi i / R))
+__device__ __forceinline__ void dft2(cplx* v) {
+  const cplx a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+__device__ __forceinline__ void dft4(cplx& a0, cplx& a1, cplx& a2, cplx& a3) {
+  const cplx t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, u = a1 - a3;
+  const cplx t3 = mk(u.y, -u.x);  // (a1 - a3) (-i)
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+__device__ __forceinline__ void dft8(cplx* v) {
+  constexpr double h = 0.70710678118654752440;  // 1 / sqrt(2)
+  cplx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cplx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  const cplx w1 = mk(h * (o1.x + o1.y), h * (o1.y - o1.x));   // W8 o1
+  const cplx w2 = mk(o2.y, -o2.x);                             // W8^2 o2 = -i o2
+  const cplx w3 = mk(h * (o3.y - o3.x), -h * (o3.x + o3.y));  // W8^3 o3
+  v[0] = e0 + o0; v[4] = e0 - o0;
+  v[1] = e1 + w1; v[5] = e1 - w1;
+  v[2] = e2 + w2; v[6] = e2 - w2;
+  v[3] = e3 + w3; v[7] = e3 - w3;
+}
+
+// LDS position of point i of a line: one pad slot per 8 points, so that the stride-R
+// stores of a step spread over the banks
+__device__ __forceinline__ int xpad(int i) { return i + (i >> 3); }
+
+// LDS points are 16-byte aligned (ds_read_b128 / ds_write_b128: one bank group per lane)
+struct __align__(16) lcplx {
+  double x, y;
+};
+__device__ __forceinline__ cplx lget(const lcplx& a) { return mk(a.x, a.y); }
+__device__ __forceinline__ void lput(lcplx& a, cplx v) { a = lcplx{v.x, v.y}; }
+
+// One Stockham step of radix R on a line of M = 2^lm points: butterfly j in [0, M/R)
+// takes x[j + r M/R], multiplies by w^r, w = W_M^{(j mod Ns) M/(Ns R)}, transforms, and
+// stores at (j / Ns) Ns R + (j mod Ns) + r Ns.  Thread t < M/8 owns butterflies
+// t + u M/8, u < 8/R.  One table read per butterfly; w^2 .. w^7 by products.
+template <int R>
+__device__ __forceinline__ void st_load(const lcplx* ln, int M, int t, cplx* v) {
+#pragma unroll
+  for (int u = 0; u < 8 / R; ++u)
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[u * R + r] = lget(ln[xpad(t + u * (M / 8) + r * (M / R))]);
+}
+template <int R>
+__device__ __forceinline__ void st_store(lcplx* ln, int lm, int Ns, int t, cplx* v, const lcplx* tq) {
+  constexpr int LR = R == 8 ? 3 : R == 4 ? 2 : 1;
+#pragma unroll
+  for (int u = 0; u < 8 / R; ++u) {
+    const int j = t + u * (1 << (lm - 3)), k = j & (Ns - 1);
+    cplx* w = v + u * R;
+    if (Ns > 1) {
+      const cplx w1 = lget(tq[(Ns - 8) / 7 + k]);  // W_{Ns R}^k
+      w[1] = w[1] * w1;
+      if (R >= 4) {
+        const cplx w2 = w1 * w1, w3 = w2 * w1;
+        w[2] = w[2] * w2;
+        w[3] = w[3] * w3;
+        if (R == 8) {
+          const cplx w4 = w2 * w2;
+          w[4] = w[4] * w4;
+          w[5] = w[5] * (w4 * w1);
+          w[6] = w[6] * (w3 * w3);
+          w[7] = w[7] * (w4 * w3);
+        }
+      }
+    }
+    if (R == 8) dft8(w);
+    else if (R == 4) dft4(w[0], w[1], w[2], w[3]);
+    else dft2(w);
+    const int base = ((j - k) << LR) + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lput(ln[xpad(base + r * Ns)], w[r]);
+  }
+}
+
+// Ordering of a step's LDS reads before its writes (and of the writes before the next
+// step's reads): a block barrier, or, when each line lies in one wavefront (M <= 512), a
+// wavefront-scope fence (the wave's LDS operations complete in order).
+template <bool WAVE>
+__device__ __forceinline__ void lines_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// forward transform of the lines [l0, LN) in place (line l at sm + l MP); every thread of
+// the block calls it
+template <bool WAVE>
+__device__ __forceinline__ void lines_fft(lcplx* sm, const lcplx* tq, int lm, int MP, int LN, int l0) {
+  const int M = 1 << lm;
+  const int l = (int)threadIdx.x >> (lm - 3), t = (int)threadIdx.x & ((M >> 3) - 1);
+  const bool act = l >= l0 && l < LN;
+  lcplx* ln = sm + (int64_t)l * MP;
+  cplx v[8];
+  for (int lns = 0; lns < lm;) {
+    const int rem = lm - lns;
+    const int R = rem >= 3 ? 8 : rem == 2 ? 4 : 2;
+    const int Ns = 1 << lns;
+    if (act) {
+      if (R == 8) st_load<8>(ln, M, t, v);
+      else if (R == 4) st_load<4>(ln, M, t, v);
+      else st_load<2>(ln, M, t, v);
+    }
+    lines_sync<WAVE>();
+    if (act) {
+      if (R == 8) st_store<8>(ln, lm, Ns, t, v, tq);
+      else if (R == 4) st_store<4>(ln, lm, Ns, t, v, tq);
+      else st_store<2>(ln, lm, Ns, t, v, tq);
+    }
+    lines_sync<WAVE>();
+    lns += R == 8 ? 3 : R == 4 ? 2 : 1;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_exch_fft(const pss_wire_desc* __restrict__ descs, int LN) {
+  const int wire = blockIdx.z, m = blockIdx.y;
   const pss_wire_desc& d = descs[wire];
-  if (!wire_on(d) || !(d.flags & PSS_EXCITONS) || d.precision == PSS_MIXED) return;
+  if (!wire_on(d) || !(d.flags & PSS_EXCITONS)) return;
   if (m > 0 && !(d.flags & PSS_LF)) return;
-  const int Nk = d.Nk, nd = 2 * Nk - 1;
-  const int dd0 = blockIdx.x * TT, k10 = blockIdx.y * TT;
-  if (dd0 >= nd || k10 >= Nk) return;
+  const int Nk = d.Nk, nd = 2 * Nk - 1, M = exch_m(Nk), D = LN - 1;
+  const int lm = __builtin_ctz(M), MP = M + (M >> 3) + 1;  // odd stride: lines start on different banks
+  const int od0 = blockIdx.x * D;
+  if (od0 >= nd) return;
   const WS L = ws_layout(Nk, d.Nr, d.nrr);
-  double* ws = d.ws;
+  const double* ws = d.ws;
   const Bufs B = bufs(d);
   const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : B.D2;
-  extern __shared__ double vrow[];  // Nk doubles: V(|q|) of this matrix
-  __shared__ double Br[16][TT + 1], Bi[16][TT + 1];
-  for (int i = threadIdx.x; i < Nk; i += 256) vrow[i] = ws[L.vc + (int64_t)m * Nk + i];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
-  v4d accr[NB][NB], acci[NB][NB];
-  for (int a = 0; a < NB; ++a)
-    for (int b = 0; b < NB; ++b) { accr[a][b] = v4d{0, 0, 0, 0}; acci[a][b] = v4d{0, 0, 0, 0}; }
-  // u range with any nonzero X~[u][dd], dd in [dd0, dd0 + TT): u - d in [0, Nk)
-  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + TT - 1, nd - 1) - (Nk - 1);
-  const int ulo = max(0, dlo), uhi = min(Nk - 1, Nk - 1 + dhi);
-  for (int u0 = (ulo / 16) * 16; u0 <= uhi; u0 += 16) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < 16 * TT; e += 256) {
-      Br[e / TT][e % TT] = 0.0;
-      Bi[e / TT][e % TT] = 0.0;
-    }
-    __syncthreads();
-    // rows r = u - d of X touched by this chunk: r in [u0 - dhi, u0 + 15 - dlo]
-    const int r_lo = max(0, u0 - dhi), r_hi = min(Nk - 1, u0 + 15 - dlo);
-    const int nslots = (r_hi - r_lo + 1) * 16;
-    for (int sidx = threadIdx.x; sidx < nslots; sidx += 256) {
-      const int r = r_lo + (sidx >> 4), u = u0 + (sidx & 15);
-      if (u >= Nk) continue;
-      const int ddl = u - r + (Nk - 1) - dd0;
-      if (ddl < 0 || ddl >= TT || dd0 + ddl >= nd) continue;
-      const cplx v = X[(int64_t)r * Nk + u];
-      Br[u - u0][ddl] = v.x;
-      Bi[u - u0][ddl] = v.y;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int uu = u0 + ks * 4 + (lane >> 4);  // A: lane -> T[k1 = row l%16][u = l/16]
-      double ta[NB];
-      for (int a = 0; a < NB; ++a) {
-        const int k1 = k10 + wr * 16 * NB + a * 16 + (lane & 15);
-        const int q = uu - k1;
-        ta[a] = (q == 0 || uu >= Nk || k1 >= Nk) ? 0.0 : vrow[q < 0 ? -q : q];
-      }
-      double br[NB], bi[NB];
-      for (int b = 0; b < NB; ++b) {  // B: lane -> X~[u = l/16][dd = l%16]
-        br[b] = Br[ks * 4 + (lane >> 4)][wc * 16 * NB + b * 16 + (lane & 15)];
-        bi[b] = Bi[ks * 4 + (lane >> 4)][wc * 16 * NB + b * 16 + (lane & 15)];
-      }
-      for (int a = 0; a < NB; ++a)
-        for (int b = 0; b < NB; ++b) {
-          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], br[b], accr[a][b], 0, 0, 0);
-          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ta[a], bi[b], acci[a][b], 0, 0, 0);
-        }
+  const double* V = ws + L.vc + (int64_t)m * Nk;
+  extern __shared__ lcplx sm[];  // LN lines of MP, then the steps' twiddle tables
+  lcplx* tq = sm + (int64_t)LN * MP;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int Ns = 8; Ns < M; Ns *= 8) {
+    const int R = M / Ns >= 8 ? 8 : M / Ns;
+    for (int k = tid; k < Ns; k += nt) {
+      double sn, cs;
+      sincospi(2.0 * k / (Ns * R), &sn, &cs);
+      lput(tq[(Ns - 8) / 7 + k], mk(cs, -sn));
     }
   }
-  // D element v of lane l: row (l >> 4) + 4 v, column l & 15
-  cplx* H = (cplx*)(ws + L.H) + (int64_t)m * Nk * Nk;
-  const double sg = m == 0 ? 1.0 : -1.0;
-  for (int a = 0; a < NB; ++a)
-    for (int b = 0; b < NB; ++b)
-      for (int v = 0; v < 4; ++v) {
-        const int k1 = k10 + wr * 16 * NB + a * 16 + (lane >> 4) + 4 * v;
-        const int dd = dd0 + wc * 16 * NB + b * 16 + (lane & 15);
-        if (k1 >= Nk || dd >= nd) continue;
-        const int k2 = k1 - (dd - (Nk - 1));
-        if (k2 < 0 || k2 >= Nk) continue;
-        cplx& h = H[(int64_t)k1 * Nk + k2];
-        h = h + mk(sg * accr[a][b][v], sg * acci[a][b][v]);
+  // line 0: c; lines 1..D: zero, then the diagonals' elements row by row
+  for (int e = tid; e < LN * M; e += nt) {
+    const int l = e >> lm, s = e & (M - 1);
+    cplx v = mk(0, 0);
+    if (l == 0) {
+      const int j = s <= M / 2 ? s : M - s;
+      if (j >= 1 && j < Nk) v = mk(V[j], 0.0);
+    }
+    lput(sm[(int64_t)l * MP + xpad(s)], v);
+  }
+  __syncthreads();
+  // thread -> (diagonal l, rows r0, r0 + rs, ...): consecutive threads read consecutive
+  // columns of one row
+  const int ld = tid % D, r0 = tid / D, rs = nt / D;
+  const int od = od0 + ld, dg = od - (Nk - 1);
+  const bool lon = r0 < rs && od < nd;
+  lcplx* dl = sm + (int64_t)(ld + 1) * MP;
+  // four rows' loads in flight together, then their LDS stores
+  constexpr int U = 4;
+  if (lon)
+    for (int rb = r0; rb < Nk; rb += U * rs) {
+      cplx v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = rb + u * rs, col = r + dg;
+        v[u] = (r < Nk && col >= 0 && col < Nk) ? X[(int64_t)r * Nk + col] : mk(0, 0);
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = rb + u * rs, col = r + dg;
+        if (r < Nk && col >= 0 && col < Nk) lput(dl[xpad(r < col ? r : col)], v[u]);
+      }
+    }
+  __syncthreads();
+  if (M <= 512) lines_fft<true>(sm, tq, lm, MP, LN, 0);
+  else lines_fft<false>(sm, tq, lm, MP, LN, 0);
+  __syncthreads();
+  // conj(x^ c^) / M: the inverse as a forward transform of the conjugate
+  const double iM = 1.0 / M;
+  for (int e = M + tid; e < LN * M; e += nt) {
+    const int l = e >> lm, s = e & (M - 1);
+    const double c = sm[xpad(s)].x * iM;
+    lcplx& x = sm[(int64_t)l * MP + xpad(s)];
+    x = lcplx{x.x * c, -x.y * c};
+  }
+  __syncthreads();
+  if (M <= 512) lines_fft<true>(sm, tq, lm, MP, LN, 1);
+  else lines_fft<false>(sm, tq, lm, MP, LN, 1);
+  __syncthreads();
+  cplx* H = (cplx*)(d.ws + L.H) + (int64_t)m * Nk * Nk;
+  const double sg = m == 0 ? 1.0 : -1.0;
+  if (lon)
+    for (int kb = r0; kb < Nk; kb += U * rs) {
+      cplx hv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k1 = kb + u * rs, k2 = k1 - dg;
+        hv[u] = (k1 < Nk && k2 >= 0 && k2 < Nk) ? H[(int64_t)k1 * Nk + k2] : mk(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k1 = kb + u * rs, k2 = k1 - dg;
+        if (k1 < Nk && k2 >= 0 && k2 < Nk) {
+          const cplx y = lget(dl[xpad(k1 < k2 ? k1 : k2)]);
+          H[(int64_t)k1 * Nk + k2] = hv[u] + mk(sg * y.x, -sg * y.y);  // conj of the forward transform
+        }
+      }
+    }
 }
 
 // ------------------------------------------- K6: RHS GEMMs + leapfrog epilogue
@@ -1415,221 +1571,6 @@ __global__ void __launch_bounds__(256) k_rhs_mx(const pss_wire_desc* __restrict_
         const cplx S = mk(accr[x][y][v], acci[x][y][v]);
         if (ks == 1) mixed_epilogue(d, B, H, gam, out, i, j, S, deriv);
         else pk[(int64_t)i * Nk + j] = S;
-      }
-}
-
-// ---- mixed-precision exchange operands.  For each exchange matrix m (p with V_eh;
-// with LF also C with V_ee and D with V_hh) k_xsplit writes the FP16 hi/lo planes
-//   X~ (re hi, re lo, im hi, im lo), [dd][u] = X[u - d][u] * sX, d = dd - (Nk - 1),
-//   T (hi, lo),                       [k1][u] = V(|u - k1|) * sT  (0 at u == k1),
-// zero outside the matrix (dd < 2 NP, u and k1 < NP), sX and sT powers of two from
-// max |X| (red slot 8 + m) and max V (red slot 16 + m).  X~ tiles of 32 dd x 64 u:
-// the parallelogram's 95 source rows are read row by row into LDS, split, and
-// written out along u.  Grid (2 NP / 32, NP / 64 + 1, 3 nwires); the last row of
-// blocks writes T, 32 rows k1 per block.
-__device__ inline _Float16* xt_plane(double* ws, const WS& L, int NP, int m, int q) {
-  return (_Float16*)(ws + L.xp) + (int64_t)(m * 4 + q) * 2 * NP * NP;
-}
-__device__ inline _Float16* tp_plane(double* ws, const WS& L, int NP, int m, int q) {
-  return (_Float16*)(ws + L.xp) + (int64_t)24 * NP * NP + (int64_t)(m * 2 + q) * NP * NP;
-}
-
-__device__ inline bool exch_active(const pss_wire_desc& d, int m) {
-  return wire_on(d) && (d.flags & PSS_EXCITONS) && d.precision == PSS_MIXED && (m == 0 || (d.flags & PSS_LF));
-}
-
-__global__ void __launch_bounds__(256) k_xsplit(const pss_wire_desc* __restrict__ descs) {
-  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
-  const pss_wire_desc& d = descs[wire];
-  if (!exch_active(d, m)) return;
-  const int Nk = d.Nk, NP = np64(Nk);
-  const WS L = ws_layout(Nk, d.Nr, d.nrr);
-  double* ws = d.ws;
-  if ((int)blockIdx.y == (int)gridDim.y - 1) {  // T planes
-    const int r0 = blockIdx.x * 32;
-    if (r0 >= NP) return;
-    const double sT = pow2_scale(ws[L.red + 16 + m]);
-    const double* vrow = ws + L.vc + (int64_t)m * Nk;
-    _Float16* th = tp_plane(ws, L, NP, m, 0);
-    _Float16* tl = tp_plane(ws, L, NP, m, 1);
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int rr = wv; rr < 32; rr += 4) {
-      const int k1 = r0 + rr;
-      for (int u4 = lane * 4; u4 < NP; u4 += 256) {
-        h4 hi, lo;
-        #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int u = u4 + j, q = u - k1;
-          const double t = (q == 0 || u >= Nk || k1 >= Nk) ? 0.0 : vrow[q < 0 ? -q : q] * sT;
-          _Float16 a, b;
-          split16(t, a, b);
-          hi[j] = a;
-          lo[j] = b;
-        }
-        *(h4*)&th[(int64_t)k1 * NP + u4] = hi;
-        *(h4*)&tl[(int64_t)k1 * NP + u4] = lo;
-      }
-    }
-    return;
-  }
-  const int dd0 = blockIdx.x * 32, u0 = blockIdx.y * 64;
-  if (dd0 >= 2 * NP || u0 >= NP) return;
-  const Bufs B = bufs(d);
-  const cplx* X = m == 0 ? B.p2 : m == 1 ? B.C2 : B.D2;
-  const double sX = pow2_scale(ws[L.red + 8 + m]);
-  // source rows r = u - d for dd in [dd0, dd0 + 32), u in [u0, u0 + 64): r = rb + rr, rr < 95
-  const int rb = u0 - (dd0 + 31 - (Nk - 1));
-  __shared__ _Float16 S[4][95][64];
-  // unrolled so that a thread's ~24 row loads are in flight together (the loop body's
-  // only other memory traffic is LDS)
-#pragma unroll 8
-  for (int e = threadIdx.x; e < 95 * 64; e += 256) {
-    const int rr = e >> 6, uu = e & 63;
-    const int r = rb + rr, u = u0 + uu;
-    cplx v = mk(0, 0);
-    if (r >= 0 && r < Nk && u < Nk) v = X[(int64_t)r * Nk + u];
-    _Float16 h0, h1, h2, h3;
-    split16(v.x * sX, h0, h1);
-    split16(v.y * sX, h2, h3);
-    S[0][rr][uu] = h0;
-    S[1][rr][uu] = h1;
-    S[2][rr][uu] = h2;
-    S[3][rr][uu] = h3;
-  }
-  __syncthreads();
-  // element (dd0 + ddl, u0 + uu) is S[.][uu + 31 - ddl][uu]: zero where r or u leaves the matrix
-  for (int e = threadIdx.x; e < 32 * 16; e += 256) {
-    const int ddl = e >> 4, u4 = (e & 15) * 4;
-    #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      h4 v;
-      #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = S[q][u4 + j + 31 - ddl][u4 + j];
-      *(h4*)&xt_plane(ws, L, NP, m, q)[(int64_t)(dd0 + ddl) * NP + u0 + u4] = v;
-    }
-  }
-}
-
-// ---- mixed-precision exchange: Y[k1][dd] = sum_u T[k1][u] X~[u][dd] of k_exch as
-// a GEMM over the k_xsplit planes, 64 x 64 output tiles (k1 x dd), 4 waves of 32 x
-// 32, 32-wide u stages staged in LDS as in k_rhs_mx; re and im each Th Xh + Th Xl +
-// Tl Xh on v_mfma_f32_16x16x32_f16, FP32 partials folded into FP64 every 4 stages.
-// Only the stages that hold the tile's parallelogram u range are visited.  The
-// epilogue adds +-Y to H on the diagonal k2 = k1 - d.  Grid (2 NP / 64, NP / 64, 3 nwires).
-__global__ void __launch_bounds__(256) k_exch_mx(const pss_wire_desc* __restrict__ descs) {
-  const int wire = blockIdx.z / 3, m = blockIdx.z % 3;
-  const pss_wire_desc& d = descs[wire];
-  if (!exch_active(d, m)) return;
-  const int Nk = d.Nk, nd = 2 * Nk - 1, NP = np64(Nk);
-  const int dd0 = blockIdx.x * 64, k10 = blockIdx.y * 64;
-  if (dd0 >= nd || k10 >= Nk) return;
-  const WS L = ws_layout(Nk, d.Nr, d.nrr);
-  double* ws = d.ws;
-  const double fold_f = 1.0 / (pow2_scale(ws[L.red + 16 + m]) * pow2_scale(ws[L.red + 8 + m]));
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
-  __shared__ _Float16 As[2][64][kLdk];
-  __shared__ _Float16 Bs[4][64][kLdk];
-  double accr[2][2][4], acci[2][2][4];
-  f4 fr[2][2], fi[2][2];
-  #pragma unroll
-  for (int a = 0; a < 2; ++a)
-    #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      fr[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-      fi[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-      #pragma unroll
-      for (int v = 0; v < 4; ++v) { accr[a][b][v] = 0.0; acci[a][b][v] = 0.0; }
-    }
-  int nacc = 0;
-  auto fold = [&]() {
-    #pragma unroll
-    for (int a = 0; a < 2; ++a)
-      #pragma unroll
-      for (int b = 0; b < 2; ++b)
-        #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          accr[a][b][v] += (double)fr[a][b][v] * fold_f;
-          acci[a][b][v] += (double)fi[a][b][v] * fold_f;
-          fr[a][b][v] = 0.f;
-          fi[a][b][v] = 0.f;
-        }
-    nacc = 0;
-  };
-  // u range with any nonzero X~[dd][u], dd in the tile: u - d in [0, Nk)
-  const int dlo = dd0 - (Nk - 1), dhi = min(dd0 + 63, nd - 1) - (Nk - 1);
-  const int ulo = max(0, dlo), uhi = min(Nk - 1, Nk - 1 + dhi);
-  const int s0 = ulo / kKs, s1 = uhi / kKs + 1;
-  // loader: quarter t & 3 of row t >> 2 of both T planes and all four X~ planes
-  const int lrow = threadIdx.x >> 2, lqt = threadIdx.x & 3;
-  const _Float16* ga = tp_plane(ws, L, NP, m, 0) + (int64_t)(k10 + lrow) * NP + 8 * lqt;
-  const _Float16* gb = xt_plane(ws, L, NP, m, 0) + (int64_t)(dd0 + lrow) * NP + 8 * lqt;
-  const int64_t PL = (int64_t)NP * NP;
-  uint4 pa0, pa1, pb0, pb1, pb2, pb3;  // named registers, not an array
-  auto prefetch = [&](int st) {
-    const int k0 = st * kKs;
-    pa0 = *(const uint4*)(ga + k0);
-    pa1 = *(const uint4*)(ga + PL + k0);
-    pb0 = *(const uint4*)(gb + k0);
-    pb1 = *(const uint4*)(gb + 2 * PL + k0);
-    pb2 = *(const uint4*)(gb + 4 * PL + k0);
-    pb3 = *(const uint4*)(gb + 6 * PL + k0);
-  };
-  if (s0 < s1) prefetch(s0);
-  for (int st = s0; st < s1; ++st) {
-    *(uint4*)&As[0][lrow][8 * lqt] = pa0;
-    *(uint4*)&As[1][lrow][8 * lqt] = pa1;
-    *(uint4*)&Bs[0][lrow][8 * lqt] = pb0;
-    *(uint4*)&Bs[1][lrow][8 * lqt] = pb1;
-    *(uint4*)&Bs[2][lrow][8 * lqt] = pb2;
-    *(uint4*)&Bs[3][lrow][8 * lqt] = pb3;
-    __syncthreads();
-    if (st + 1 < s1) prefetch(st + 1);
-    {
-      const int li = lane & 15, kq = (lane >> 4) * 8;
-      h8 a[2][2], b[2][4];
-      #pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        a[x][0] = *(const h8*)&As[0][wr * 32 + x * 16 + li][kq];
-        a[x][1] = *(const h8*)&As[1][wr * 32 + x * 16 + li][kq];
-        #pragma unroll
-        for (int q = 0; q < 4; ++q) b[x][q] = *(const h8*)&Bs[q][wc * 32 + x * 16 + li][kq];
-      }
-      #pragma unroll
-      for (int x = 0; x < 2; ++x)
-        #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          f4 r = fr[x][y], im = fi[x][y];
-          r = mfma32(a[x][0], b[y][0], r);
-          r = mfma32(a[x][0], b[y][1], r);
-          r = mfma32(a[x][1], b[y][0], r);
-          im = mfma32(a[x][0], b[y][2], im);
-          im = mfma32(a[x][0], b[y][3], im);
-          im = mfma32(a[x][1], b[y][2], im);
-          fr[x][y] = r;
-          fi[x][y] = im;
-        }
-    }
-    if (++nacc == kFold) fold();
-    __syncthreads();
-  }
-  fold();
-  // result element (x, y, v): row k1 = k10 + 32 wr + 16 x + 4 (lane / 16) + v, column dd0 + 32 wc + 16 y + lane % 16
-  cplx* H = (cplx*)(ws + L.H) + (int64_t)m * Nk * Nk;
-  const double sg = m == 0 ? 1.0 : -1.0;
-  #pragma unroll
-  for (int x = 0; x < 2; ++x)
-    #pragma unroll
-    for (int y = 0; y < 2; ++y)
-      #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int k1 = k10 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
-        const int dd = dd0 + wc * 32 + y * 16 + (lane & 15);
-        if (k1 >= Nk || dd >= nd) continue;
-        const int k2 = k1 - (dd - (Nk - 1));
-        if (k2 < 0 || k2 >= Nk) continue;
-        cplx& h = H[(int64_t)k1 * Nk + k2];
-        h = h + mk(sg * accr[x][y][v], sg * acci[x][y][v]);
       }
 }
 
@@ -2581,23 +2522,16 @@ static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_
   if (stages & ST_CALCH) {
     const int t16 = (max_Nk + 15) / 16;
     hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
-    const int tn = (2 * max_Nk - 1 + 63) / 64, tm = (max_Nk + 63) / 64;
-    const size_t lds = (size_t)max_Nk * sizeof(double);
-    // each kernel below returns at once for wires of the other precision; launched only
-    // when the batch holds wires of its own
-    if (any64) {
-      if ((int64_t)tn * tm * 3 * nwires >= 512)
-        hipLaunchKernelGGL((k_exch<2>), dim3(tn, tm, 3 * nwires), dim3(256), lds, s, d_descs);
-      else
-        hipLaunchKernelGGL((k_exch<1>), dim3((2 * max_Nk - 1 + 31) / 32, (max_Nk + 31) / 32, 3 * nwires),
-                           dim3(256), lds, s, d_descs);
+    // exchange of both precisions: FFT convolutions along the diagonals (FP64)
+    const int M = exch_m(max_Nk), LN = exch_lines(M);
+    const size_t lds_x = ((size_t)LN * (M + M / 8 + 1) + exch_tw(M)) * 16;
+    static size_t lds_x_granted = 65536;
+    if (lds_x > lds_x_granted) {
+      PSS_CHECK(hipFuncSetAttribute((const void*)k_exch_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_x));
+      lds_x_granted = lds_x;
     }
-    // mixed wires: max |p|, |C|, |D| (taken with K3) -> exchange planes -> exchange GEMM
-    const int np = np64(max_Nk);
-    if (anymx) {
-      hipLaunchKernelGGL(k_xsplit, dim3(np / 16, np / 64 + 1, 3 * nwires), dim3(256), 0, s, d_descs);
-      hipLaunchKernelGGL(k_exch_mx, dim3(np / 32, np / 64, 3 * nwires), dim3(256), 0, s, d_descs);
-    }
+    hipLaunchKernelGGL(k_exch_fft, dim3((2 * max_Nk - 1 + LN - 2) / (LN - 1), 3, nwires), dim3(LN * M / 8), lds_x, s,
+                       d_descs, LN);
   }
   if (stages & ST_RHS) {
     const int np = np64(max_Nk), t64 = np / 64;

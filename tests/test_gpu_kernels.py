@@ -86,6 +86,59 @@ def test_calc_h_entry(precision, tol):
     assert torch.equal(dev.packed_state(0), before)  # the state is not touched
 
 
+def _direct_exchange(X, v, k1, k2):
+    """sum_{q != 0} v[|q|] X[k2+q, k1+q] over the q that keep both indices in range."""
+    nk = X.shape[0]
+    q = np.arange(max(-k1, -k2), min(nk - 1 - k1, nk - 1 - k2) + 1)
+    q = q[q != 0]
+    return np.sum(v[np.abs(q)] * X[k2 + q, k1 + q])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nk", [4, 6, 70, 130, 260, 600, 1100])
+def test_exchange_against_direct_sums(nk, precision):
+    """The exchange sums of CalcH (PSTD3D/SBEs.py:860-907), FFT convolutions along the
+    diagonals on lines of every size the sizes select (M = 8 ... 4096 points, 16 ... 2 lines
+    per workgroup), against direct sums over q at 300 seeded entries: Heh (Optics off, so Heh
+    is the exchange of p alone), and Hee / Hhh off the diagonal (exchange of C / D plus the
+    direct terms V(|qd|) tr - V_eh(|qd|) tr', qd = k1 - k2).  Relative L2 <= 1e-12."""
+    import torch
+
+    from pulsesuite_amd.SBEs import SBESolver
+    from pulsesuite_amd.wire_model import L_of_nk, MBFlags, QWParams
+
+    L = L_of_nk(nk)
+    rr = np.linspace(-3 * L, 3 * L, 48)
+    gpu = SBESolver(None, rr, 0.0, 1e7, 800e-9, 1, True, qw_params=QWParams(L=L),
+                    mb_flags=MBFlags(Optics=False), precision=precision)
+    assert gpu.dev.models[0].Nk == nk
+    rng = np.random.default_rng(nk)
+    p = 0.1 * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk)))
+    C = herm(nk, rng)
+    D = herm(nk, rng)
+    gpu.set_state(1, p1=p, p2=p, C1=C, C2=C, D1=D, D2=D)
+    dev = gpu.dev
+    dev.force_on()
+    dev.kernel("pss_calc_h", 3)
+    torch.cuda.synchronize()
+    vc = dev.workspace_view(0, "vc").cpu().numpy().reshape(3, nk)
+    H = dev.workspace_view(0, "H").cpu().numpy().view(complex).reshape(3, nk, nk)
+    ks = rng.integers(0, nk, (300, 2))
+    ks = ks[ks[:, 0] != ks[:, 1]] if nk > 4 else np.array([(a, b) for a in range(nk) for b in range(nk) if a != b])
+    want = {0: [], 1: [], 2: []}
+    for k1, k2 in ks:
+        qd = k1 - k2
+        aq = abs(qd)
+        want[0].append(_direct_exchange(p, vc[0], k1, k2))
+        want[1].append(vc[1][aq] * np.trace(C, offset=qd) - vc[0][aq] * np.trace(D, offset=-qd)
+                       - _direct_exchange(C, vc[1], k1, k2))
+        want[2].append(vc[2][aq] * np.trace(D, offset=qd) - vc[0][aq] * np.trace(C, offset=-qd)
+                       - _direct_exchange(D, vc[2], k1, k2))
+    for n in range(3):
+        got = H[n][ks[:, 0], ks[:, 1]]
+        assert rel(got, np.array(want[n])) < 1e-12, (n, rel(got, np.array(want[n])))
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
 def test_sbe_rhs_entry(precision, tol):
     """dp/dt, dC/dt, dD/dt of the current state into the next buffers: the oracle's dpdt,
