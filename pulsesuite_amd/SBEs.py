@@ -420,8 +420,6 @@ def ws_layout(Nk, Nr, nrr):
     out["total_fp64"] = o
     out["sp"] = o  # mixed precision only: FP16 operand planes
     o += al(16 * np64 * np64)
-    out["xp"] = o  # mixed precision only: exchange planes (X~ and Toeplitz T, FP16 hi/lo)
-    o += al(8 * np64 * np64)
     out["total"] = o
     return out
 
