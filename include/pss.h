@@ -364,6 +364,27 @@ int pss_jline(const double* P, const double* Rho, double* Pold, double* RhoOld, 
               double dt, double dx, double* jline, int32_t fft, int32_t keep_old, void* stream);
 
 /*
+ * Longitudinal-field diagnostics of QWArray.QuantumWire (no counterpart entry points in the
+ * reference; they carry its ElongfromRho and RhoB steps, PSTD3D/rhoPJ.py:784-969, 1206-1285).
+ * pss_elong_k: the k-space loop of ElongfromRho (rhoPJ.py:596-700) on the transforms of rho
+ * and P (all complex [Nx][Ny][Nz], the grid's FFT-order kx, ky, kz): out[10] = E from P + rho
+ * (x, y, z), E from P (x, y, z), E from rho (x, y, z), RhoB, still in k-space; small and
+ * inv_eps as the reference's (dq_x / 1e4, 1 / (eps0 eps_r)).
+ */
+int pss_elong_k(const pss_grid* g, const double* Rho_k, const double* Px, const double* Py, const double* Pz,
+                double* const* out, double small, double inv_eps, void* stream);
+/* QWPlacement (rhoPJ.py:736-781) of nterms <= 3 placed terms summed in order into out
+ * [Nx][Ny][Nz] complex: term t is sign[t] (gx[t][i] gy[t][j]) gz[t][k] F_t(i), F_t the mean of
+ * nw complex lines (lines[t] + w lstride[t] complex), or for mode[t] = 1 its periodic central
+ * difference (F(i+1) - F(i-1)) / (2 dx); gx null (or gx[t] null) means gx = 1. */
+int pss_qw_place(int32_t Nx, int32_t Ny, int32_t Nz, int32_t nw, int32_t nterms, const double* const* lines,
+                 const int64_t* lstride, const double* const* gx, const double* const* gy, const double* const* gz,
+                 const double* sign, const int32_t* mode, double dx, double* out, void* stream);
+/* F *= exp(-(sx^2 kx^2 + sy^2 ky^2 + sz^2 kz^2) / 2) on a complex k-space grid (the
+ * smoothing filter of RhoBSmooth / RhoBAnSmooth, rhoPJ.py:1263-1285). */
+int pss_kfilter(const pss_grid* g, double* F, double sx, double sy, double sz, void* stream);
+
+/*
  * Fused stage combine of the classic RK4 step (libpulsesuite/integrator.py:143-157,
  * Numerical Recipes stage order) on n doubles (a complex array is 2n doubles):
  *   stage 0: ytmp = y + (h/2) dydt

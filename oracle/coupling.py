@@ -10,18 +10,27 @@ run every sub-band wire's QWCalculator on the x line, J = sum_w (P_w - P_old_w)/
 Current semantics (``current=``): "reference" follows the reference's code, which
 re-creates PxxOld/PyyOld/PzzOld/RhoOld as zeros on every call (rhoPJ.py:1138-1147)
 before forming J, so J = sum_w P_w/dt/Nw; "delta" keeps the previous call's P and
-rho (J = dP/dt, as the reference's comment at rhoPJ.py:1182 describes).  The
-longitudinal-field diagnostics (ElongfromRho, RhoB*) are not on the J path and are
-omitted.
+rho (J = dP/dt, as the reference's comment at rhoPJ.py:1182 describes).
 
-Pinning: EAtXYZ, CalcJfx, QWPlacement pinned (closed forms); the composition is
-build-defined (no reference driver ships) and parity unpinned.
+Longitudinal-field diagnostics of QuantumWire (``longitudinal=True``): ElongfromRho
+(PSTD3D/rhoPJ.py:784-969, k-space loop :596-700) on the placed rho and P, and the
+bound-charge diagnostics RhoBAnalytic, RhoBSmooth, RhoBAnSmooth (rhoPJ.py:1235-1285).
+
+Pinning: EAtXYZ, CalcJfx, QWPlacement pinned (closed forms); ElongfromRho pinned by
+closed forms (one Fourier mode of rho or P: amplitude, sign and axis of each output,
+Gauss's law for RhoB, the q = 0 mode, longitudinality); the RhoB diagnostics pinned by
+the product rule on a quadratic line (exact central difference) and by the filter of
+one Fourier mode; the composition is build-defined (no reference driver ships) and
+parity unpinned.
 """
 
 import numpy as np
 
 from .helpers import locator
-from .pstd import GetDx, GetXArray, GetYArray, GetZArray, GetKxArray, fft_3D, ifft_3D
+from scipy.constants import epsilon_0 as eps0
+
+from .pstd import (GetDx, GetDy, GetDz, GetEpsr, GetKxArray, GetKyArray, GetKzArray, GetXArray, GetYArray,
+                   GetZArray, fft_3D, ifft_3D)
 from .sbe import OracleWire
 
 
@@ -58,12 +67,63 @@ def CalcJfx(RhoNew, RhoPrev, dt, dx):
     return out
 
 
+def QWPlacement(Fwire, gx, gy, gz):
+    """Fgrid[i, j, k] = gx[i] gy[j] gz[k] Fwire[i].  PSTD3D/rhoPJ.py:736-781"""
+    return (gx * Fwire)[:, None, None] * gy[None, :, None] * gz[None, None, :]
+
+
+def ElongfromRho(space, Rho, Px, Py, Pz):
+    """Longitudinal fields from P and the free charge.  PSTD3D/rhoPJ.py:784-969 (k-space
+    loop :596-700).  Px, Py, Pz are transformed to k-space in place, as the reference does.
+    Returns (E from P + rho, E from P, E from rho) as three (x, y, z) triples of real-space
+    grids, and RhoBSpec (real space)."""
+    qx, qy, qz = GetKxArray(space), GetKyArray(space), GetKzArray(space)
+    small = (qx[1] - qx[0]) / 1e4
+    inv_epsr = 1.0 / (eps0 * GetEpsr(space))
+    Rk = np.fft.fftn(Rho)
+    for P in (Px, Py, Pz):
+        P[...] = np.fft.fftn(P)
+    QX, QY, QZ = qx[:, None, None], qy[None, :, None], qz[None, None, :]
+    qsq = QX**2 + QY**2 + QZ**2 + small
+    qdotP = QX * Px + QY * Py + QZ * Pz
+    rho_term = -1j * Rk * inv_epsr / qsq
+    P_term = -qdotP * inv_epsr / qsq
+    ER = [rho_term * Q for Q in (QX, QY, QZ)]
+    EP = [P_term * Q for Q in (QX, QY, QZ)]
+    EPR = [(rho_term + P_term) * Q for Q in (QX, QY, QZ)]
+    qdotE = QX * EPR[0] + QY * EPR[1] + QZ * EPR[2]
+    RhoB = 1j * qdotE / inv_epsr - Rk
+    back = lambda F: np.fft.ifftn(F)  # noqa: E731
+    return [back(F) for F in EPR], [back(F) for F in EP], [back(F) for F in ER], back(RhoB)
+
+
+def rhob_diagnostics(space, Pavg, gy, gz, dgy, dgz, RhoBSpec):
+    """RhoBAnalytic (product rule, -div P of the placed P), RhoBSmooth, RhoBAnSmooth
+    (Gaussian k-space filter of widths 2 dx, 2 dy, 2 dz).  PSTD3D/rhoPJ.py:1235-1285"""
+    Nx = space.Nx
+    gx = np.ones(Nx)
+    dx = GetDx(space)
+    dPxdx = (np.roll(Pavg[0], -1) - np.roll(Pavg[0], 1)) / (2.0 * dx)
+    An = -QWPlacement(dPxdx, gx, gy, gz)
+    An = An - QWPlacement(Pavg[1], gx, dgy, gz)
+    An = An - QWPlacement(Pavg[2], gx, gy, dgz)
+    s = (2.0 * GetDx(space), 2.0 * GetDy(space), 2.0 * GetDz(space))
+    qx, qy, qz = GetKxArray(space), GetKyArray(space), GetKzArray(space)
+    filt = np.exp(-0.5 * (s[0]**2 * qx[:, None, None]**2 + s[1]**2 * qy[None, :, None]**2
+                          + s[2]**2 * qz[None, None, :]**2))
+    smooth = np.fft.ifftn(np.fft.fftn(RhoBSpec) * filt)
+    an_smooth = np.fft.ifftn(np.fft.fftn(An) * filt)
+    return An, smooth, an_smooth
+
+
 class OracleCoupling:
     """Nw sub-band wires along x through (x0, y0, z0) feeding J into the grid."""
 
     def __init__(self, space, lam, Emax0, Nw=1, qw=None, mb=None, y0=0.0, z0=0.0, ay=5e-9, az=5e-9, x0=0.0,
-                 current="reference", integrator="leapfrog"):
+                 current="reference", integrator="leapfrog", longitudinal=False):
         self.current = current
+        self.longitudinal = longitudinal
+        self.diag = None
         self.space = space
         self.rx = GetXArray(space)
         self.ry = GetYArray(space)
@@ -77,6 +137,8 @@ class OracleCoupling:
         self.RhoOld = np.zeros((Nx, Nw), dtype=np.complex128)
         self.gy = np.exp(-((self.ry - y0) ** 2) / (2.0 * ay**2))
         self.gz = np.exp(-((self.rz - z0) ** 2) / (2.0 * az**2))
+        self.dgy = -(self.ry - y0) / ay**2 * self.gy
+        self.dgz = -(self.rz - z0) / az**2 * self.gz
         self.last_line = None
 
     def calc_j(self, space, time, Ex, Ey, Ez, Jx, Jy, Jz):
@@ -102,6 +164,8 @@ class OracleCoupling:
             Rho[:, w] = R
         J = np.sum(P - self.Pold, axis=2) / dt / float(self.Nw)
         J[0] = J[0] + CalcJfx(np.sum(Rho, axis=1), np.sum(self.RhoOld, axis=1), dt, GetDx(space)) / float(self.Nw)
+        if self.longitudinal:
+            self._diagnostics(space, P, Rho)
         self.Pold = P
         self.RhoOld = Rho
         self.last_line = J
@@ -109,3 +173,18 @@ class OracleCoupling:
         for comp, Jg in enumerate((Jx, Jy, Jz)):
             Jg[...] = J[comp][:, None, None] * g
             fft_3D(Jg)
+
+    def _diagnostics(self, space, P, Rho):
+        """The longitudinal fields and bound-charge diagnostics of QuantumWire
+        (PSTD3D/rhoPJ.py:1206-1285) from this call's wire outputs."""
+        gx = np.ones(space.Nx)
+        Pavg = np.sum(P, axis=2) / float(self.Nw)
+        Rgrid = QWPlacement(np.sum(Rho, axis=1) / float(self.Nw), gx, self.gy, self.gz)
+        Pg = [QWPlacement(Pavg[c], gx, self.gy, self.gz) for c in range(3)]
+        EPR, EP, ER, RhoB = ElongfromRho(space, Rgrid, *Pg)
+        An, smooth, an_smooth = rhob_diagnostics(space, Pavg, self.gy, self.gz, self.dgy, self.dgz, RhoB)
+        self.diag = {"Rho": Rgrid, "Px": Pg[0], "Py": Pg[1], "Pz": Pg[2],
+                     "ExlfromPRho": EPR[0], "EylfromPRho": EPR[1], "EzlfromPRho": EPR[2],
+                     "ExlfromP": EP[0], "EylfromP": EP[1], "EzlfromP": EP[2],
+                     "ExlfromRho": ER[0], "EylfromRho": ER[1], "EzlfromRho": ER[2],
+                     "RhoBSpec": RhoB, "RhoBAnalytic": An, "RhoBSmooth": smooth, "RhoBAnSmooth": an_smooth}

@@ -1357,6 +1357,101 @@ __global__ void __launch_bounds__(256) k_jline(const cplx* P, const cplx* Rho, c
   }
 }
 
+// ---- longitudinal fields and bound charge (diagnostics of QuantumWire)
+// ElongfromRho's k-space loop (PSTD3D/rhoPJ.py:596-700), one cell per thread, on the
+// transforms of the free charge and of P:  s = |q|^2 + small,
+//   E from rho = -i rho ieps / s q,   E from P = -(q.P) ieps / s q,   E from both = their sum q,
+//   RhoB = i (q.E_both) / ieps - rho   (Gauss's law),  ieps = 1 / (eps0 eps_r).
+struct ElongOut {
+  cplx* e[10];  // ExPR, EyPR, EzPR, ExP, EyP, EzP, ExR, EyR, EzR, RhoB
+};
+
+__global__ void __launch_bounds__(256) k_elong(pss_grid g, const cplx* __restrict__ Rk, const cplx* __restrict__ Px,
+                                               const cplx* __restrict__ Py, const cplx* __restrict__ Pz, ElongOut o,
+                                               double small, double ieps) {
+  const int Ny = g.Ny, Nz = g.Nz;
+  const int64_t n = (int64_t)g.Nx * Ny * Nz;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int k = (int)(c % Nz), j = (int)((c / Nz) % Ny), i = (int)(c / ((int64_t)Ny * Nz));
+  const double qx = g.kx[i], qy = g.ky[j], qz = g.kz[k];
+  const double qs = qx * qx + qy * qy + qz * qz + small;
+  const cplx R = Rk[c], X = Px[c], Y = Py[c], Z = Pz[c];
+  const cplx qP = qx * X + qy * Y + qz * Z;
+  const cplx rt = mk(R.y * ieps / qs, -R.x * ieps / qs);     // -i rho ieps / s
+  const cplx pt = mk(-qP.x * ieps / qs, -qP.y * ieps / qs);  // -(q.P) ieps / s
+  const cplx sm = rt + pt;
+  const cplx ex = sm * qx, ey = sm * qy, ez = sm * qz;
+  o.e[0][c] = ex;
+  o.e[1][c] = ey;
+  o.e[2][c] = ez;
+  o.e[3][c] = pt * qx;
+  o.e[4][c] = pt * qy;
+  o.e[5][c] = pt * qz;
+  o.e[6][c] = rt * qx;
+  o.e[7][c] = rt * qy;
+  o.e[8][c] = rt * qz;
+  const cplx qE = qx * ex + qy * ey + qz * ez;
+  o.e[9][c] = mk(-qE.y / ieps - R.x, qE.x / ieps - R.y);  // i (q.E) / ieps - rho
+}
+
+// QWPlacement (PSTD3D/rhoPJ.py:736-781) of wire lines, up to three placed terms summed in
+// order: out = (((s0 T0) + s1 T1) + s2 T2), T_t[i][j][k] = (gx[i] gy[j]) gz[k] F_t(i) with F_t
+// the mean of nw lines (lines + w lstride) at i, or for mode 1 its periodic central
+// difference (F(i+1) - F(i-1)) / (2 dx) (the reference's np.roll difference); gx null = 1.
+struct PlaceTerm {
+  const cplx* lines;
+  const cplx *gx, *gy, *gz;
+  int64_t lstride;
+  double sign;
+  int mode;
+};
+struct PlaceTerms {
+  PlaceTerm t[3];
+};
+
+__device__ inline cplx line_mean(const PlaceTerm& t, int nw, int i) {
+  cplx s = mk(0, 0);
+  for (int w = 0; w < nw; ++w) s = s + t.lines[(int64_t)w * t.lstride + i];
+  return mk(s.x / (double)nw, s.y / (double)nw);
+}
+
+__global__ void __launch_bounds__(256) k_qwplace(int Nx, int Ny, int Nz, int nw, PlaceTerms pt, int nterms, double dx,
+                                                 cplx* __restrict__ out) {
+  const int64_t n = (int64_t)Nx * Ny * Nz;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int k = (int)(c % Nz), j = (int)((c / Nz) % Ny), i = (int)(c / ((int64_t)Ny * Nz));
+  cplx acc = mk(0, 0);
+  for (int q = 0; q < nterms; ++q) {
+    const PlaceTerm& t = pt.t[q];
+    cplx F;
+    if (t.mode == 1) {
+      const cplx a = line_mean(t, nw, i + 1 < Nx ? i + 1 : 0), b = line_mean(t, nw, i > 0 ? i - 1 : Nx - 1);
+      const cplx df = a - b;
+      F = mk(df.x / (2.0 * dx), df.y / (2.0 * dx));
+    } else {
+      F = line_mean(t, nw, i);
+    }
+    const cplx gxy = t.gx ? t.gx[i] * t.gy[j] : t.gy[j];
+    const cplx v = (gxy * t.gz[k]) * F;
+    acc = q == 0 ? t.sign * v : acc + t.sign * v;
+  }
+  out[c] = acc;
+}
+
+// F *= exp(-(sx^2 qx^2 + sy^2 qy^2 + sz^2 qz^2) / 2): the Gaussian k-space filter of the
+// smoothed bound-charge diagnostics (PSTD3D/rhoPJ.py:1263-1285).
+__global__ void __launch_bounds__(256) k_kfilter(pss_grid g, cplx* __restrict__ F, double sx2, double sy2, double sz2) {
+  const int Ny = g.Ny, Nz = g.Nz;
+  const int64_t n = (int64_t)g.Nx * Ny * Nz;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int k = (int)(c % Nz), j = (int)((c / Nz) % Ny), i = (int)(c / ((int64_t)Ny * Nz));
+  const double qx = g.kx[i], qy = g.ky[j], qz = g.kz[k];
+  F[c] = F[c] * exp(-0.5 * (sx2 * (qx * qx) + sy2 * (qy * qy) + sz2 * (qz * qz)));
+}
+
 }  // namespace
 
 // ================================================================== C ABI
@@ -1773,5 +1868,52 @@ extern "C" int pss_pass_timing_read(float* ms, int32_t n, int32_t* got) {
   const int k = std::min(n, g_tev_next);
   for (int i = 0; i < k; ++i) PSS_CHECK(hipEventElapsedTime(&ms[i], g_tev[2 * i], g_tev[2 * i + 1]));
   *got = k;
+  return 0;
+}
+
+
+// ---- longitudinal-field diagnostics (no counterpart entry in the reference; ElongfromRho's loop)
+extern "C" int pss_elong_k(const pss_grid* g, const double* Rho_k, const double* Px, const double* Py,
+                           const double* Pz, double* const* out, double small, double inv_eps, void* stream) {
+  if (!g || !Rho_k || !Px || !Py || !Pz || !out) return pss_set_msg(2, "pss_elong_k: null argument");
+  ElongOut o;
+  for (int q = 0; q < 10; ++q) {
+    if (!out[q]) return pss_set_msg(2, "pss_elong_k: null output");
+    o.e[q] = (cplx*)out[q];
+  }
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  hipLaunchKernelGGL(k_elong, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g,
+                     (const cplx*)Rho_k, (const cplx*)Px, (const cplx*)Py, (const cplx*)Pz, o, small, inv_eps);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_qw_place(int32_t Nx, int32_t Ny, int32_t Nz, int32_t nw, int32_t nterms, const double* const* lines,
+                            const int64_t* lstride, const double* const* gx, const double* const* gy,
+                            const double* const* gz, const double* sign, const int32_t* mode, double dx, double* out,
+                            void* stream) {
+  if (Nx < 1 || Ny < 1 || Nz < 1 || nw < 1 || nterms < 1 || nterms > 3 || !lines || !lstride || !gy || !gz ||
+      !sign || !mode || !out)
+    return pss_set_msg(2, "pss_qw_place: bad sizes or null argument (1 <= nterms <= 3)");
+  PlaceTerms pt{};
+  for (int q = 0; q < nterms; ++q) {
+    if (!lines[q] || !gy[q] || !gz[q] || (mode[q] != 0 && mode[q] != 1))
+      return pss_set_msg(2, "pss_qw_place: null line or gate, or mode not 0/1");
+    pt.t[q] = PlaceTerm{(const cplx*)lines[q], gx ? (const cplx*)gx[q] : nullptr, (const cplx*)gy[q],
+                        (const cplx*)gz[q], lstride[q], sign[q], mode[q]};
+  }
+  const int64_t n = (int64_t)Nx * Ny * Nz;
+  hipLaunchKernelGGL(k_qwplace, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, Nx, Ny, Nz, nw,
+                     pt, nterms, dx, (cplx*)out);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pss_kfilter(const pss_grid* g, double* F, double sx, double sy, double sz, void* stream) {
+  if (!g || !F) return pss_set_msg(2, "pss_kfilter: null argument");
+  const int64_t n = (int64_t)g->Nx * g->Ny * g->Nz;
+  hipLaunchKernelGGL(k_kfilter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g, (cplx*)F,
+                     sx * sx, sy * sy, sz * sz);
+  PSS_CHECK(hipGetLastError());
   return 0;
 }

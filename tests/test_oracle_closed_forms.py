@@ -407,3 +407,92 @@ def test_rk4_mode_is_fourth_order_and_leapfrog_is_not():
     l1 = np.linalg.norm(run(T / 8, "leapfrog") - lref)
     l2 = np.linalg.norm(run(T / 16, "leapfrog") - lref)
     assert l1 / l2 < 4.5 and l1 > 100 * e1
+
+
+# ---------------------------------------------------------------- longitudinal fields
+def _elong_space():
+    from oracle import pstd as OP
+
+    return OP.Space(3, 8, 4, 16, 20e-9, 30e-9, 10e-9, 9.1)
+
+
+def _mode(space, a, b, c):
+    i, j, k = np.meshgrid(np.arange(space.Nx), np.arange(space.Ny), np.arange(space.Nz), indexing="ij")
+    return np.exp(2j * np.pi * (a * i / space.Nx + b * j / space.Ny + c * k / space.Nz))
+
+
+@pytest.mark.parametrize("mode", [(1, 0, 0), (0, 3, 0), (0, 0, 13), (3, 1, 5)])
+def test_elong_of_one_fourier_mode(mode):
+    """ElongfromRho (PSTD3D/rhoPJ.py:784-969) of one Fourier mode exp(i q.r): from rho,
+    E = -i q rho / (eps (|q|^2 + s)); from P = x_hat exp(i q.r), E = -q q_x / (eps (|q|^2 + s));
+    RhoB = i eps q.E - rho (Gauss's law), so -i q_x |q|^2 / (|q|^2 + s) for that P (the bound
+    charge -div P), and -s / (|q|^2 + s) rho for rho alone; s = dq_x / 1e4.  Every output is
+    longitudinal (parallel to q), and P is returned transformed in place."""
+    from scipy.constants import epsilon_0
+
+    from oracle import coupling as OC
+    from oracle import pstd as OP
+
+    sp = _elong_space()
+    a, b, c = mode
+    q = np.array([OP.GetKxArray(sp)[a], OP.GetKyArray(sp)[b], OP.GetKzArray(sp)[c]])
+    s = (OP.GetKxArray(sp)[1] - OP.GetKxArray(sp)[0]) / 1e4
+    eps = epsilon_0 * OP.GetEpsr(sp)
+    q2 = float(q @ q) + s
+    f = _mode(sp, a, b, c)
+    zero = np.zeros_like(f)
+    EPR, EP, ER, RhoB = OC.ElongfromRho(sp, f, zero.copy(), zero.copy(), zero.copy())
+    for n in range(3):
+        assert np.allclose(ER[n], -1j * q[n] / (eps * q2) * f, rtol=1e-12, atol=0)
+        assert np.allclose(EPR[n], ER[n], rtol=1e-15, atol=0)
+        assert np.abs(EP[n]).max() == 0.0
+    # q^2 / (q^2 + s) rho - rho cancels to ~7e-14 of rho: rounding leaves ~1e-16 absolute
+    assert np.allclose(RhoB, -s / q2 * f, rtol=0, atol=1e-15)
+    Px = f.copy()
+    EPR, EP, ER, RhoB = OC.ElongfromRho(sp, zero, Px, zero.copy(), zero.copy())
+    assert np.allclose(Px, np.fft.fftn(f), rtol=0, atol=1e-9)
+    for n in range(3):
+        assert np.allclose(EP[n], -q[n] * q[0] / (eps * q2) * f, rtol=1e-12, atol=1e-300)
+        assert np.abs(ER[n]).max() == 0.0
+    assert np.allclose(RhoB, -1j * q[0] * (q2 - s) / q2 * f, rtol=1e-12, atol=1e-300)
+
+
+def test_elong_uniform_charge():
+    """The q = 0 mode: a uniform rho gives no field and RhoB = -rho."""
+    from oracle import coupling as OC
+
+    sp = _elong_space()
+    f = np.full((sp.Nx, sp.Ny, sp.Nz), 2.5 + 1.0j)
+    zero = np.zeros_like(f)
+    EPR, EP, ER, RhoB = OC.ElongfromRho(sp, f, zero.copy(), zero.copy(), zero.copy())
+    for F in EPR + EP + ER:
+        assert np.abs(F).max() < 1e-30
+    assert np.allclose(RhoB, -f, rtol=1e-14)
+
+
+def test_rhob_analytic_product_rule_and_filter():
+    """RhoBAnalytic (PSTD3D/rhoPJ.py:1235-1261) of P placed with Gaussian gates is
+    -dPx/dx gy gz - Py dgy gz - Pz gy dgz, with the central difference exact for a quadratic
+    line away from the periodic ends; RhoBSmooth of one Fourier mode is the mode times
+    exp(-(sx^2 qx^2 + sy^2 qy^2 + sz^2 qz^2) / 2), s = 2 (dx, dy, dz)  (:1263-1285)."""
+    from oracle import coupling as OC
+    from oracle import pstd as OP
+
+    sp = _elong_space()
+    i = np.arange(sp.Nx)
+    dx = OP.GetDx(sp)
+    Pavg = np.array([0.3 + 0.7 * i + 0.2 * i**2, 1.0 - 0.5 * i, 0.25 * i**2], dtype=complex)
+    ry, rz = OP.GetYArray(sp), OP.GetZArray(sp)
+    gy, gz = np.exp(-ry**2 / (2 * 25e-18)), np.exp(-rz**2 / (2 * 36e-18))
+    dgy, dgz = -ry / 25e-18 * gy, -rz / 36e-18 * gz
+    f = _mode(sp, 1, 2, 3)
+    An, smooth, _ = OC.rhob_diagnostics(sp, Pavg, gy, gz, dgy, dgz, f)
+    dP = (0.7 + 0.4 * i) / dx
+    want = (-dP[:, None, None] * gy[None, :, None] * gz[None, None, :]
+            - Pavg[1][:, None, None] * dgy[None, :, None] * gz[None, None, :]
+            - Pavg[2][:, None, None] * gy[None, :, None] * dgz[None, None, :])
+    assert np.allclose(An[1:-1], want[1:-1], rtol=1e-12, atol=0)
+    q = (OP.GetKxArray(sp)[1], OP.GetKyArray(sp)[2], OP.GetKzArray(sp)[3])
+    s = (2 * dx, 2 * OP.GetDy(sp), 2 * OP.GetDz(sp))
+    damp = np.exp(-0.5 * sum((s[n] * q[n]) ** 2 for n in range(3)))
+    assert np.allclose(smooth, damp * f, rtol=1e-12, atol=1e-15)
