@@ -5,8 +5,10 @@ backend)`` (:4635), the module shims ``InitializeSBE`` / ``QWCalculator`` /
 ``chiqw`` / ``getqc`` / ``QWArea`` / ``ShutOffOptics`` (:6020-6078), and
 ``QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP,
 DoQWDl)`` with 1-based ``w`` and in-place outputs.  The wire state lives on the
-GPU; every step runs in libpss (there is no host fallback).  File output of the
-reference (dataQW/*.dat, PdotE) is not produced.
+GPU; every step runs in libpss (there is no host fallback).  With
+``write_fields=True`` the reference's files of the jmax cadence are written (the state,
+energies, dephasing and scattering rates, the fields and sources on both grids; see
+``pulsesuite_amd.output``) on a host thread; the per-step info and PdotE files are not.
 
 Extra, GPU-native entry points: ``step_all`` advances all Nw wires in one
 batched launch from device-resident field lines; ``get_state`` / ``set_state``
@@ -376,8 +378,8 @@ class DeviceWires:
         Nk, Nr, nrr = m.Nk, m.Nr, self.nrr[i]
         lay = ws_layout(Nk, Nr, nrr)
         sizes = {"vc": 3 * Nk, "e1d": Nk, "gam": 2 * Nk, "De": (2 * Nk + 1) * Nk, "Dh": (2 * Nk + 1) * Nk,
-                 "H": 3 * Nk * Nk * 2, "W": 4 * Nk, "n3": 2 * Nk, "eq": 3 * (2 * Nk - 1) * 2, "pr": 3 * Nr,
-                 "rho": 2 * Nr * 2, "er": 3 * Nr, "rsp": Nk, "dcc": 8 * Nk}
+                 "H": 3 * Nk * Nk * 2, "W": 4 * Nk, "n3": 4 * Nk, "eq": 3 * (2 * Nk - 1) * 2, "pr": 3 * Nr,
+                 "rho": 2 * Nr * 2, "er": 3 * Nr, "rsp": Nk, "dcc": 8 * Nk, "outr": 5 * nrr}
         o = lay[name]
         return self.ws[i][o:o + sizes[name]]
 
@@ -430,11 +432,14 @@ class SBESolver:
     Keyword-only extras: ``qw_params`` / ``mb_flags`` replace the files
     params/qw.params and params/mb.params read from the CWD; ``precision``
     selects "fp64" (default) or "mixed" contractions; ``device`` the GPU;
-    ``integrator`` "leapfrog" (the reference's step) or "rk4" (DeviceWires.step_rk4).
+    ``integrator`` "leapfrog" (the reference's step) or "rk4" (DeviceWires.step_rk4);
+    ``write_fields`` the reference's output files every jmax steps (dataQW/Wire/*, dataQW/
+    Prop/*, PSTD3D/SBEs.py:5724-5727, 5931-5934, PSTD3D/qwoptics.py:798-799) written on a
+    host thread (``flush_output`` waits for them).
     """
 
     def __init__(self, q, rr, r0, Emaxxx, lam, Nw, QW, backend="auto", *, qw_params=None, mb_flags=None,
-                 precision="fp64", device=None, integrator="leapfrog"):
+                 precision="fp64", device=None, integrator="leapfrog", write_fields=False):
         self._backend = backend  # stored and not read, as the reference does (PSTD3D/SBEs.py:4750)
         if integrator not in ("leapfrog", "rk4"):
             raise ValueError(f"integrator must be 'leapfrog' or 'rk4', not {integrator!r}")
@@ -461,6 +466,16 @@ class SBESolver:
         self.jjj = 1
         self.jmax = qw.jmax
         self._on_host = [False] * Nw
+        self.write_fields = bool(write_fields)
+        self._writer = None
+        if self.write_fields:
+            from .output import AsyncWriter
+            self._writer = AsyncWriter()
+
+    def flush_output(self):
+        """Wait until every submitted output file is written."""
+        if self._writer is not None:
+            self._writer.flush()
 
     # ---------------------------------------------------------- public step
     def QWCalculator(self, Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl):
@@ -476,10 +491,27 @@ class SBESolver:
         dev.set_dt(dt)
         for c, F in enumerate((Exx, Eyy, Ezz)):
             dev.E_in[i][c].copy_(torch.as_tensor(F).to(dev.device, _CDT), non_blocking=True)
+        write = self.write_fields and self.jjj == self.jmax and self.integrator == "leapfrog"
+        Hd = None
         if self.integrator == "rk4":
             dev.step_rk4((i, i + 1))
+        elif write:
+            # the same stages in two launches: the Hamiltonian's diagonals (the Ee/Eh files)
+            # are read before the tail, where a DCTrans shift reuses their memory
+            pm = precision_mask(dev.prec_code, dev.models[i:i + 1])
+            dev.kernel("pss_sbe_stages", pm, ST_DERIV, 0, wires=(i, i + 1))
+            H = dev.workspace_view(i, "H").view(_CDT).reshape(3, self.Nk, self.Nk)
+            Hd = (H[1].diagonal().cpu().numpy(), H[2].diagonal().cpu().numpy())
+            dev.kernel("pss_sbe_stages", pm, ST_COLL | ST_TAIL, 0, wires=(i, i + 1))
         else:
             dev.step((i, i + 1))
+        if write and dev.is_on(i):  # the reference returns before any output while the wire is off
+            from .output import snapshot, write_snapshot
+            lines = [np.zeros(len(rr), complex) if F is None else
+                     np.asarray(F.cpu().numpy() if isinstance(F, torch.Tensor) else F) for F in (Exx, Eyy, Ezz, Vrr)]
+            snap = snapshot(dev, i, H=Hd, flags=self.mb)
+            snap["prop"] = lines + [snap["outr"][c].astype(complex) for c in range(3)]
+            self._writer.submit(write_snapshot, snap, w, self.xxx, np.asarray(rr, dtype=np.float64))
         for c, F in enumerate((Pxx, Pyy, Pzz)):
             _copy_out(F, dev.P_out[i][c])
         _copy_out(Rho, dev.Rho_out[i])

@@ -1961,6 +1961,8 @@ __global__ void __launch_bounds__(256) k_dc_relax(const pss_wire_desc* __restric
   for (int k = threadIdx.x; k < Nk; k += 256) {
     const int64_t o = (int64_t)k * Nk + k;
     cplx ne = B.Cn[o], nh = B.Dn[o];
+    d.ws[L.n3 + 2 * Nk + k] = ne.x;  // populations before the relaxation (the rates' output files)
+    d.ws[L.n3 + 3 * Nk + k] = nh.x;
     relax_pair(d, d.ws, L, k, ne, nh);
     B.Cn[o] = ne;
     B.Dn[o] = nh;
@@ -2110,7 +2112,11 @@ __device__ void pops_body(const pss_wire_desc& d, double* sh) {
   for (int k = threadIdx.x; k < Nk; k += 256) {
     const int64_t o = (int64_t)k * Nk + k;
     cplx ne = B.Cn[o], nh = B.Dn[o];
-    if (relax) relax_pair(d, ws, L, k, ne, nh);
+    if (relax) {
+      ws[L.n3 + 2 * Nk + k] = ne.x;  // populations before the relaxation (the rates' output files)
+      ws[L.n3 + 3 * Nk + k] = nh.x;
+      relax_pair(d, ws, L, k, ne, nh);
+    }
     const double ae = cabs(ne), ah = cabs(nh);
     ws[L.n3 + k] = ae;
     ws[L.n3 + Nk + k] = ah;
@@ -2350,6 +2356,8 @@ __device__ void qw2prop_body(const pss_wire_desc& d, const bool on, const bool n
     d.Pzz[2 * i] = ws[L.outr + 2 * n + i]; d.Pzz[2 * i + 1] = 0;
     const double re = lf ? ws[L.outr + 3 * n + i] * fe : 0.0;
     const double rh = lf ? ws[L.outr + 4 * n + i] * fh : 0.0;
+    ws[L.outr + 3 * n + i] = re;  // RhoE, RhoH on the propagation grid (the output files)
+    ws[L.outr + 4 * n + i] = rh;
     d.Rho[2 * i] = rh - re;
     d.Rho[2 * i + 1] = 0;
   }
