@@ -785,6 +785,12 @@ __host__ __device__ inline int exch_m(int Nk) {
   return M;
 }
 __host__ __device__ inline int exch_lines(int M) { return M <= 128 ? 16 : M <= 256 ? 8 : M <= 2048 ? 4 : 2; }
+// Lines per workgroup for a wire of line size M in a launch sized for Mmax with LN lines:
+// the launch's threads (LN Mmax / 8) all busy, at most 64 lines.
+__host__ __device__ inline int exch_lines_w(int LN, int Mmax, int M) {
+  const int n = LN * (Mmax / M);
+  return n < 64 ? n : 64;
+}
 // Twiddles: a step with span Ns = 8^a > 1 and radix R reads w = W_{Ns R}^k, k < Ns, from
 // its own table at offset (Ns - 8) / 7 (consecutive butterflies read consecutive entries);
 // the tables of all steps hold exch_tw(M) entries.
@@ -920,12 +926,12 @@ __device__ __forceinline__ void lines_fft(lcplx* sm, const lcplx* tq, int lm, in
   }
 }
 
-__global__ void __launch_bounds__(1024) k_exch_fft(const pss_wire_desc* __restrict__ descs, int LN) {
+__global__ void __launch_bounds__(1024) k_exch_fft(const pss_wire_desc* __restrict__ descs, int LN0, int Mmax) {
   const int wire = blockIdx.z, m = blockIdx.y;
   const pss_wire_desc& d = descs[wire];
   if (!wire_on(d) || !(d.flags & PSS_EXCITONS)) return;
   if (m > 0 && !(d.flags & PSS_LF)) return;
-  const int Nk = d.Nk, nd = 2 * Nk - 1, M = exch_m(Nk), D = LN - 1;
+  const int Nk = d.Nk, nd = 2 * Nk - 1, M = exch_m(Nk), LN = exch_lines_w(LN0, Mmax, M), D = LN - 1;
   const int lm = __builtin_ctz(M), MP = M + (M >> 3) + 1;  // odd stride: lines start on different banks
   const int od0 = blockIdx.x * D;
   if (od0 >= nd) return;
@@ -2531,15 +2537,21 @@ static int sbe_stages(const pss_wire_desc* d_descs, int32_t nwires, int32_t min_
     const int t16 = (max_Nk + 15) / 16;
     hipLaunchKernelGGL(k_calc_h, dim3(t16, t16, nwires), dim3(256), 0, s, d_descs);
     // exchange of both precisions: FFT convolutions along the diagonals (FP64)
+    // LDS for the largest need of any line size in the launch (a wire of a smaller M packs
+    // more lines into the same workgroup)
     const int M = exch_m(max_Nk), LN = exch_lines(M);
-    const size_t lds_x = ((size_t)LN * (M + M / 8 + 1) + exch_tw(M)) * 16;
+    size_t lds_x = 0;
+    for (int Mw = 8; Mw <= M; Mw *= 2) {
+      const size_t need = ((size_t)exch_lines_w(LN, M, Mw) * (Mw + Mw / 8 + 1) + exch_tw(Mw)) * 16;
+      lds_x = need > lds_x ? need : lds_x;
+    }
     static size_t lds_x_granted = 65536;
     if (lds_x > lds_x_granted) {
       PSS_CHECK(hipFuncSetAttribute((const void*)k_exch_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_x));
       lds_x_granted = lds_x;
     }
     hipLaunchKernelGGL(k_exch_fft, dim3((2 * max_Nk - 1 + LN - 2) / (LN - 1), 3, nwires), dim3(LN * M / 8), lds_x, s,
-                       d_descs, LN);
+                       d_descs, LN, M);
   }
   if (stages & ST_RHS) {
     const int np = np64(max_Nk), t64 = np / 64;
