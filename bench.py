@@ -217,9 +217,9 @@ def precision_gate(grid, nk, npml, current="reference", closed_steps=GATE_CLOSED
 DOM_BYTES_PER_CELL = 168
 # HBM bytes per launch of that kernel at 128^3, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `bench.py --workloads traffic`, each counter scaled by what it
-# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 172282.7 KiB x 2 read +
+# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 124423.7 KiB x 2 read +
 # 98304 KiB written (profiles/dominant_pass_128cube_traffic.md).
-DOM_TRAFFIC_128 = (172282.7 * 2.0 + 98304.0) * 1024.0
+DOM_TRAFFIC_128 = (124423.7 * 2.0 + 98304.0) * 1024.0
 
 
 def dominant_pass_timing(prop, reps=50):
