@@ -2479,7 +2479,7 @@ extern "C" int pss_sbe_tune(int32_t ksplit) {
 //   FRONT   K1: wireoff gate, Prop2QW splines onto r; K2: populations, screened V, E1D; K1b: E(q)
 //   TRACES  K3: diagonal traces (direct terms), CalcGammaE/H, max |p|, |C|, |D| of mixed wires
 //   OFFDIAG K4: off-diagonal dephasing tables De, Dh
-//   CALCH   K5: Meh, band energies, direct terms; K5b: exchange (Toeplitz MFMA products)
+//   CALCH   K5: Meh, band energies, direct terms; K5b: exchange (FFT convolutions along the diagonals)
 //   RHS     K6: the dpdt/dCdt/dDdt contractions and their epilogue (leapfrog, or dX/dt only)
 //   COLL    K7: collision and phonon rates Win/Wout from the populations of the next state
 //   TAIL    K8 relaxation + renormalisation, K9 diagonal sums, P(r) and rho(r), QW2Prop
