@@ -122,6 +122,7 @@ struct PassArgs {
   int accum;  // 0: out = result, 1: out += result (F_k += FFT(dF)), 2: real part to ((double*)out)[i]
   int slab;   // > 0 (two-level passes only): write only positions k < slab or k >= N - slab along the
               // axis, into an array whose axis length is 2 slab (k -> k, or k - (N - 2 slab))
+  int kmajor; // slab pass along z: the array is (2 slab, Nx, Ny), slab position outermost
   // accum 3 (two-level passes, complex128 out): out_c += result + ccoef i (q x G)_c, the
   // spectral curl update of the other field G fused into the accumulating last pass
   // (batch element c = component); q from the grid's wavenumber tables
@@ -322,9 +323,9 @@ __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz, const 
     st = N;
     cx = (int)(l0 / Ny);
     cy = (int)(l0 % Ny);
-    obase0 = SLAB ? l0 * P2 : base0;
-    ost = SLAB ? P2 : st;
-    osa = 1;
+    obase0 = SLAB ? (a.kmajor ? l0 : l0 * P2) : base0;
+    ost = SLAB ? (a.kmajor ? 1 : P2) : st;
+    osa = SLAB && a.kmajor ? nlines : 1;
   } else {
     const int nch = (Nz + TB - 1) / TB;
     const int other = (int)(bx / nch);
@@ -615,6 +616,7 @@ PassArgs make_pass_args(const pss_grid* g, const cplx* const* ins, cplx* const* 
   a.in32 = in32;
   a.out32 = out32;
   a.slab = slab;
+  a.kmajor = 0;
   if (cu) {
     a.cg0 = cu->G[0]; a.cg1 = cu->G[1]; a.cg2 = cu->G[2];
     a.qx = g->kx; a.qy = g->ky; a.qz = g->kz;
@@ -769,6 +771,7 @@ struct SlabR {
   int full;         // 1: full-grid arrays R[q n + o] (fallback path), 0: slab arrays
   int64_t poff[6];  // element offsets of psi_xy, psi_xz, psi_yx, psi_yz, psi_zx, psi_zy, each
                     // stored on the slab of its derivative axis (elsewhere c = 0 keeps it 0)
+  int zmajor;       // the z derivatives (1, 2) are stored (2 Pz, Nx, Ny), slab position outermost
 };
 
 // derivative axis of psi_xy, psi_xz, psi_yx, psi_yz, psi_zx, psi_zy
@@ -802,6 +805,7 @@ struct PassSpec {
   const double* kmul;
   int mul_axis;
   int slab;
+  int kmajor = 0;
 };
 
 int launch_pass_multi(const PassSpec* sp, int ns, int dir, hipStream_t s, int accum, int in32, int out32) {
@@ -826,6 +830,7 @@ int launch_pass_multi(const PassSpec* sp, int ns, int dir, hipStream_t s, int ac
   for (int i = 0; i < ns; ++i) {
     m.p[i] = make_pass_args(sp[i].g, sp[i].in, sp[i].out, 2, sp[i].axis, dir, sp[i].kmul, sp[i].mul_axis, accum,
                             in32, out32, sp[i].slab, nullptr);
+    m.p[i].kmajor = sp[i].kmajor;
     for (int b = 0; b < 2; ++b) {
       m.q[e] = i;
       m.b[e] = b;
@@ -840,6 +845,112 @@ int launch_pass_multi(const PassSpec* sp, int ns, int dir, hipStream_t s, int ac
     default: launch_fft2_multi<32, 32>(m, ns, e, s); break;
   }
   return 0;
+}
+
+// The passes after the slab pass of the slab derivatives, as one 2-D transform per slab
+// plane in LDS (mixed precision, equal axis lengths N = 64 or 128: an N x N complex64 plane
+// fits in LDS): load the plane, N-point inverse transforms along its rows, then along its
+// columns (each the two-level N1 x N2 transform of the pass kernels), write the real parts
+// times 1/N^2.  Entry e: derivative array S[e] -> R[e], nplanes planes pstride apart, rows
+// rstride apart.  Workgroup = one plane.
+struct PlaneSet {
+  const cflt* S[6];
+  float* R[6];
+  int64_t pstride[6], rstride[6];
+  int first[7];  // first workgroup of entry e (prefix sums of the plane counts)
+  int ne;
+  const double* tw;  // exp(-2 pi i m/N), interleaved, m < N/2
+};
+
+template <int N1, int N2, int NT>
+__global__ void __launch_bounds__(NT) k_slab_planes(PlaneSet ps) {
+  constexpr int N = N1 * N2, PITCH = N + 1;  // odd pitch: a column's points on different banks
+  constexpr int R1 = N * N2 / NT, R2 = N * N1 / NT;  // level-1 / level-2 tasks per thread
+  static_assert(R1 * NT == N * N2 && R2 * NT == N * N1, "tasks must divide evenly");
+  extern __shared__ cpx<float> lds_pl[];
+  cpx<float>* buf = lds_pl;
+  cpx<float>* tw = lds_pl + N * PITCH;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int e = b >= ps.first[5] ? 5 : b >= ps.first[4] ? 4 : b >= ps.first[3] ? 3 : b >= ps.first[2] ? 2
+              : b >= ps.first[1] ? 1 : 0;
+  if (e >= ps.ne) return;
+  const int pl = b - (e == 0 ? ps.first[0] : e == 1 ? ps.first[1] : e == 2 ? ps.first[2] : e == 3 ? ps.first[3]
+                      : e == 4 ? ps.first[4] : ps.first[5]);
+  const cflt* S = e == 0 ? ps.S[0] : e == 1 ? ps.S[1] : e == 2 ? ps.S[2] : e == 3 ? ps.S[3] : e == 4 ? ps.S[4] : ps.S[5];
+  float* R = e == 0 ? ps.R[0] : e == 1 ? ps.R[1] : e == 2 ? ps.R[2] : e == 3 ? ps.R[3] : e == 4 ? ps.R[4] : ps.R[5];
+  const int64_t pst = e == 0 ? ps.pstride[0] : e == 1 ? ps.pstride[1] : e == 2 ? ps.pstride[2]
+                    : e == 3 ? ps.pstride[3] : e == 4 ? ps.pstride[4] : ps.pstride[5];
+  const int64_t rst = e == 0 ? ps.rstride[0] : e == 1 ? ps.rstride[1] : e == 2 ? ps.rstride[2]
+                    : e == 3 ? ps.rstride[3] : e == 4 ? ps.rstride[4] : ps.rstride[5];
+  const int64_t base = (int64_t)pl * pst;
+  for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)ps.tw[2 * m], (float)-ps.tw[2 * m + 1]};  // inverse
+#pragma unroll
+  for (int m = 0; m < N * N / NT; ++m) {
+    const int idx = tid + NT * m, r = idx / N, c = idx % N;
+    const cflt v = S[base + (int64_t)r * rst + c];
+    buf[r * PITCH + c] = cpx<float>{v.x, v.y};
+  }
+  __syncthreads();
+  // one 1-D transform direction: point n of line l at buf[l * ls + n * ps_]
+  auto pass = [&](int ls, int pn, bool lines_fast) {
+    // level 1: task (line l, n2): x[n1 N2 + n2] -> DFT over n1, twiddle, back in place
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {
+      const int task = tid + NT * q;
+      const int l = lines_fast ? task % N : task / N2, n2 = lines_fast ? task / N : task % N2;
+      cpx<float> v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[n1] = buf[l * ls + (n1 * N2 + n2) * pn];
+      dft_reg<N1, N, float>(v, tw);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const int mm = (n2 * k1) % N;
+        const cpx<float> w = mm < N / 2 ? tw[mm] : cpx<float>{-tw[mm - N / 2].x, -tw[mm - N / 2].y};
+        buf[l * ls + (k1 * N2 + n2) * pn] = v[k1] * w;
+      }
+    }
+    __syncthreads();
+    // level 2: task (line l, k1): y[k1 N2 + n2] -> DFT over n2 -> X[k1 + N1 k2]
+    cpx<float> u[R2][N2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q;
+      const int l = lines_fast ? task % N : task / N1, k1 = lines_fast ? task / N : task % N1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[l * ls + (k1 * N2 + n2) * pn];
+      dft_reg<N2, N, float>(u[q], tw);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q;
+      const int l = lines_fast ? task % N : task / N1, k1 = lines_fast ? task / N : task % N1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[l * ls + (k1 + N1 * k2) * pn] = u[q][k2];
+    }
+    __syncthreads();
+  };
+  pass(PITCH, 1, false);  // along each row (points contiguous)
+  pass(1, PITCH, true);   // along each column (consecutive threads on consecutive columns)
+  constexpr float sc = 1.0f / (float)(N * N);
+#pragma unroll
+  for (int m = 0; m < N * N / NT; ++m) {
+    const int idx = tid + NT * m, r = idx / N, c = idx % N;
+    R[base + (int64_t)r * rst + c] = buf[r * PITCH + c].x * sc;
+  }
+}
+
+template <int N1, int N2, int NT>
+void launch_slab_planes(const PlaneSet& ps, int nblocks, hipStream_t s) {
+  constexpr int N = N1 * N2;
+  const size_t lds = ((size_t)N * (N + 1) + N / 2) * sizeof(cpx<float>);
+  static bool granted = false;  // once per process, outside any stream work
+  if (!granted) {
+    (void)hipFuncSetAttribute((const void*)k_slab_planes<N1, N2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);  // a refusal shows as the launch's error
+    granted = true;
+  }
+  hipLaunchKernelGGL((k_slab_planes<N1, N2, NT>), dim3(nblocks), dim3(NT), lds, s, ps);
 }
 
 // The two derivatives along one axis (of two fields) share every launch: batch of 2.
@@ -909,6 +1020,13 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
       ap[k].Sq[1] = S + ap[k].m;
     }
   const int np = na ? ap[0].np : 0;
+  // plane mode: mixed precision, three axes of one length N = 64 or 128, at least two PML
+  // axes run together: the slab pass, then one 2-D transform per slab plane (k_slab_planes);
+  // the z derivatives then live (2 Pz, Nx, Ny), slab position outermost, so that a z-slab
+  // plane is contiguous
+  const int Nc = len[0];
+  const bool planes = fp32 && together && na >= 2 && (Nc == 64 || Nc == 128) && len[1] == Nc && len[2] == Nc;
+  sr.zmajor = 0;
   auto spec = [&](const AxisPlan& x, int i) {
     const bool last = i == x.np - 1;
     PassSpec p;
@@ -923,6 +1041,36 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
     p.slab = i == 0 ? x.P : 0;
     return p;
   };
+  if (planes) {
+    PassSpec sp[3];
+    for (int k = 0; k < na; ++k) {
+      sp[k] = spec(ap[k], 0);
+      sp[k].kmajor = ap[k].a == 2;
+    }
+    const int rc = launch_pass_multi(sp, na, +1, s, 0, 0, fp32);
+    if (rc) return rc;
+    PlaneSet ps{};
+    int e = 0, nb = 0;
+    for (int k = 0; k < na; ++k) {
+      const AxisPlan& x = ap[k];
+      for (int h = 0; h < 2; ++h, ++e) {
+        ps.S[e] = (const cflt*)x.Sq[h];
+        ps.R[e] = (float*)x.Rq[h];
+        // slab arrays: x (2P, N, N) planes along x; y (N, 2P, N) planes at j'; z (2P, N, N)
+        ps.pstride[e] = x.a == 1 ? Nc : (int64_t)Nc * Nc;
+        ps.rstride[e] = x.a == 1 ? (int64_t)2 * x.P * Nc : Nc;
+        ps.first[e] = nb;
+        nb += 2 * x.P;
+      }
+      if (x.a == 2) sr.zmajor = 1;
+    }
+    for (int k = e; k <= 6; ++k) ps.first[k] = nb;
+    ps.ne = e;
+    ps.tw = g->twx;
+    if (Nc == 128) launch_slab_planes<8, 16, 512>(ps, nb, s);
+    else launch_slab_planes<8, 8, 256>(ps, nb, s);
+    return 0;
+  }
   if (together) {
     for (int i = 0; i < np; ++i) {
       PassSpec sp[3];
@@ -1016,8 +1164,9 @@ __global__ void __launch_bounds__(256) k_cpml(pss_grid g, int which, const TR* _
       cY = g.cy[j];
     }
     if (inz) {
-      sz = ((int64_t)i * Ny + j) * (2 * Pz) + (k < Pz ? k : k - (Nz - 2 * Pz));
-      const int64_t r = sr.full ? o : sz;
+      const int kz = k < Pz ? k : k - (Nz - 2 * Pz);
+      sz = ((int64_t)i * Ny + j) * (2 * Pz) + kz;
+      const int64_t r = sr.full ? o : sr.zmajor ? ((int64_t)kz * Nx + i) * Ny + j : sz;
       d1 = (double)R[sr.off[1] + r];
       d2 = (double)R[sr.off[2] + r];
       p1 = psi[sr.poff[1] + sz];
