@@ -884,59 +884,68 @@ __global__ void __launch_bounds__(NT) k_slab_planes(PlaneSet ps) {
                     : e == 3 ? ps.rstride[3] : e == 4 ? ps.rstride[4] : ps.rstride[5];
   const int64_t base = (int64_t)pl * pst;
   for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)ps.tw[2 * m], (float)-ps.tw[2 * m + 1]};  // inverse
+  __syncthreads();
+  auto twid = [&](int mm) {  // W^mm, mm < N
+    return mm < N / 2 ? tw[mm] : cpx<float>{-tw[mm - N / 2].x, -tw[mm - N / 2].y};
+  };
+  // rows, level 1, straight from memory: task (row l, n2) transforms x[n1 N2 + n2] over n1
 #pragma unroll
-  for (int m = 0; m < N * N / NT; ++m) {
-    const int idx = tid + NT * m, r = idx / N, c = idx % N;
-    const cflt v = S[base + (int64_t)r * rst + c];
-    buf[r * PITCH + c] = cpx<float>{v.x, v.y};
+  for (int q = 0; q < R1; ++q) {
+    const int task = tid + NT * q, l = task / N2, n2 = task % N2;
+    cpx<float> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const cflt x = S[base + (int64_t)l * rst + n1 * N2 + n2];
+      v[n1] = cpx<float>{x.x, x.y};
+    }
+    dft_reg<N1, N, float>(v, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * PITCH + k1 * N2 + n2] = v[k1] * twid((n2 * k1) % N);
   }
   __syncthreads();
-  // one 1-D transform direction: point n of line l at buf[l * ls + n * ps_]
-  auto pass = [&](int ls, int pn, bool lines_fast) {
-    // level 1: task (line l, n2): x[n1 N2 + n2] -> DFT over n1, twiddle, back in place
-#pragma unroll
-    for (int q = 0; q < R1; ++q) {
-      const int task = tid + NT * q;
-      const int l = lines_fast ? task % N : task / N2, n2 = lines_fast ? task / N : task % N2;
-      cpx<float> v[N1];
-#pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) v[n1] = buf[l * ls + (n1 * N2 + n2) * pn];
-      dft_reg<N1, N, float>(v, tw);
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) {
-        const int mm = (n2 * k1) % N;
-        const cpx<float> w = mm < N / 2 ? tw[mm] : cpx<float>{-tw[mm - N / 2].x, -tw[mm - N / 2].y};
-        buf[l * ls + (k1 * N2 + n2) * pn] = v[k1] * w;
-      }
-    }
-    __syncthreads();
-    // level 2: task (line l, k1): y[k1 N2 + n2] -> DFT over n2 -> X[k1 + N1 k2]
+  // rows, level 2: task (row l, k1) transforms y[k1 N2 + n2] over n2 -> X[k1 + N1 k2] (all
+  // reads before the writes: the two index sets of a row overlap)
+  {
     cpx<float> u[R2][N2];
 #pragma unroll
     for (int q = 0; q < R2; ++q) {
-      const int task = tid + NT * q;
-      const int l = lines_fast ? task % N : task / N1, k1 = lines_fast ? task / N : task % N1;
+      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[l * ls + (k1 * N2 + n2) * pn];
+      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[l * PITCH + k1 * N2 + n2];
       dft_reg<N2, N, float>(u[q], tw);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < R2; ++q) {
-      const int task = tid + NT * q;
-      const int l = lines_fast ? task % N : task / N1, k1 = lines_fast ? task / N : task % N1;
+      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) buf[l * ls + (k1 + N1 * k2) * pn] = u[q][k2];
+      for (int k2 = 0; k2 < N2; ++k2) buf[l * PITCH + k1 + N1 * k2] = u[q][k2];
     }
-    __syncthreads();
-  };
-  pass(PITCH, 1, false);  // along each row (points contiguous)
-  pass(1, PITCH, true);   // along each column (consecutive threads on consecutive columns)
+  }
+  __syncthreads();
+  // columns, level 1 in place: task (column c, n2), consecutive threads on consecutive columns
+#pragma unroll
+  for (int q = 0; q < R1; ++q) {
+    const int task = tid + NT * q, c = task % N, n2 = task / N;
+    cpx<float> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = buf[(n1 * N2 + n2) * PITCH + c];
+    dft_reg<N1, N, float>(v, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * PITCH + c] = v[k1] * twid((n2 * k1) % N);
+  }
+  __syncthreads();
+  // columns, level 2, straight to memory: the real parts times 1/N^2
   constexpr float sc = 1.0f / (float)(N * N);
 #pragma unroll
-  for (int m = 0; m < N * N / NT; ++m) {
-    const int idx = tid + NT * m, r = idx / N, c = idx % N;
-    R[base + (int64_t)r * rst + c] = buf[r * PITCH + c].x * sc;
+  for (int q = 0; q < R2; ++q) {
+    const int task = tid + NT * q, c = task % N, k1 = task / N;
+    cpx<float> u[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) u[n2] = buf[(k1 * N2 + n2) * PITCH + c];
+    dft_reg<N2, N, float>(u, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) R[base + (int64_t)(k1 + N1 * k2) * rst + c] = u[k2].x * sc;
   }
 }
 
@@ -1067,8 +1076,8 @@ int slab_derivs(const pss_grid* g, const cplx* const G[3], void* R, cplx* S, int
     for (int k = e; k <= 6; ++k) ps.first[k] = nb;
     ps.ne = e;
     ps.tw = g->twx;
-    if (Nc == 128) launch_slab_planes<8, 16, 512>(ps, nb, s);
-    else launch_slab_planes<8, 8, 256>(ps, nb, s);
+    if (Nc == 128) launch_slab_planes<8, 16, 1024>(ps, nb, s);
+    else launch_slab_planes<8, 8, 512>(ps, nb, s);
     return 0;
   }
   if (together) {
