@@ -669,6 +669,203 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
   return 0;
 }
 
+// 2-D transform of one N x N complex plane in LDS (N = N1 N2; buf: N rows of pitch N + 1,
+// the odd pitch putting a column's points on different banks; tw: W^m, m < N/2, of the
+// transform's direction): N-point transforms along the rows, then along the columns, each
+// the two-level N1 x N2 transform of the pass kernels.  The rows' first level takes its
+// points from load(r, c) and the columns' second level hands the results to
+// store(r, c, X), so the plane makes 6 LDS sweeps.  NT threads; ends without a barrier.
+template <int N1, int N2, int NT, class Load, class Store>
+__device__ __forceinline__ void plane_fft2d(cpx<float>* buf, const cpx<float>* tw, Load load, Store store) {
+  constexpr int N = N1 * N2, PITCH = N + 1;
+  constexpr int R1 = N * N2 / NT, R2 = N * N1 / NT;  // level-1 / level-2 tasks per thread
+  static_assert(R1 * NT == N * N2 && R2 * NT == N * N1, "tasks must divide evenly");
+  const int tid = threadIdx.x;
+  auto twid = [&](int mm) {  // W^mm, mm < N
+    return mm < N / 2 ? tw[mm] : cpx<float>{-tw[mm - N / 2].x, -tw[mm - N / 2].y};
+  };
+  // rows, level 1: task (row l, n2) transforms x[n1 N2 + n2] over n1
+#pragma unroll
+  for (int q = 0; q < R1; ++q) {
+    const int task = tid + NT * q, l = task / N2, n2 = task % N2;
+    cpx<float> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = load(l, n1 * N2 + n2);
+    dft_reg<N1, N, float>(v, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * PITCH + k1 * N2 + n2] = v[k1] * twid((n2 * k1) % N);
+  }
+  __syncthreads();
+  // rows, level 2: task (row l, k1) transforms y[k1 N2 + n2] over n2 -> X[k1 + N1 k2] (all
+  // reads before the writes: the two index sets of a row overlap)
+  {
+    cpx<float> u[R2][N2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[l * PITCH + k1 * N2 + n2];
+      dft_reg<N2, N, float>(u[q], tw);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[l * PITCH + k1 + N1 * k2] = u[q][k2];
+    }
+  }
+  __syncthreads();
+  // columns, level 1 in place: task (column c, n2), consecutive threads on consecutive columns
+#pragma unroll
+  for (int q = 0; q < R1; ++q) {
+    const int task = tid + NT * q, c = task % N, n2 = task / N;
+    cpx<float> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = buf[(n1 * N2 + n2) * PITCH + c];
+    dft_reg<N1, N, float>(v, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * PITCH + c] = v[k1] * twid((n2 * k1) % N);
+  }
+  __syncthreads();
+  // columns, level 2: results to store
+#pragma unroll
+  for (int q = 0; q < R2; ++q) {
+    const int task = tid + NT * q, c = task % N, k1 = task / N;
+    cpx<float> u[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) u[n2] = buf[(k1 * N2 + n2) * PITCH + c];
+    dft_reg<N2, N, float>(u, tw);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) store(k1 + N1 * k2, c, u[k2]);
+  }
+}
+
+// LDS of a plane transform: the plane, then the twiddle table
+template <int N>
+constexpr size_t plane_lds() { return ((size_t)N * (N + 1) + N / 2) * sizeof(cpx<float>); }
+
+// The passes after the slab pass of the slab derivatives, as one 2-D transform per slab
+// plane in LDS (mixed precision, equal axis lengths N = 64 or 128: an N x N complex64 plane
+// fits in LDS): N-point inverse transforms along the plane's rows, then along its columns,
+// the real parts times 1/N^2 written.  Entry e: derivative array S[e] -> R[e], planes
+// pstride apart, rows rstride apart.  Workgroup = one plane.
+struct PlaneSet {
+  const cflt* S[6];
+  float* R[6];
+  int64_t pstride[6], rstride[6];
+  int first[7];  // first workgroup of entry e (prefix sums of the plane counts)
+  int ne;
+  const double* tw;  // exp(-2 pi i m/N), interleaved, m < N/2
+};
+
+template <int N1, int N2, int NT>
+__global__ void __launch_bounds__(NT) k_slab_planes(PlaneSet ps) {
+  constexpr int N = N1 * N2;
+  extern __shared__ cpx<float> lds_pl[];
+  cpx<float>* buf = lds_pl;
+  cpx<float>* tw = lds_pl + N * (N + 1);
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int e = b >= ps.first[5] ? 5 : b >= ps.first[4] ? 4 : b >= ps.first[3] ? 3 : b >= ps.first[2] ? 2
+              : b >= ps.first[1] ? 1 : 0;
+  if (e >= ps.ne) return;
+  const int pl = b - (e == 0 ? ps.first[0] : e == 1 ? ps.first[1] : e == 2 ? ps.first[2] : e == 3 ? ps.first[3]
+                      : e == 4 ? ps.first[4] : ps.first[5]);
+  const cflt* S = e == 0 ? ps.S[0] : e == 1 ? ps.S[1] : e == 2 ? ps.S[2] : e == 3 ? ps.S[3] : e == 4 ? ps.S[4] : ps.S[5];
+  float* R = e == 0 ? ps.R[0] : e == 1 ? ps.R[1] : e == 2 ? ps.R[2] : e == 3 ? ps.R[3] : e == 4 ? ps.R[4] : ps.R[5];
+  const int64_t pst = e == 0 ? ps.pstride[0] : e == 1 ? ps.pstride[1] : e == 2 ? ps.pstride[2]
+                    : e == 3 ? ps.pstride[3] : e == 4 ? ps.pstride[4] : ps.pstride[5];
+  const int64_t rst = e == 0 ? ps.rstride[0] : e == 1 ? ps.rstride[1] : e == 2 ? ps.rstride[2]
+                    : e == 3 ? ps.rstride[3] : e == 4 ? ps.rstride[4] : ps.rstride[5];
+  const int64_t base = (int64_t)pl * pst;
+  for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)ps.tw[2 * m], (float)-ps.tw[2 * m + 1]};  // inverse
+  __syncthreads();
+  constexpr float sc = 1.0f / (float)(N * N);
+  plane_fft2d<N1, N2, NT>(
+      buf, tw,
+      [&](int r, int c) {
+        const cflt x = S[base + (int64_t)r * rst + c];
+        return cpx<float>{x.x, x.y};
+      },
+      [&](int r, int c, cpx<float> X) { R[base + (int64_t)r * rst + c] = X.x * sc; });
+}
+
+// The last two forward passes of dF (along y and z) as one 2-D transform per x-plane in
+// LDS (mixed precision, Ny = Nz = N = 64 or 128), with the accumulation into the field
+// and the spectral curl update of the other field fused in as the accumulating last pass
+// does it (F_c += FFT(dF_c) + ccoef i (q x G)_c, term for term).  dF[c] holds the x pass's
+// output (complex64).  Workgroup 24 (g / 8) + 8 c + g % 8 takes component c of x-plane g,
+// so the three components of a plane go to one XCD back to back and the other field's
+// planes that two components' curl terms both read are still in that XCD's L2.
+struct CurlPlanes {
+  const cflt* dF[3];
+  cplx* F[3];
+  const cplx* G[3];
+  const double *qx, *qy, *qz;
+  double ccoef;
+  const double* tw;  // exp(-2 pi i m/N), interleaved, m < N/2
+  int Nx;
+};
+
+template <int N1, int N2, int NT>
+__global__ void __launch_bounds__(NT) k_dF_planes(CurlPlanes cp) {
+  constexpr int N = N1 * N2;
+  extern __shared__ cpx<float> lds_pl[];
+  cpx<float>* buf = lds_pl;
+  cpx<float>* tw = lds_pl + N * (N + 1);
+  const int lin = blockIdx.x, grp = lin / 24, r8 = lin - 24 * grp;
+  const int c = r8 >> 3, gx = grp * 8 + (r8 & 7);
+  if (gx >= cp.Nx) return;
+  const int tid = threadIdx.x;
+  const cflt* dF = c == 0 ? cp.dF[0] : c == 1 ? cp.dF[1] : cp.dF[2];
+  cplx* F = c == 0 ? cp.F[0] : c == 1 ? cp.F[1] : cp.F[2];
+  // component c: (q x G)_c = q_{c+1} G_{c+2} - q_{c+2} G_{c+1}
+  const cplx* ga = c == 0 ? cp.G[1] : c == 1 ? cp.G[2] : cp.G[0];
+  const cplx* gb = c == 0 ? cp.G[2] : c == 1 ? cp.G[0] : cp.G[1];
+  const int64_t base = (int64_t)gx * N * N;
+  const double q0 = cp.qx[gx];
+  for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)cp.tw[2 * m], (float)cp.tw[2 * m + 1]};  // forward
+  __syncthreads();
+  plane_fft2d<N1, N2, NT>(
+      buf, tw,
+      [&](int r, int col) {  // row r = y, column = z
+        const cflt x = dF[base + (int64_t)r * N + col];
+        return cpx<float>{x.x, x.y};
+      },
+      [&](int r, int col, cpx<float> X) {
+        const int64_t o = base + (int64_t)r * N + col;
+        const double q1 = cp.qy[r], q2 = cp.qz[col];
+        const double qa = c == 0 ? q1 : c == 1 ? q2 : q0, qb = c == 0 ? q2 : c == 1 ? q0 : q1;
+        const cplx cr = qa * gb[o] - qb * ga[o];
+        const cpx<float> v = X * 1.0f;  // forward: scale 1
+        F[o] = F[o] + (mk((double)v.x, (double)v.y) + mk(-cr.y, cr.x) * cp.ccoef);
+      });
+}
+
+template <int N1, int N2, int NT>
+void launch_dF_planes(const CurlPlanes& cp, hipStream_t s) {
+  constexpr size_t lds = plane_lds<N1 * N2>();
+  static bool granted = false;  // once per process, outside any stream work
+  if (!granted) {
+    (void)hipFuncSetAttribute((const void*)k_dF_planes<N1, N2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);  // a refusal shows as the launch's error
+    granted = true;
+  }
+  hipLaunchKernelGGL((k_dF_planes<N1, N2, NT>), dim3((unsigned)((cp.Nx + 7) / 8 * 24)), dim3(NT), lds, s, cp);
+}
+
+template <int N1, int N2, int NT>
+void launch_slab_planes(const PlaneSet& ps, int nblocks, hipStream_t s) {
+  constexpr size_t lds = plane_lds<N1 * N2>();
+  static bool granted = false;  // once per process, outside any stream work
+  if (!granted) {
+    (void)hipFuncSetAttribute((const void*)k_slab_planes<N1, N2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);  // a refusal shows as the launch's error
+    granted = true;
+  }
+  hipLaunchKernelGGL((k_slab_planes<N1, N2, NT>), dim3(nblocks), dim3(NT), lds, s, ps);
+}
+
 // Launch timing of the accumulating last pass with the curl term (the largest kernel of a
 // mixed-precision coupled step), for measurement: armed for n launches, each such launch on a
 // stream that is not capturing records an event pair around itself.
@@ -694,6 +891,33 @@ int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, in
   for (int axis = 0; axis < 3; ++axis)
     if ((axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz) > 1) last = axis;
   if (last < 0) return -1;
+  // complex64 temporaries, curl fused, Ny = Nz = 64 or 128: the pass along x, then the
+  // passes along y and z with the accumulation as one 2-D transform per x-plane
+  if (cu && tmp32 && nb == 3 && dir < 0 && g->Ny == g->Nz && (g->Ny == 64 || g->Ny == 128)) {
+    if (g->Nx > 1) {
+      const cplx* ins[3] = {tmps[0], tmps[1], tmps[2]};
+      const int rc = launch_pass_b(g, ins, tmps, 3, 0, dir, nullptr, -1, s, 0, tmp32, tmp32, 0, nullptr);
+      if (rc) return rc;
+    }
+    CurlPlanes cp;
+    for (int c = 0; c < 3; ++c) {
+      cp.dF[c] = (const cflt*)tmps[c];
+      cp.F[c] = outs[c];
+      cp.G[c] = cu->G[c];
+    }
+    cp.qx = g->kx;
+    cp.qy = g->ky;
+    cp.qz = g->kz;
+    cp.ccoef = cu->coef;
+    cp.tw = g->twy;
+    cp.Nx = g->Nx;
+    const int slot = timed_slot(s);
+    if (slot >= 0) hipEventRecord(g_tev[2 * slot], s);
+    if (g->Ny == 128) launch_dF_planes<8, 16, 1024>(cp, s);
+    else launch_dF_planes<8, 8, 512>(cp, s);
+    if (slot >= 0) hipEventRecord(g_tev[2 * slot + 1], s);
+    return 0;
+  }
   for (int axis = 0; axis < 3; ++axis) {
     const int N = axis == 0 ? g->Nx : axis == 1 ? g->Ny : g->Nz;
     if (N == 1) continue;
@@ -845,121 +1069,6 @@ int launch_pass_multi(const PassSpec* sp, int ns, int dir, hipStream_t s, int ac
     default: launch_fft2_multi<32, 32>(m, ns, e, s); break;
   }
   return 0;
-}
-
-// The passes after the slab pass of the slab derivatives, as one 2-D transform per slab
-// plane in LDS (mixed precision, equal axis lengths N = 64 or 128: an N x N complex64 plane
-// fits in LDS): load the plane, N-point inverse transforms along its rows, then along its
-// columns (each the two-level N1 x N2 transform of the pass kernels), write the real parts
-// times 1/N^2.  Entry e: derivative array S[e] -> R[e], nplanes planes pstride apart, rows
-// rstride apart.  Workgroup = one plane.
-struct PlaneSet {
-  const cflt* S[6];
-  float* R[6];
-  int64_t pstride[6], rstride[6];
-  int first[7];  // first workgroup of entry e (prefix sums of the plane counts)
-  int ne;
-  const double* tw;  // exp(-2 pi i m/N), interleaved, m < N/2
-};
-
-template <int N1, int N2, int NT>
-__global__ void __launch_bounds__(NT) k_slab_planes(PlaneSet ps) {
-  constexpr int N = N1 * N2, PITCH = N + 1;  // odd pitch: a column's points on different banks
-  constexpr int R1 = N * N2 / NT, R2 = N * N1 / NT;  // level-1 / level-2 tasks per thread
-  static_assert(R1 * NT == N * N2 && R2 * NT == N * N1, "tasks must divide evenly");
-  extern __shared__ cpx<float> lds_pl[];
-  cpx<float>* buf = lds_pl;
-  cpx<float>* tw = lds_pl + N * PITCH;
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int e = b >= ps.first[5] ? 5 : b >= ps.first[4] ? 4 : b >= ps.first[3] ? 3 : b >= ps.first[2] ? 2
-              : b >= ps.first[1] ? 1 : 0;
-  if (e >= ps.ne) return;
-  const int pl = b - (e == 0 ? ps.first[0] : e == 1 ? ps.first[1] : e == 2 ? ps.first[2] : e == 3 ? ps.first[3]
-                      : e == 4 ? ps.first[4] : ps.first[5]);
-  const cflt* S = e == 0 ? ps.S[0] : e == 1 ? ps.S[1] : e == 2 ? ps.S[2] : e == 3 ? ps.S[3] : e == 4 ? ps.S[4] : ps.S[5];
-  float* R = e == 0 ? ps.R[0] : e == 1 ? ps.R[1] : e == 2 ? ps.R[2] : e == 3 ? ps.R[3] : e == 4 ? ps.R[4] : ps.R[5];
-  const int64_t pst = e == 0 ? ps.pstride[0] : e == 1 ? ps.pstride[1] : e == 2 ? ps.pstride[2]
-                    : e == 3 ? ps.pstride[3] : e == 4 ? ps.pstride[4] : ps.pstride[5];
-  const int64_t rst = e == 0 ? ps.rstride[0] : e == 1 ? ps.rstride[1] : e == 2 ? ps.rstride[2]
-                    : e == 3 ? ps.rstride[3] : e == 4 ? ps.rstride[4] : ps.rstride[5];
-  const int64_t base = (int64_t)pl * pst;
-  for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)ps.tw[2 * m], (float)-ps.tw[2 * m + 1]};  // inverse
-  __syncthreads();
-  auto twid = [&](int mm) {  // W^mm, mm < N
-    return mm < N / 2 ? tw[mm] : cpx<float>{-tw[mm - N / 2].x, -tw[mm - N / 2].y};
-  };
-  // rows, level 1, straight from memory: task (row l, n2) transforms x[n1 N2 + n2] over n1
-#pragma unroll
-  for (int q = 0; q < R1; ++q) {
-    const int task = tid + NT * q, l = task / N2, n2 = task % N2;
-    cpx<float> v[N1];
-#pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) {
-      const cflt x = S[base + (int64_t)l * rst + n1 * N2 + n2];
-      v[n1] = cpx<float>{x.x, x.y};
-    }
-    dft_reg<N1, N, float>(v, tw);
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) buf[l * PITCH + k1 * N2 + n2] = v[k1] * twid((n2 * k1) % N);
-  }
-  __syncthreads();
-  // rows, level 2: task (row l, k1) transforms y[k1 N2 + n2] over n2 -> X[k1 + N1 k2] (all
-  // reads before the writes: the two index sets of a row overlap)
-  {
-    cpx<float> u[R2][N2];
-#pragma unroll
-    for (int q = 0; q < R2; ++q) {
-      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
-#pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[l * PITCH + k1 * N2 + n2];
-      dft_reg<N2, N, float>(u[q], tw);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < R2; ++q) {
-      const int task = tid + NT * q, l = task / N1, k1 = task % N1;
-#pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) buf[l * PITCH + k1 + N1 * k2] = u[q][k2];
-    }
-  }
-  __syncthreads();
-  // columns, level 1 in place: task (column c, n2), consecutive threads on consecutive columns
-#pragma unroll
-  for (int q = 0; q < R1; ++q) {
-    const int task = tid + NT * q, c = task % N, n2 = task / N;
-    cpx<float> v[N1];
-#pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = buf[(n1 * N2 + n2) * PITCH + c];
-    dft_reg<N1, N, float>(v, tw);
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * PITCH + c] = v[k1] * twid((n2 * k1) % N);
-  }
-  __syncthreads();
-  // columns, level 2, straight to memory: the real parts times 1/N^2
-  constexpr float sc = 1.0f / (float)(N * N);
-#pragma unroll
-  for (int q = 0; q < R2; ++q) {
-    const int task = tid + NT * q, c = task % N, k1 = task / N;
-    cpx<float> u[N2];
-#pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) u[n2] = buf[(k1 * N2 + n2) * PITCH + c];
-    dft_reg<N2, N, float>(u, tw);
-#pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) R[base + (int64_t)(k1 + N1 * k2) * rst + c] = u[k2].x * sc;
-  }
-}
-
-template <int N1, int N2, int NT>
-void launch_slab_planes(const PlaneSet& ps, int nblocks, hipStream_t s) {
-  constexpr int N = N1 * N2;
-  const size_t lds = ((size_t)N * (N + 1) + N / 2) * sizeof(cpx<float>);
-  static bool granted = false;  // once per process, outside any stream work
-  if (!granted) {
-    (void)hipFuncSetAttribute((const void*)k_slab_planes<N1, N2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);  // a refusal shows as the launch's error
-    granted = true;
-  }
-  hipLaunchKernelGGL((k_slab_planes<N1, N2, NT>), dim3(nblocks), dim3(NT), lds, s, ps);
 }
 
 // The two derivatives along one axis (of two fields) share every launch: batch of 2.

@@ -504,14 +504,17 @@ def test_cpml_anisotropic_matches_oracle(shape):
 
 
 @pytest.mark.parametrize("shape,fused", [((64, 64, 128), True), ((128, 64, 1), True), ((1024, 1, 1), True),
+                                         ((64, 64, 64), True), ((128, 128, 128), True), ((256, 64, 64), True),
                                          ((64, 64, 32), False), ((2048, 1, 1), False)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_cpml_e_curl_matches_update_then_cpml(shape, fused, precision):
     """pss_cpml_e_curl (E += i v2 dt k x B added in the correction's last forward pass
     when that axis is 64 ... 1024 long) against pss_update_e3d without current followed by
     pss_cpml_half (which = 0), soft source on Ey, two half steps (psi carried): the same
-    sums in another order, FP64 rounding (1e-13 on E as a vector; mixed 1e-7, whose FP32
-    passes round differently in the two kernels).  Other lengths take the two calls."""
+    sums in another order, FP64 rounding (1e-13 on E as a vector).  Mixed: FP32 passes that
+    round differently in the two forms (with Ny = Nz = 64 or 128 the fused form transforms
+    each x-plane along z, then y, in LDS; the split form along y, then z): 3e-7, a few units
+    of FP32 roundoff (6e-8) through the three passes.  Other lengths take the two calls."""
     import torch
 
     from pulsesuite_amd import _lib
@@ -550,10 +553,11 @@ def test_cpml_e_curl_matches_update_then_cpml(shape, fused, precision):
         torch.cuda.synchronize()
         out.append(np.stack([t.cpu().numpy() for t in dE]))
     assert np.isfinite(out[1]).all()
-    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 1e-7)
+    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 3e-7)
 
 
 @pytest.mark.parametrize("shape,fused", [((64, 64, 128), True), ((128, 64, 1), True), ((1024, 1, 1), True),
+                                         ((64, 64, 64), True), ((128, 128, 128), True), ((256, 64, 64), True),
                                          ((64, 64, 32), False), ((2048, 1, 1), False)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_b_half_matches_update_then_cpml(shape, fused, precision):
@@ -563,7 +567,9 @@ def test_b_half_matches_update_then_cpml(shape, fused, precision):
     FP64 rounding (1e-13 on B as a vector).  Mixed: the correction's last pass is an FP32
     transform in both, but in two different kernels (plain and curl-adding), whose FP32
     butterflies the compiler may contract differently: FP32 rounding of the correction,
-    1e-7 on B.  The non-fused lengths take the two calls."""
+    3e-7 on B (with Ny = Nz = 64 or 128 the fused form transforms each x-plane along z,
+    then y, in LDS; the split form along y, then z).  The non-fused lengths take the two
+    calls."""
     import torch
 
     from pulsesuite_amd import _lib
@@ -599,7 +605,7 @@ def test_b_half_matches_update_then_cpml(shape, fused, precision):
         torch.cuda.synchronize()
         out.append(np.stack([t.cpu().numpy() for t in dB]))
     assert np.isfinite(out[1]).all()
-    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 1e-7)
+    assert rel(out[1], out[0]) < (1e-13 if precision == "fp64" else 3e-7)
     if not fused:  # the same two calls: bit-identical
         assert np.array_equal(out[0], out[1])
 
