@@ -209,23 +209,31 @@ def precision_gate(grid, nk, npml, current="reference", closed_steps=GATE_CLOSED
     return out
 
 
-# The dominant kernel of the timed mixed step: k_fft2<8,16,3,float>, the accumulating last
-# (z) pass of the CPML correction's dF transform with the curl term of the field update
-# fused in (pss_b_half and pss_cpml_e_curl each launch it once per step).  Algorithmic bytes
-# per launch and cell: 3 x 8 B complex64 dF read, 3 x 16 B field read, 3 x 16 B field
-# written, 3 x 16 B of the other field read once (the curl operands): 168 B.
+# The dominant kernel of the timed mixed step at 128^3: k_dF_planes<8,16,1024>, the last two
+# forward passes (y, z) of the CPML correction's dF transform as one 2-D LDS transform per
+# x-plane, accumulating into the field with the curl term of the field update fused in
+# (pss_b_half and pss_cpml_e_curl each launch it once per step); on other grids the
+# accumulating last pass k_fft2<...,3,float>.  Algorithmic bytes per launch and cell, both:
+# 3 x 8 B complex64 dF read, 3 x 16 B field read, 3 x 16 B field written, 3 x 16 B of the
+# other field read once (the curl operands): 168 B.
 DOM_BYTES_PER_CELL = 168
 # HBM bytes per launch of that kernel at 128^3, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `bench.py --workloads traffic`, each counter scaled by what it
-# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 124423.7 KiB x 2 read +
+# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 127650.8 KiB x 2 read +
 # 98304 KiB written (profiles/dominant_pass_128cube_traffic.md).
-DOM_TRAFFIC_128 = (124423.7 * 2.0 + 98304.0) * 1024.0
+DOM_TRAFFIC_128 = (127650.8 * 2.0 + 98304.0) * 1024.0
+
+
+def planes_path(shape):
+    """The mixed correction's y/z passes run as plane transforms (Ny = Nz = 64 or 128)."""
+    return shape[1] == shape[2] and shape[1] in (64, 128)
 
 
 def dominant_pass_timing(prop, reps=50):
     """Average duration of one launch of the dominant kernel on this grid (HIP events on
-    the stream it is launched on): pss_fft_pass_acc on complex64 dF operands accumulating
-    into copies of the field, with the curl term of the other field."""
+    the stream it is launched on): pss_fft_planes_acc (Ny = Nz = 64 or 128) or
+    pss_fft_pass_acc on complex64 dF operands accumulating into copies of the field, with
+    the curl term of the other field."""
     import ctypes as C
 
     import torch
@@ -245,9 +253,15 @@ def dominant_pass_timing(prop, reps=50):
     coef = prop.v2 * prop.time.dt
     s = torch.cuda.current_stream()
 
+    planes = planes_path(shape)
+
     def launch():
-        _lib.check(_lib.lib.pss_fft_pass_acc(prop.ctx.gp, ins, outs, 3, last, -1, 1, *(_lib.ptr(t) for t in G), coef,
-                                             _lib.stream_ptr(s)), "pss_fft_pass_acc")
+        if planes:
+            _lib.check(_lib.lib.pss_fft_planes_acc(prop.ctx.gp, ins, outs, *(_lib.ptr(t) for t in G), coef,
+                                                   _lib.stream_ptr(s)), "pss_fft_planes_acc")
+        else:
+            _lib.check(_lib.lib.pss_fft_pass_acc(prop.ctx.gp, ins, outs, 3, last, -1, 1, *(_lib.ptr(t) for t in G),
+                                                 coef, _lib.stream_ptr(s)), "pss_fft_pass_acc")
 
     for _ in range(5):
         launch()
@@ -427,8 +441,8 @@ def traffic_kernels(grid, reps=3):
     """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs: `reps` plain device
     copies of 512 MiB (the counters' calibration: known bytes read and written), then
     `reps` FP64 3-D FFTs of one grid^3 field (3 k_fft2 passes each), then `reps` launches
-    of the dominant kernel of the mixed step (pss_fft_pass_acc: complex64 dF accumulated
-    into a complex128 field with the curl term, last axis)."""
+    of the dominant kernel of the mixed step (pss_fft_planes_acc, or pss_fft_pass_acc on
+    the last axis: complex64 dF accumulated into a complex128 field with the curl term)."""
     import ctypes as C
 
     import torch
@@ -451,8 +465,12 @@ def traffic_kernels(grid, reps=3):
     ins = (C.c_void_p * 3)(*(t.data_ptr() for t in dF))
     outs = (C.c_void_p * 3)(*(t.data_ptr() for t in E))
     for _ in range(reps):
-        _lib.check(_lib.lib.pss_fft_pass_acc(ctx.gp, ins, outs, 3, 2, -1, 1, *(_lib.ptr(t) for t in B), 1e-3,
-                                             _lib.stream_ptr()), "pass_acc")
+        if planes_path((grid, grid, grid)):
+            _lib.check(_lib.lib.pss_fft_planes_acc(ctx.gp, ins, outs, *(_lib.ptr(t) for t in B), 1e-3,
+                                                   _lib.stream_ptr()), "planes_acc")
+        else:
+            _lib.check(_lib.lib.pss_fft_pass_acc(ctx.gp, ins, outs, 3, 2, -1, 1, *(_lib.ptr(t) for t in B), 1e-3,
+                                                 _lib.stream_ptr()), "pass_acc")
     torch.cuda.synchronize()
     print(json.dumps({"traffic_kernels": {"copy_bytes": src.numel() * 8, "fft_grid": grid, "reps": reps}}))
 
@@ -618,7 +636,9 @@ def main():
             us, where = us_alone, "alone, HIP events on its stream"
         ach = b / (us * 1e-6)
         line["roofline"] = {"bound": "hbm",
-                            "kernel": f"k_fft2<8,16,3,float>: last CPML dF pass + fused curl update, {a.grid}^3",
+                            "kernel": (f"k_dF_planes: y/z CPML dF passes + fused curl update, {a.grid}^3"
+                                       if planes_path((a.grid,) * 3) else
+                                       f"k_fft2<...,3,float>: last CPML dF pass + fused curl update, {a.grid}^3"),
                             "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
                             "traffic": DOM_TRAFFIC_128 if a.grid == 128 else None,
                             "algorithmic_bytes": b, "avg_us": us, "timed": where, "launches_timed": nl,

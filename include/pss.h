@@ -346,6 +346,15 @@ int pss_fft_line_batch(const pss_grid* g, double* const* in, double* const* out,
  * for tests and for timing that pass alone. */
 int pss_fft_pass_acc(const pss_grid* g, double* const* in, double* const* out, int32_t nb, int32_t axis, int32_t dir,
                      int32_t in32, const double* Gx, const double* Gy, const double* Gz, double coef, void* stream);
+/* The last two forward passes (along y and z) of the CPML correction's dF transform with
+ * the accumulation and the curl term, as the mixed-precision correction runs them when
+ * Ny = Nz = 64 or 128: one 2-D transform per x-plane in LDS.  in: three complex64 arrays
+ * (dF after its pass along x), out: complex128 fields, out_c += FFT_yz(in_c) + coef i (k x G)_c
+ * (PSTD3D/cpml.py:575-702 and the curl term of PSTD3D/PSTD3D.py:29-113).  A kernel-level
+ * entry for tests and for timing that kernel alone; no counterpart entry point in the
+ * reference. */
+int pss_fft_planes_acc(const pss_grid* g, const double* const* in, double* const* out, const double* Gx,
+                       const double* Gy, const double* Gz, double coef, void* stream);
 
 /* Fill three uniform wire lines [3][n] with the point sample (complex [3]): the
  * wire sees E at one point (QWArray.QuantumWire, PSTD3D/rhoPJ.py:1160-1164). */

@@ -100,6 +100,7 @@ def _load():
         "pss_fill_line": (C.c_int, [_vp, _vp, _i, _vp]),
         "pss_mask_half": (C.c_int, [C.POINTER(Grid), _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_d), _vp, _vp, _vp,
                                     _vp, _vp]),
+        "pss_fft_planes_acc": (C.c_int, [C.POINTER(Grid), C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, _vp, _d, _vp]),
         "pss_fft_pass_acc": (C.c_int, [C.POINTER(Grid), C.POINTER(_vp), C.POINTER(_vp), _i, _i, _i, _i, _vp, _vp,
                                        _vp, _d, _vp]),
         "pss_jline": (C.c_int, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _i, _vp]),
@@ -130,7 +131,7 @@ lib = _load()
 
 EXPORTS = ("pss_sbe_workspace_bytes", "pss_sbe_tune", "pss_sbe_step", "pss_sbe_step_p", "pss_sbe_stages", "pss_calc_h", "pss_sbe_rhs", "pss_coll_rates",
            "pss_qwpol", "pss_rhs_dense", "pss_sbe_get_state", "pss_sbe_set_state", "pss_vint", "pss_update_e3d", "pss_update_e3d_sample", "pss_update_b3d",
-           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_fft_pass_acc", "pss_ksample", "pss_ksample_noalloc", "pss_pass_timing_arm", "pss_pass_timing_read", "pss_b_half", "pss_cpml_e_curl", "pss_cpml_half", "pss_mask_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_elong_k", "pss_qw_place", "pss_kfilter", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
+           "pss_fft3d", "pss_fft_line", "pss_fft_line_batch", "pss_fft_pass_acc", "pss_fft_planes_acc", "pss_ksample", "pss_ksample_noalloc", "pss_pass_timing_arm", "pss_pass_timing_read", "pss_b_half", "pss_cpml_e_curl", "pss_cpml_half", "pss_mask_half", "pss_cpml_psi_len", "pss_fill_line", "pss_jline", "pss_elong_k", "pss_qw_place", "pss_kfilter", "pss_rk4_stage", "pss_drive_uniform", "pss_add_scalar",
            "pss_last_error", "pss_version")
 
 

@@ -1899,6 +1899,31 @@ extern "C" int pss_fft_pass_acc(const pss_grid* g, double* const* in, double* co
   return 0;
 }
 
+extern "C" int pss_fft_planes_acc(const pss_grid* g, const double* const* in, double* const* out, const double* Gx,
+                                  const double* Gy, const double* Gz, double coef, void* stream) {
+  if (!(g->Ny == g->Nz && (g->Ny == 64 || g->Ny == 128)))
+    return pss_set_msg(2, "pss_fft_planes_acc: needs Ny = Nz = 64 or 128");
+  if (!in || !out || !in[0] || !in[1] || !in[2] || !out[0] || !out[1] || !out[2] || !Gx || !Gy || !Gz)
+    return pss_set_msg(2, "pss_fft_planes_acc: null array");
+  CurlPlanes cp;
+  const cplx* G[3] = {(const cplx*)Gx, (const cplx*)Gy, (const cplx*)Gz};
+  for (int c = 0; c < 3; ++c) {
+    cp.dF[c] = (const cflt*)in[c];
+    cp.F[c] = (cplx*)out[c];
+    cp.G[c] = G[c];
+  }
+  cp.qx = g->kx;
+  cp.qy = g->ky;
+  cp.qz = g->kz;
+  cp.ccoef = coef;
+  cp.tw = g->twy;
+  cp.Nx = g->Nx;
+  if (g->Ny == 128) launch_dF_planes<8, 16, 1024>(cp, (hipStream_t)stream);
+  else launch_dF_planes<8, 8, 512>(cp, (hipStream_t)stream);
+  PSS_CHECK(hipGetLastError());
+  return 0;
+}
+
 // cu: the curl update of F from G, fused into the last forward pass (null: none)
 static int cpml_half_impl(const pss_grid* g, int32_t which, double* Fx, double* Fy, double* Fz, const double* Gx,
                    const double* Gy, const double* Gz, double* psi, double* scratch, int32_t src_comp,
