@@ -349,7 +349,9 @@ int pss_fft_pass_acc(const pss_grid* g, double* const* in, double* const* out, i
 /* The last two forward passes (along y and z) of the CPML correction's dF transform with
  * the accumulation and the curl term, as the mixed-precision correction runs them when
  * Ny = Nz = 64 or 128: one 2-D transform per x-plane in LDS.  in: three complex64 arrays
- * (dF after its pass along x), out: complex128 fields, out_c += FFT_yz(in_c) + coef i (k x G)_c
+ * (dF after its forward pass along x; dF is real, so only the x-planes 0 ... Nx/2 are read
+ * and plane Nx - g of the result is taken as conj of plane g with (ky, kz) reversed), out:
+ * complex128 fields, out_c += FFT_yz(in_c) + coef i (k x G)_c
  * (PSTD3D/cpml.py:575-702 and the curl term of PSTD3D/PSTD3D.py:29-113).  A kernel-level
  * entry for tests and for timing that kernel alone; no counterpart entry point in the
  * reference. */

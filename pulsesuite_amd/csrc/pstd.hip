@@ -123,6 +123,8 @@ struct PassArgs {
   int slab;   // > 0 (two-level passes only): write only positions k < slab or k >= N - slab along the
               // axis, into an array whose axis length is 2 slab (k -> k, or k - (N - 2 slab))
   int kmajor; // slab pass along z: the array is (2 slab, Nx, Ny), slab position outermost
+  int half;   // store only the outputs k <= N / 2 (a real field's forward pass whose other half
+              // the next stage takes from the conjugate pairs)
   // accum 3 (two-level passes, complex128 out): out_c += result + ccoef i (q x G)_c, the
   // spectral curl update of the other field G fused into the accumulating last pass
   // (batch element c = component); q from the grid's wavenumber tables
@@ -412,6 +414,7 @@ __device__ __forceinline__ void fft2_body(const PassArgs a, const int bz, const 
           if (k >= a.slab && k < N - a.slab) continue;
           kk = k < a.slab ? k : k - (N - P2);
         }
+        if (MODE == 0 && a.half && k > N / 2) continue;
         const int64_t o = obase0 + (int64_t)t * ost + (int64_t)kk * osa;
         if (MODE == 3) {
           int ix, iy, iz;
@@ -617,6 +620,7 @@ PassArgs make_pass_args(const pss_grid* g, const cplx* const* ins, cplx* const* 
   a.out32 = out32;
   a.slab = slab;
   a.kmajor = 0;
+  a.half = 0;
   if (cu) {
     a.cg0 = cu->G[0]; a.cg1 = cu->G[1]; a.cg2 = cu->G[2];
     a.qx = g->kx; a.qy = g->ky; a.qz = g->kz;
@@ -675,7 +679,9 @@ int fft3(const pss_grid* g, const cplx* in, cplx* out, int dir, const double* km
 // the two-level N1 x N2 transform of the pass kernels.  The rows' first level takes its
 // points from load(r, c) and the columns' second level hands the results to
 // store(r, c, X), so the plane makes 6 LDS sweeps.  NT threads; ends without a barrier.
-template <int N1, int N2, int NT, class Load, class Store>
+// SYNC: store may write the plane buffer (every read of the last level precedes a barrier
+// before the first store).
+template <int N1, int N2, int NT, bool SYNC = false, class Load, class Store>
 __device__ __forceinline__ void plane_fft2d(cpx<float>* buf, const cpx<float>* tw, Load load, Store store) {
   constexpr int N = N1 * N2, PITCH = N + 1;
   constexpr int R1 = N * N2 / NT, R2 = N * N1 / NT;  // level-1 / level-2 tasks per thread
@@ -729,6 +735,24 @@ __device__ __forceinline__ void plane_fft2d(cpx<float>* buf, const cpx<float>* t
   }
   __syncthreads();
   // columns, level 2: results to store
+  if (SYNC) {
+    cpx<float> u[R2][N2];
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q, c = task % N, k1 = task / N;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) u[q][n2] = buf[(k1 * N2 + n2) * PITCH + c];
+      dft_reg<N2, N, float>(u[q], tw);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int task = tid + NT * q, c = task % N, k1 = task / N;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) store(k1 + N1 * k2, c, u[q][k2]);
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < R2; ++q) {
     const int task = tid + NT * q, c = task % N, k1 = task / N;
@@ -794,9 +818,11 @@ __global__ void __launch_bounds__(NT) k_slab_planes(PlaneSet ps) {
 // LDS (mixed precision, Ny = Nz = N = 64 or 128), with the accumulation into the field
 // and the spectral curl update of the other field fused in as the accumulating last pass
 // does it (F_c += FFT(dF_c) + ccoef i (q x G)_c, term for term).  dF[c] holds the x pass's
-// output (complex64).  Workgroup 24 (g / 8) + 8 c + g % 8 takes component c of x-plane g,
-// so the three components of a plane go to one XCD back to back and the other field's
-// planes that two components' curl terms both read are still in that XCD's L2.
+// output (complex64) of a real field, so x-planes come in conjugate pairs:
+// X(Nx - g, ky, kz) = conj X(g, -ky, -kz).  A workgroup transforms plane g <= Nx / 2 and
+// updates planes g and Nx - g.  Workgroup 24 (g / 8) + 8 c + g % 8 takes component c of
+// plane g, so the three components of a plane go to one XCD back to back and the other
+// field's planes that two components' curl terms both read are still in that XCD's L2.
 struct CurlPlanes {
   const cflt* dF[3];
   cplx* F[3];
@@ -809,13 +835,14 @@ struct CurlPlanes {
 
 template <int N1, int N2, int NT>
 __global__ void __launch_bounds__(NT) k_dF_planes(CurlPlanes cp) {
-  constexpr int N = N1 * N2;
+  constexpr int N = N1 * N2, PITCH = N + 1;
   extern __shared__ cpx<float> lds_pl[];
   cpx<float>* buf = lds_pl;
-  cpx<float>* tw = lds_pl + N * (N + 1);
+  cpx<float>* tw = lds_pl + N * PITCH;
   const int lin = blockIdx.x, grp = lin / 24, r8 = lin - 24 * grp;
   const int c = r8 >> 3, gx = grp * 8 + (r8 & 7);
-  if (gx >= cp.Nx) return;
+  const int Nx = cp.Nx;
+  if (gx > Nx / 2) return;
   const int tid = threadIdx.x;
   const cflt* dF = c == 0 ? cp.dF[0] : c == 1 ? cp.dF[1] : cp.dF[2];
   cplx* F = c == 0 ? cp.F[0] : c == 1 ? cp.F[1] : cp.F[2];
@@ -823,23 +850,35 @@ __global__ void __launch_bounds__(NT) k_dF_planes(CurlPlanes cp) {
   const cplx* ga = c == 0 ? cp.G[1] : c == 1 ? cp.G[2] : cp.G[0];
   const cplx* gb = c == 0 ? cp.G[2] : c == 1 ? cp.G[0] : cp.G[1];
   const int64_t base = (int64_t)gx * N * N;
-  const double q0 = cp.qx[gx];
   for (int m = tid; m < N / 2; m += NT) tw[m] = cpx<float>{(float)cp.tw[2 * m], (float)cp.tw[2 * m + 1]};  // forward
   __syncthreads();
-  plane_fft2d<N1, N2, NT>(
+  plane_fft2d<N1, N2, NT, true>(
       buf, tw,
       [&](int r, int col) {  // row r = y, column = z
         const cflt x = dF[base + (int64_t)r * N + col];
         return cpx<float>{x.x, x.y};
       },
-      [&](int r, int col, cpx<float> X) {
-        const int64_t o = base + (int64_t)r * N + col;
-        const double q1 = cp.qy[r], q2 = cp.qz[col];
-        const double qa = c == 0 ? q1 : c == 1 ? q2 : q0, qb = c == 0 ? q2 : c == 1 ? q0 : q1;
-        const cplx cr = qa * gb[o] - qb * ga[o];
-        const cpx<float> v = X * 1.0f;  // forward: scale 1
-        F[o] = F[o] + (mk((double)v.x, (double)v.y) + mk(-cr.y, cr.x) * cp.ccoef);
-      });
+      [&](int r, int col, cpx<float> X) { buf[r * PITCH + col] = X; });
+  __syncthreads();
+  // the field updates of plane gx and, when it is not its own partner, plane Nx - gx
+  const int gp = (Nx - gx) & (Nx - 1);
+  const bool pair = gp != gx;
+  auto upd = [&](int g, int r, int col, cpx<float> X) {
+    const int64_t o = ((int64_t)g * N + r) * N + col;
+    const double q0 = cp.qx[g], q1 = cp.qy[r], q2 = cp.qz[col];
+    const double qa = c == 0 ? q1 : c == 1 ? q2 : q0, qb = c == 0 ? q2 : c == 1 ? q0 : q1;
+    const cplx cr = qa * gb[o] - qb * ga[o];
+    F[o] = F[o] + (mk((double)X.x, (double)X.y) + mk(-cr.y, cr.x) * cp.ccoef);
+  };
+#pragma unroll 4
+  for (int e = tid; e < N * N; e += NT) {
+    const int r = e / N, col = e % N;
+    upd(gx, r, col, buf[r * PITCH + col]);
+    if (pair) {
+      const cpx<float> Y = buf[((N - r) & (N - 1)) * PITCH + ((N - col) & (N - 1))];
+      upd(gp, r, col, cpx<float>{Y.x, -Y.y});
+    }
+  }
 }
 
 template <int N1, int N2, int NT>
@@ -851,7 +890,7 @@ void launch_dF_planes(const CurlPlanes& cp, hipStream_t s) {
                               (int)lds);  // a refusal shows as the launch's error
     granted = true;
   }
-  hipLaunchKernelGGL((k_dF_planes<N1, N2, NT>), dim3((unsigned)((cp.Nx + 7) / 8 * 24)), dim3(NT), lds, s, cp);
+  hipLaunchKernelGGL((k_dF_planes<N1, N2, NT>), dim3((unsigned)((cp.Nx / 2 + 1 + 7) / 8 * 24)), dim3(NT), lds, s, cp);
 }
 
 template <int N1, int N2, int NT>
@@ -894,10 +933,19 @@ int fft3_acc(const pss_grid* g, cplx* const* tmps, cplx* const* outs, int nb, in
   // complex64 temporaries, curl fused, Ny = Nz = 64 or 128: the pass along x, then the
   // passes along y and z with the accumulation as one 2-D transform per x-plane
   if (cu && tmp32 && nb == 3 && dir < 0 && g->Ny == g->Nz && (g->Ny == 64 || g->Ny == 128)) {
-    if (g->Nx > 1) {
+    if (g->Nx > 1) {  // only the x-planes 0 ... Nx/2 are read (conjugate pairs)
       const cplx* ins[3] = {tmps[0], tmps[1], tmps[2]};
-      const int rc = launch_pass_b(g, ins, tmps, 3, 0, dir, nullptr, -1, s, 0, tmp32, tmp32, 0, nullptr);
-      if (rc) return rc;
+      const int two = g->Nx == 64 || g->Nx == 128 || g->Nx == 256 || g->Nx == 512 || g->Nx == 1024;
+      if (!two) return -3;  // complex64 temporaries exist for two-level lengths only
+      PassArgs a = make_pass_args(g, ins, tmps, 3, 0, dir, nullptr, -1, 0, tmp32, tmp32, 0, nullptr);
+      a.half = 1;
+      switch (g->Nx) {
+        case 64: launch_fft2<8, 8>(a, g, s, 3); break;
+        case 128: launch_fft2<8, 16>(a, g, s, 3); break;
+        case 256: launch_fft2<16, 16>(a, g, s, 3); break;
+        case 512: launch_fft2<16, 32>(a, g, s, 3); break;
+        default: launch_fft2<32, 32>(a, g, s, 3); break;
+      }
     }
     CurlPlanes cp;
     for (int c = 0; c < 3; ++c) {
