@@ -213,15 +213,24 @@ def precision_gate(grid, nk, npml, current="reference", closed_steps=GATE_CLOSED
 # forward passes (y, z) of the CPML correction's dF transform as one 2-D LDS transform per
 # x-plane, accumulating into the field with the curl term of the field update fused in
 # (pss_b_half and pss_cpml_e_curl each launch it once per step); on other grids the
-# accumulating last pass k_fft2<...,3,float>.  Algorithmic bytes per launch and cell, both:
+# accumulating last pass k_fft2<...,3,float>.  Algorithmic bytes per launch and cell:
 # 3 x 8 B complex64 dF read, 3 x 16 B field read, 3 x 16 B field written, 3 x 16 B of the
-# other field read once (the curl operands): 168 B.
+# other field read once (the curl operands): 168 B; the plane kernel reads dF of the
+# x-planes 0 ... Nx/2 only (the others are their conjugate partners): 3 x 8 B x (Nx/2 + 1)/Nx.
 DOM_BYTES_PER_CELL = 168
+
+
+def dom_bytes(shape):
+    """Algorithmic bytes of one launch of the dominant kernel on a grid of this shape."""
+    n = shape[0] * shape[1] * shape[2]
+    if planes_path(shape):
+        return int(round(24 * (shape[0] // 2 + 1) / shape[0] * n)) + 144 * n
+    return DOM_BYTES_PER_CELL * n
 # HBM bytes per launch of that kernel at 128^3, from `rocprofv3 --pmc FETCH_SIZE` and
 # `--pmc WRITE_SIZE` runs of `bench.py --workloads traffic`, each counter scaled by what it
-# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 127650.8 KiB x 2 read +
+# reads for a 512 MiB device copy (FETCH_SIZE x 2, WRITE_SIZE x 1): 112287.4 KiB x 2 read +
 # 98304 KiB written (profiles/dominant_pass_128cube_traffic.md).
-DOM_TRAFFIC_128 = (127650.8 * 2.0 + 98304.0) * 1024.0
+DOM_TRAFFIC_128 = (112287.4 * 2.0 + 98304.0) * 1024.0
 
 
 def planes_path(shape):
@@ -272,7 +281,7 @@ def dominant_pass_timing(prop, reps=50):
     ev1.record(s)
     torch.cuda.synchronize()
     us = 1e3 * ev0.elapsed_time(ev1) / reps
-    return us, DOM_BYTES_PER_CELL * n
+    return us, dom_bytes(shape)
 
 
 def in_step_pass_timing(prop, steps=12):
