@@ -13,13 +13,25 @@ Pinning (details in the tests):
   _K03_jit and Vint                                   pinned by limits (x->0 leading
                                                       term, asymptotic branch, cutoffs);
                                                       series branch value unpinned
-  dpdt / dCdt / dDdt / CalcH / MBCE / MBCH / MBPE / MBPH / CalcGammaE/H /
-  OffDiagDephasing2 / QWPolarization3 / QWRho5        loop transcription + invariants
-                                                      (Hermiticity, W>=0, zero/full
-                                                      population); values unpinned by
-                                                      any reference fixture
-  whole step (_sbe_calculator, Prop2QW, QW2Prop)      invariants + OBE special case;
-                                                      parity unpinned
+  CalcH                                               pinned (closed forms: one-entry
+                                                      exchange and direct terms)
+  dpdt                                                pinned (free carriers: the band
+                                                      frequency), + loop transcription
+  QWPolarization3 / QWRho5 / Prop2QW                  pinned (one entry = one plane
+                                                      wave; diagonal coherences = the
+                                                      window profile; a cubic's windowed
+                                                      real part)
+  MBCE / MBCH / MBPE / MBPH / CalcGammaE/H /
+  OffDiagDephasing2                                   pinned (each term vanishes at zero
+                                                      and full occupation), + loop
+                                                      transcription, W >= 0
+  dCdt / dDdt                                         loop transcription + Hermiticity
+                                                      (no closed form beyond the shared
+                                                      free-carrier case)
+  whole step (_sbe_calculator)                        free-carrier convergence to the
+                                                      closed form, OBE special case,
+                                                      RK4 order; otherwise build-defined
+                                                      composition of the pinned parts
 """
 
 from dataclasses import dataclass, field
