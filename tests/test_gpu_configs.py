@@ -121,6 +121,20 @@ def _mixed_vs_fp64(name, steps, feedback, boundary="cpml"):
     return errs, driven
 
 
+def test_c2_grid128_mixed_vs_fp64():
+    """C2 in mixed precision (the headline form: at 128^3 the CPML derivatives' later passes
+    and dF's y/z passes with the field update run as 2-D plane transforms in LDS, dF's x pass
+    on the conjugate-pair half) against the FP64 device run, wire on the source plane, every
+    output within 1e-6: after 50 steps with the current fed back and after 200 driven steps
+    with the current not fed back."""
+    for steps, feedback in ((50, True), (200, False)):
+        errs, driven = _mixed_vs_fp64("C2", steps, feedback)
+        print(f"C2 mixed vs fp64, {steps} steps, feedback {feedback}: "
+              + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        assert driven > 1e-20
+        assert max(errs.values()) <= 1e-6, (steps, feedback, errs)
+
+
 def test_c3_grid256_nk1024_fp64_then_mixed():
     """C3: 256^3 grid with an Nk = 1024 wire on the source plane: two FP64 steps against the
     oracle; then the mixed form against the FP64 device run, every output within 1e-6:

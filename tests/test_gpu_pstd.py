@@ -461,7 +461,7 @@ def test_cpml_128_matches_oracle():
     B = [1e-8 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))) for _ in range(3)]
     dE, dB = [cu(a) for a in E], [cu(a) for a in B]
     sp = OP.Space(3, N, N, N, dx, dx, dx, epsr)
-    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
+    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=npml)
     kx, ky, kz = OP.GetKxArray(sp), OP.GetKyArray(sp), OP.GetKzArray(sp)
     v2 = 2.99792458e8**2 / epsr
     ocp.ApplyCPML_E(*E, *B, kx, ky, kz, v2, dt)
@@ -610,26 +610,28 @@ def test_b_half_matches_update_then_cpml(shape, fused, precision):
         assert np.array_equal(out[0], out[1])
 
 
-def test_cpml_mixed_precision_derivatives():
+@pytest.mark.parametrize("N,npml", [(128, 12), (64, 8)])
+def test_cpml_mixed_precision_derivatives(N, npml):
     """Mixed precision: the six spectral derivatives of a CPML half step in FP32 (real
-    parts, used only in PML cells), fields FP64.  Bar 1e-6 against the FP64 oracle on
-    one E and one B half step at 128^3 with 12-cell layers."""
+    parts, used only in PML cells; at 64 and 128 points per axis the passes after the slab
+    pass run as 2-D plane transforms in LDS), fields FP64.  Bar 1e-6 against the FP64
+    oracle on one E and one B half step."""
     from pulsesuite_amd import _lib
     from pulsesuite_amd.cpml import CPML
 
-    N, dx, epsr = 128, 20e-9, 9.1
+    dx, epsr = 20e-9, 9.1
     dt = 0.9 * dx / (2.99792458e8 / np.sqrt(epsr) * np.sqrt(3.0))
     rng = np.random.default_rng(6)
     E = [rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)) for _ in range(3)]
     B = [1e-8 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))) for _ in range(3)]
     dE, dB = [cu(a) for a in E], [cu(a) for a in B]
     sp = OP.Space(3, N, N, N, dx, dx, dx, epsr)
-    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
+    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=npml)
     kx, ky, kz = OP.GetKxArray(sp), OP.GetKyArray(sp), OP.GetKzArray(sp)
     v2 = 2.99792458e8**2 / epsr
     ocp.ApplyCPML_E(*E, *B, kx, ky, kz, v2, dt)
     ocp.ApplyCPML_B(*B, *E, kx, ky, kz, dt)
-    cp = CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
+    cp = CPML(N, N, N, dx, dx, dx, dt, epsr, npml=npml)
     cp.ctx.g.precision = _lib.MIXED
     cp.ApplyCPML_E(*dE, *dB, None, None, None, v2, dt)
     cp.ApplyCPML_B(*dB, *dE, None, None, None, dt)
