@@ -461,7 +461,7 @@ def test_cpml_128_matches_oracle():
     B = [1e-8 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))) for _ in range(3)]
     dE, dB = [cu(a) for a in E], [cu(a) for a in B]
     sp = OP.Space(3, N, N, N, dx, dx, dx, epsr)
-    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=npml)
+    ocp = OP.CPML(N, N, N, dx, dx, dx, dt, epsr, npml=12)
     kx, ky, kz = OP.GetKxArray(sp), OP.GetKyArray(sp), OP.GetKzArray(sp)
     v2 = 2.99792458e8**2 / epsr
     ocp.ApplyCPML_E(*E, *B, kx, ky, kz, v2, dt)
