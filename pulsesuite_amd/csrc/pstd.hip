@@ -1951,6 +1951,8 @@ extern "C" int pss_fft_planes_acc(const pss_grid* g, const double* const* in, do
                                   const double* Gy, const double* Gz, double coef, void* stream) {
   if (!(g->Ny == g->Nz && (g->Ny == 64 || g->Ny == 128)))
     return pss_set_msg(2, "pss_fft_planes_acc: needs Ny = Nz = 64 or 128");
+  if (g->Nx < 1 || (g->Nx & (g->Nx - 1)))
+    return pss_set_msg(2, "pss_fft_planes_acc: Nx must be a power of two (conjugate x-planes Nx - g)");
   if (!in || !out || !in[0] || !in[1] || !in[2] || !out[0] || !out[1] || !out[2] || !Gx || !Gy || !Gz)
     return pss_set_msg(2, "pss_fft_planes_acc: null array");
   CurlPlanes cp;

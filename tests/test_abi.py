@@ -123,3 +123,18 @@ def test_pipelined_bookkeeping_over_multi_step_replays():
         PSTD3DPropagator._book_steps(fake, k)
     assert fake._nstep == 15 and fake.time.n == 15 and fake.time.t == 7.5
     assert checks == [4, 8, 9, 14, 15]  # replays that crossed 3, 6, 9, 12, 15
+
+
+def test_fft_planes_acc_refuses_bad_shapes():
+    """pss_fft_planes_acc needs Ny = Nz = 64 or 128 and a power-of-two Nx (its x-planes come
+    in conjugate pairs g, Nx - g): other shapes are refused before any device work."""
+    import ctypes as C
+
+    from pulsesuite_amd import _lib
+
+    nul = (C.c_void_p * 3)()
+    for nx, ny, nz, msg in ((96, 128, 128, b"power of two"), (128, 64, 128, b"Ny = Nz"), (128, 256, 256, b"Ny = Nz")):
+        g = _lib.Grid()
+        g.Nx, g.Ny, g.Nz = nx, ny, nz
+        assert _lib.lib.pss_fft_planes_acc(C.byref(g), nul, nul, None, None, None, 1.0, None) != 0
+        assert msg in _lib.lib.pss_last_error()
